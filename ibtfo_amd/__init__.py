@@ -1,0 +1,226 @@
+"""Python front end of the MI355X-native IBTFO Euler update (ctypes over the C ABI).
+
+The compute path is libibtfo_amd.so (hand-written gfx950 HIP kernels behind
+include/ibtfo.h). This module only marshals arguments; there is no Python or
+CPU fallback: if the library is missing or no GPU is present, calls fail.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libibtfo_amd.so")
+RUN_PATH = os.path.join(HERE, "ibtfo_run")
+
+SCHEME = {"Lax": 0, "MUSCL": 1, "MHM": 2, "Godunov": 3}
+RIEMANN = {"HLL": 0, "HLLC": 1, "PVRS": 2, "StegerWarming": 3}
+LIMITER = {"minmod": 0, "VanLeer": 1, "VanAlbada": 2}
+BOUNDARY = {"Regular": 0, "IDWBaseGFM": 1, "LinearBaseGFM": 2, "RiemannBaseGFM": 3, "BCGFM": 4, "HBCGFM": 5}
+TIME = {"SingleStep": 0, "RK2": 1}
+EDGE = {"reference": 0, "periodic": 1, "inflow": 2}
+PRECISION = {"FP64": 0, "mixed": 1}
+LEVELSET = {"none": 0, "channel": 1, "annulus": 2, "field": 3, "ellipse": 4}
+INIT = {"sod_rows": 0, "incline": 1, "vortex": 2, "sod_x": 3, "uniform": 4, "shu": 5}
+
+OK, E_ARG, E_HIP, E_NOMEM, E_IO, E_STATE, E_NONFINITE = range(7)
+
+
+class Config(C.Structure):
+    _fields_ = [("N_x", C.c_int), ("N_y", C.c_int), ("num_ghost_cell", C.c_int), ("max_iter", C.c_int),
+                ("lo_x", C.c_double), ("lo_y", C.c_double), ("hi_x", C.c_double), ("hi_y", C.c_double),
+                ("Psy_time", C.c_double), ("CFL_number", C.c_double), ("gamma", C.c_double),
+                ("scheme", C.c_int), ("riemann", C.c_int), ("limiter", C.c_int), ("boundary", C.c_int),
+                ("time", C.c_int), ("edge", C.c_int), ("precision", C.c_int), ("quirks", C.c_int),
+                ("edge_state", C.c_double * 4)]
+
+
+class IbtfoError(RuntimeError):
+    def __init__(self, rc, what, msg):
+        super().__init__("%s failed (%d): %s" % (what, rc, msg))
+        self.rc = rc
+
+
+_lib = None
+_D = C.POINTER(C.c_double)
+
+
+def lib():
+    """Load libibtfo_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libibtfo_amd.so is not built; run __graft_entry__.build() or make -C ibtfo_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        L.ibtfo_last_error.restype = C.c_char_p
+        L.ibtfo_footprint.restype = C.c_longlong
+        L.ibtfo_footprint.argtypes = [C.POINTER(Config)]
+        L.ibtfo_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+        for name in ["ibtfo_destroy", "ibtfo_step", "ibtfo_step_graph", "ibtfo_status", "ibtfo_log",
+                     "ibtfo_compute_dt", "ibtfo_mass", "ibtfo_boundary", "ibtfo_prepare", "ibtfo_geometry",
+                     "ibtfo_initialize", "ibtfo_upload_state", "ibtfo_download_state", "ibtfo_download_geometry",
+                     "ibtfo_download_scheme_index", "ibtfo_sch_width", "ibtfo_num_ghosts", "ibtfo_dims",
+                     "ibtfo_time_steps", "ibtfo_device_state", "ibtfo_set_stream"]:
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def _check(rc, what):
+    if rc != OK:
+        raise IbtfoError(rc, what, lib().ibtfo_last_error().decode())
+
+
+def config(**kw):
+    """Config with the reference driver defaults, overridden by keyword (slot
+    names may be given as strings, e.g. Scheme='MHM')."""
+    c = Config()
+    lib().ibtfo_config_default(C.byref(c))
+    names = {"Scheme": ("scheme", SCHEME), "Riemann": ("riemann", RIEMANN), "Limiter": ("limiter", LIMITER),
+             "Boundary": ("boundary", BOUNDARY), "Time": ("time", TIME), "Edge": ("edge", EDGE),
+             "Precision": ("precision", PRECISION)}
+    for k, v in kw.items():
+        if k in names:
+            field, table = names[k]
+            setattr(c, field, table[v] if isinstance(v, str) else int(v))
+        elif k == "edge_state":
+            for q in range(4):
+                c.edge_state[q] = float(v[q])
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def config_from_inp(path):
+    c = Config()
+    _check(lib().ibtfo_config_from_inp(path.encode(), C.byref(c)), "ibtfo_config_from_inp")
+    return c
+
+
+class Solver:
+    """One solve on the GPU (a wrapper of ibtfo_ctx)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.h = C.c_void_p()
+        _check(lib().ibtfo_create(C.byref(cfg), C.byref(self.h)), "ibtfo_create")
+        M, N = C.c_int(), C.c_int()
+        lib().ibtfo_dims(self.h, C.byref(M), C.byref(N))
+        self.M, self.N = M.value, N.value
+
+    def close(self):
+        if self.h:
+            lib().ibtfo_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def geometry(self, kind="none", prm=(), phi=None):
+        k = LEVELSET[kind] if isinstance(kind, str) else int(kind)
+        p = np.ascontiguousarray(prm, dtype=np.float64)
+        ph = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+        _check(lib().ibtfo_geometry(self.h, k, ptr(p) if p.size else None, int(p.size), ptr(ph)), "ibtfo_geometry")
+
+    def initialize(self, kind, prm):
+        k = INIT[kind] if isinstance(kind, str) else int(kind)
+        p = np.ascontiguousarray(prm, dtype=np.float64)
+        _check(lib().ibtfo_initialize(self.h, k, ptr(p), int(p.size)), "ibtfo_initialize")
+
+    def upload(self, U_OLD, U_NEW=None):
+        _check(lib().ibtfo_upload_state(self.h, ptr(np.ascontiguousarray(U_OLD)),
+                                        ptr(None if U_NEW is None else np.ascontiguousarray(U_NEW))),
+               "ibtfo_upload_state")
+
+    def prepare(self, apply_boundary):
+        _check(lib().ibtfo_prepare(self.h, int(apply_boundary)), "ibtfo_prepare")
+
+    def step(self, n, graph_chunk=0):
+        if graph_chunk:
+            _check(lib().ibtfo_step_graph(self.h, int(n), int(graph_chunk)), "ibtfo_step_graph")
+        else:
+            _check(lib().ibtfo_step(self.h, int(n)), "ibtfo_step")
+
+    def status(self):
+        it, t, dt = C.c_int(), C.c_double(), C.c_double()
+        rc = lib().ibtfo_status(self.h, C.byref(it), C.byref(t), C.byref(dt))
+        if rc not in (OK, E_NONFINITE):
+            _check(rc, "ibtfo_status")
+        return it.value, t.value, dt.value, rc
+
+    def log(self, count):
+        d, t, m = np.zeros(count), np.zeros(count), np.zeros(count)
+        if count:
+            _check(lib().ibtfo_log(self.h, int(count), ptr(d), ptr(t), ptr(m)), "ibtfo_log")
+        return d, t, m
+
+    def state(self):
+        U = np.zeros(self.M * self.N * 4)
+        _check(lib().ibtfo_download_state(self.h, ptr(U)), "ibtfo_download_state")
+        return U
+
+    def xycoord(self):
+        XY = np.zeros(self.M * self.N * 6)
+        _check(lib().ibtfo_download_geometry(self.h, ptr(XY)), "ibtfo_download_geometry")
+        return XY
+
+    def scheme_index(self):
+        W = lib().ibtfo_sch_width(self.h)
+        S = np.zeros(self.M * self.N * W)
+        _check(lib().ibtfo_download_scheme_index(self.h, ptr(S)), "ibtfo_download_scheme_index")
+        return S
+
+    def num_ghosts(self):
+        return lib().ibtfo_num_ghosts(self.h)
+
+    def compute_dt(self):
+        v = C.c_double()
+        _check(lib().ibtfo_compute_dt(self.h, C.byref(v)), "ibtfo_compute_dt")
+        return v.value
+
+    def mass(self):
+        v = C.c_double()
+        _check(lib().ibtfo_mass(self.h, C.byref(v)), "ibtfo_mass")
+        return v.value
+
+    def boundary(self):
+        _check(lib().ibtfo_boundary(self.h), "ibtfo_boundary")
+
+    def time_steps(self, n, kernel=True):
+        tot, ker = C.c_double(), C.c_double()
+        _check(lib().ibtfo_time_steps(self.h, int(n), C.byref(tot), C.byref(ker) if kernel else None),
+               "ibtfo_time_steps")
+        return tot.value, (ker.value if kernel else None)
+
+
+def run_case(cfg, levelset=("none", ()), init=("sod_rows", ()), nsteps=None, apply_boundary=None, graph_chunk=0,
+             chunk=256):
+    """Reference driver order on the GPU: geometry, Initialize, [Boundary],
+    ComputeDt, then steps until the device stops (t_end / max_iter) or `nsteps`."""
+    s = Solver(cfg)
+    s.geometry(levelset[0], levelset[1])
+    s.initialize(init[0], init[1])
+    if apply_boundary is None:
+        apply_boundary = cfg.boundary != BOUNDARY["Regular"]
+    s.prepare(apply_boundary)
+    limit = cfg.max_iter if nsteps is None else min(nsteps, cfg.max_iter)
+    done = 0
+    while done < limit:
+        n = min(chunk, limit - done)
+        s.step(n, graph_chunk)
+        it, _, _, _ = s.status()
+        if it - done < n:
+            done = it
+            break
+        done = it
+    return s

@@ -1,0 +1,650 @@
+// C ABI of the IBTFO MI355X library: context, device memory, geometry, state
+// exchange and the device-resident time loop (include/ibtfo.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/ibtfo.h"
+#include "ibtfo_geom.h"
+#include "ibtfo_host.h"
+#include "ibtfo_internal.h"
+
+using namespace ibtfo;
+
+namespace ibtfo {
+static thread_local char g_err[512];
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+}  // namespace ibtfo
+
+struct ibtfo_ctx {
+    ibtfo_config cfg;
+    Layout L;
+    int device;
+    hipStream_t stream;
+    hipStream_t own_stream;
+    double* state[2];
+    uint8_t* type;
+    double *phi, *gnx, *gny;
+    Ghost* ghosts;
+    int nghost;
+    double* gvals;
+    unsigned int* dcount;
+    Ctl* ctl;
+    double* mass_part;
+    int mass_cap;
+    double* corner;
+    double Li, Hi, dxc;  // cell-centre origin and spacing (Psy_coord)
+    int have_geometry, have_state, prepared;
+    StepArgs sa;
+    FillArgs fa;
+    EdgeArgs ea;
+    ReduceArgs ra;
+    hipGraphExec_t graph;
+    int graph_chunk;
+    std::vector<double> corner_host;
+};
+
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            set_error("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return IBTFO_E_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+
+static Layout make_layout(const ibtfo_config* c) {
+    Layout L;
+    L.nx = c->N_x; L.ny = c->N_y; L.ng = c->num_ghost_cell;
+    L.M = L.nx + 2 * L.ng; L.N = L.ny + 2 * L.ng;
+    L.off = (16 - L.ng % 16) % 16;
+    L.pitch = (int)round_up(L.M + L.off, 16);
+    L.plane = round_up((long long)L.off + (long long)L.pitch * L.N + 16, 64);
+    return L;
+}
+
+static int step_grid(const Layout& L, int* nbx, int* nby, int* rows) {
+    *nbx = (L.nx + STRIP_W - 1) / STRIP_W;
+    // aim at >= 2048 blocks (8 per CU); at least 8 rows per strip
+    int want_y = std::max(1, 2048 / *nbx);
+    int r = std::max(8, (L.ny + want_y - 1) / want_y);
+    r = std::min(r, std::max(L.ny, 1));
+    *rows = r;
+    *nby = (L.ny + r - 1) / r;
+    return 0;
+}
+
+static int validate(const ibtfo_config* c) {
+    if (c->num_ghost_cell < 2 || c->num_ghost_cell > 4) { set_error("num_ghost_cell must be 2..4"); return IBTFO_E_ARG; }
+    if (c->N_x < 4 || c->N_y < 2 * c->num_ghost_cell) { set_error("grid too small"); return IBTFO_E_ARG; }
+    if ((long long)c->N_x * c->N_y > (1ll << 31)) { set_error("grid too large"); return IBTFO_E_ARG; }
+    if (c->scheme < 0 || c->scheme > 3 || c->riemann < 0 || c->riemann > 3 || c->limiter < 0 || c->limiter > 2 ||
+        c->boundary < 0 || c->boundary > 5 || c->edge < 0 || c->edge > 2) {
+        set_error("slot value out of range");
+        return IBTFO_E_ARG;
+    }
+    if (c->time != IBTFO_SINGLESTEP) { set_error("Time=RK2 is not available in this build"); return IBTFO_E_ARG; }
+    if (c->precision != IBTFO_FP64) { set_error("mixed precision is not available in this build"); return IBTFO_E_ARG; }
+    if (!(c->gamma > 1.0) || !(c->hi_x > c->lo_x) || !(c->hi_y > c->lo_y)) { set_error("bad physical parameters"); return IBTFO_E_ARG; }
+    return IBTFO_OK;
+}
+
+extern "C" const char* ibtfo_last_error(void) { return g_err; }
+
+extern "C" long long ibtfo_footprint(const ibtfo_config* c) {
+    if (!c) return -1;
+    Layout L = make_layout(c);
+    long long b = 8 * L.plane * 8;            // two 4-plane state buffers
+    b += L.plane;                              // types
+    b += 3 * 8 * L.plane;                      // phi, normals
+    b += (long long)sizeof(Ctl) + (1ll << 20);  // control, partials
+    b += (long long)(2 * (L.M + L.N) * L.ng) * (sizeof(Ghost) + 32);  // ghost list upper bound
+    return b;
+}
+
+static void free_ctx(ibtfo_ctx* x) {
+    if (!x) return;
+    if (x->graph) hipGraphExecDestroy(x->graph);
+    hipFree(x->state[0]); hipFree(x->state[1]); hipFree(x->type);
+    hipFree(x->phi); hipFree(x->gnx); hipFree(x->gny);
+    hipFree(x->ghosts); hipFree(x->gvals); hipFree(x->dcount);
+    hipFree(x->ctl); hipFree(x->mass_part); hipFree(x->corner);
+    if (x->own_stream) hipStreamDestroy(x->own_stream);
+    delete x;
+}
+
+extern "C" int ibtfo_create(const ibtfo_config* cfg, ibtfo_ctx** out) {
+    if (!cfg || !out) { set_error("null argument"); return IBTFO_E_ARG; }
+    *out = nullptr;
+    int rc = validate(cfg);
+    if (rc) return rc;
+    ibtfo_ctx* x = new ibtfo_ctx();  // value-initialised: every plain member starts at zero
+    x->cfg = *cfg;
+    x->L = make_layout(cfg);
+    const Layout& L = x->L;
+    if (hipGetDevice(&x->device) != hipSuccess) { delete x; set_error("no HIP device"); return IBTFO_E_HIP; }
+    auto fail = [&](const char* what) {
+        set_error("allocation of %s failed (%s)", what, hipGetErrorString(hipGetLastError()));
+        free_ctx(x);
+        return IBTFO_E_NOMEM;
+    };
+    if (hipMalloc(&x->state[0], 4 * L.plane * sizeof(double)) != hipSuccess) return fail("state");
+    if (hipMalloc(&x->state[1], 4 * L.plane * sizeof(double)) != hipSuccess) return fail("state");
+    if (hipMalloc(&x->type, L.plane) != hipSuccess) return fail("types");
+    if (hipMalloc(&x->phi, L.plane * sizeof(double)) != hipSuccess) return fail("phi");
+    if (hipMalloc(&x->gnx, L.plane * sizeof(double)) != hipSuccess) return fail("normals");
+    if (hipMalloc(&x->gny, L.plane * sizeof(double)) != hipSuccess) return fail("normals");
+    if (hipMalloc(&x->ctl, sizeof(Ctl)) != hipSuccess) return fail("control");
+    if (hipMalloc(&x->dcount, sizeof(unsigned int)) != hipSuccess) return fail("counter");
+    int nbx, nby, rows;
+    step_grid(L, &nbx, &nby, &rows);
+    x->mass_cap = std::max(nbx * nby, 4096);
+    if (hipMalloc(&x->mass_part, x->mass_cap * sizeof(double)) != hipSuccess) return fail("partials");
+    if (hipMalloc(&x->corner, 2 * 2 * L.ng * 4 * sizeof(double)) != hipSuccess) return fail("corner");
+    if (hipStreamCreate(&x->own_stream) != hipSuccess) return fail("stream");
+    x->stream = x->own_stream;
+    // zero everything the kernels may read before it is written
+    if (hipMemset(x->state[0], 0, 4 * L.plane * sizeof(double)) != hipSuccess ||
+        hipMemset(x->state[1], 0, 4 * L.plane * sizeof(double)) != hipSuccess ||
+        hipMemset(x->type, 2, L.plane) != hipSuccess || hipMemset(x->ctl, 0, sizeof(Ctl)) != hipSuccess ||
+        hipMemset(x->phi, 0, L.plane * sizeof(double)) != hipSuccess ||
+        hipMemset(x->gnx, 0, L.plane * sizeof(double)) != hipSuccess ||
+        hipMemset(x->gny, 0, L.plane * sizeof(double)) != hipSuccess ||
+        hipMemset(x->corner, 0, 2 * 2 * L.ng * 4 * sizeof(double)) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess)
+        return fail("initial clear");
+    const double Psy_L = cfg->hi_x - cfg->lo_x;
+    x->dxc = Psy_L / cfg->N_x;  // Psy_coord.cpp:21-22 (y spaced with dx as well)
+    x->Li = -x->dxc * L.ng + cfg->lo_x + x->dxc / 2;
+    x->Hi = -x->dxc * L.ng + cfg->lo_y + x->dxc / 2;
+    *out = x;
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_destroy(ibtfo_ctx* x) {
+    if (!x) return IBTFO_E_ARG;
+    hipStreamSynchronize(x->stream);
+    free_ctx(x);
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_set_stream(ibtfo_ctx* x, void* s) {
+    if (!x) return IBTFO_E_ARG;
+    x->stream = s ? (hipStream_t)s : x->own_stream;
+    if (x->graph) { hipGraphExecDestroy(x->graph); x->graph = nullptr; x->graph_chunk = 0; }
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_dims(const ibtfo_ctx* x, int* M, int* N) {
+    if (!x) return IBTFO_E_ARG;
+    if (M) *M = x->L.M;
+    if (N) *N = x->L.N;
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_sch_width(const ibtfo_ctx* x) {
+    switch (x->cfg.boundary) {
+        case IBTFO_IDW_GFM: return 8;
+        case IBTFO_LINEAR_GFM: case IBTFO_RIEMANN_GFM: return 4;
+        case IBTFO_BCGFM: case IBTFO_HBCGFM: return 16;
+        default: return 1;
+    }
+}
+extern "C" int ibtfo_num_ghosts(const ibtfo_ctx* x) { return x ? x->nghost : -1; }
+
+// --------------------------------------------------------------- geometry
+extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int nprm, const double* phi_host) {
+    if (!x) return IBTFO_E_ARG;
+    const Layout& L = x->L;
+    GeomArgs g;
+    memset(&g, 0, sizeof g);
+    g.L = L;
+    g.Li = x->Li; g.Hi = x->Hi; g.dx = x->dxc;
+    g.kind = kind;
+    g.phi = x->phi; g.nx = x->gnx; g.ny = x->gny; g.type = x->type;
+    g.bc = x->cfg.boundary;
+    double* dphi = nullptr;
+    auto need = [&](int n) {
+        if (nprm < n || !prm) { set_error("level-set hook %d needs %d parameters", kind, n); return false; }
+        return true;
+    };
+    switch (kind) {
+        case IBTFO_LS_NONE: break;
+        case IBTFO_LS_CHANNEL: {  // Test/InclineSod/Level_Set_function.cpp:35-49, constants on the host
+            if (!need(2)) return IBTFO_E_ARG;
+            double angle = 3.1415926535 * prm[0] / 180;
+            double AxisD = prm[1] / cos(angle);
+            double k = tan(angle);
+            g.p[0] = k; g.p[1] = pow(1 + k * k, 0.5); g.p[2] = 0.05; g.p[3] = AxisD;
+            break;
+        }
+        case IBTFO_LS_ANNULUS: {  // Test/Vortex/Level_Set_function.cpp:16-20
+            if (!need(2)) return IBTFO_E_ARG;
+            g.p[0] = prm[0] * 0.40; g.p[1] = prm[1] * 0.40;
+            break;
+        }
+        case IBTFO_LS_ELLIPSE: {
+            if (!need(5)) return IBTFO_E_ARG;
+            double ang = prm[4] * M_PI / 180.0;
+            g.p[0] = prm[0]; g.p[1] = prm[1]; g.p[2] = prm[2]; g.p[3] = prm[3]; g.p[4] = cos(ang); g.p[5] = sin(ang);
+            break;
+        }
+        case IBTFO_LS_FIELD: {
+            if (!phi_host) { set_error("IBTFO_LS_FIELD needs phi"); return IBTFO_E_ARG; }
+            HIPCHK(hipMalloc(&dphi, (size_t)L.M * L.N * sizeof(double)));
+            HIPCHK(hipMemcpy(dphi, phi_host, (size_t)L.M * L.N * sizeof(double), hipMemcpyHostToDevice));
+            g.phi_in = dphi;
+            break;
+        }
+        default: set_error("unknown level-set kind %d", kind); return IBTFO_E_ARG;
+    }
+    HIPCHK(geom_run(g, x->stream));
+    // ghost list (upper bound: all interior cells of type 2; count first)
+    hipFree(x->ghosts); x->ghosts = nullptr;
+    hipFree(x->gvals); x->gvals = nullptr;
+    x->nghost = 0;
+    if (x->cfg.boundary != IBTFO_REGULAR) {
+        unsigned int zero = 0, cnt = 0;
+        HIPCHK(hipMemcpyAsync(x->dcount, &zero, sizeof zero, hipMemcpyHostToDevice, x->stream));
+        g.count = x->dcount; g.cap = 0; g.ghosts = nullptr;
+        HIPCHK(geom_ghosts(g, x->stream));
+        HIPCHK(hipMemcpyAsync(&cnt, x->dcount, sizeof cnt, hipMemcpyDeviceToHost, x->stream));
+        HIPCHK(hipStreamSynchronize(x->stream));
+        if (cnt > 0) {
+            HIPCHK(hipMalloc(&x->ghosts, cnt * sizeof(Ghost)));
+            HIPCHK(hipMalloc(&x->gvals, cnt * 4 * sizeof(double)));
+            HIPCHK(hipMemcpyAsync(x->dcount, &zero, sizeof zero, hipMemcpyHostToDevice, x->stream));
+            g.ghosts = x->ghosts; g.cap = (int)cnt;
+            HIPCHK(geom_ghosts(g, x->stream));
+        }
+        x->nghost = (int)cnt;
+    }
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (dphi) hipFree(dphi);
+    x->have_geometry = 1;
+    x->prepared = 0;
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_download_geometry(ibtfo_ctx* x, double* XY) {
+    if (!x || !XY) return IBTFO_E_ARG;
+    const Layout& L = x->L;
+    std::vector<double> phi(L.plane), nx(L.plane), ny(L.plane);
+    std::vector<uint8_t> t(L.plane);
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(phi.data(), x->phi, L.plane * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nx.data(), x->gnx, L.plane * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ny.data(), x->gny, L.plane * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(t.data(), x->type, L.plane, hipMemcpyDeviceToHost));
+    for (int j = 0; j < L.N; j++)
+        for (int i = 0; i < L.M; i++) {
+            double* r = XY + ((size_t)L.M * j + i) * 6;
+            long long o = L.at(i, j);
+            r[0] = x->Li + i * x->dxc;
+            r[1] = x->Hi + j * x->dxc;
+            r[2] = phi[o]; r[3] = nx[o]; r[4] = ny[o]; r[5] = t[o];
+        }
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_download_scheme_index(ibtfo_ctx* x, double* S) {
+    if (!x || !S) return IBTFO_E_ARG;
+    const Layout& L = x->L;
+    const int W = ibtfo_sch_width(x);
+    memset(S, 0, sizeof(double) * (size_t)L.M * L.N * W);
+    if (x->cfg.boundary == IBTFO_REGULAR || x->nghost == 0) return IBTFO_OK;
+    std::vector<Ghost> gh(x->nghost);
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(gh.data(), x->ghosts, x->nghost * sizeof(Ghost), hipMemcpyDeviceToHost));
+    for (const Ghost& g : gh) {
+        long long rel = g.cell - L.off;
+        int j = (int)(rel / L.pitch), i = (int)(rel % L.pitch);
+        double* r = S + ((size_t)L.M * j + i) * W;
+        r[0] = g.xm; r[1] = g.ym; r[2] = g.idx; r[3] = g.idy;
+        if (W >= 8) for (int q = 0; q < 4; q++) r[4 + q] = g.a[q];
+        if (W == 16) {
+            r[8] = g.xe; r[9] = g.ye; r[10] = g.idxe; r[11] = g.idye;
+            for (int q = 0; q < 4; q++) r[12 + q] = g.ae[q];
+        }
+    }
+    return IBTFO_OK;
+}
+
+// ------------------------------------------------------------------ state
+static void corner_from_aos(ibtfo_ctx* x, const double* U) {
+    const Layout& L = x->L;
+    x->corner_host.assign(2 * 2 * L.ng * 4, 0.0);
+    for (int side = 0; side < 2; side++)
+        for (int ri = 0; ri < 2 * L.ng; ri++) {
+            int row = ri < L.ng ? ri : L.ny + ri;
+            int col = side == 0 ? L.ng : L.nx + L.ng - 1;
+            for (int k = 0; k < 4; k++)
+                x->corner_host[(side * 2 * L.ng + ri) * 4 + k] = U[((size_t)L.M * row + col) * 4 + k];
+        }
+}
+
+static int upload_aos(ibtfo_ctx* x, const double* U, int buf) {
+    const Layout& L = x->L;
+    std::vector<double> planes(4 * L.plane, 0.0);
+    for (int j = 0; j < L.N; j++)
+        for (int i = 0; i < L.M; i++) {
+            const double* q = U + ((size_t)L.M * j + i) * 4;
+            long long o = L.at(i, j);
+            for (int k = 0; k < 4; k++) planes[o + k * L.plane] = q[k];
+        }
+    HIPCHK(hipMemcpy(x->state[buf], planes.data(), 4 * L.plane * sizeof(double), hipMemcpyHostToDevice));
+    return IBTFO_OK;
+}
+
+static int read_ctl_head(ibtfo_ctx* x, Ctl& c) {
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(&c, x->ctl, offsetof(Ctl, log_dt), hipMemcpyDeviceToHost));
+    return IBTFO_OK;
+}
+static int write_ctl_head(ibtfo_ctx* x, const Ctl& c) {
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(x->ctl, &c, offsetof(Ctl, log_dt), hipMemcpyHostToDevice));
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_upload_state(ibtfo_ctx* x, const double* U_OLD, const double* U_NEW) {
+    if (!x || !U_OLD) return IBTFO_E_ARG;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    int rc = upload_aos(x, U_OLD, 0);
+    if (rc) return rc;
+    corner_from_aos(x, U_NEW ? U_NEW : U_OLD);
+    HIPCHK(hipMemcpy(x->corner, x->corner_host.data(), x->corner_host.size() * sizeof(double), hipMemcpyHostToDevice));
+    Ctl c;
+    if ((rc = read_ctl_head(x, c))) return rc;
+    c.cur = 0;
+    c.ran = 0;
+    if ((rc = write_ctl_head(x, c))) return rc;
+    x->have_state = 1;
+    x->prepared = 0;
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_download_state(ibtfo_ctx* x, double* U) {
+    if (!x || !U) return IBTFO_E_ARG;
+    const Layout& L = x->L;
+    Ctl c;
+    int rc = read_ctl_head(x, c);
+    if (rc) return rc;
+    std::vector<double> planes(4 * L.plane);
+    HIPCHK(hipMemcpy(planes.data(), x->state[c.cur], 4 * L.plane * sizeof(double), hipMemcpyDeviceToHost));
+    for (int j = 0; j < L.N; j++)
+        for (int i = 0; i < L.M; i++) {
+            double* q = U + ((size_t)L.M * j + i) * 4;
+            long long o = L.at(i, j);
+            for (int k = 0; k < 4; k++) q[k] = planes[o + k * L.plane];
+        }
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_device_state(ibtfo_ctx* x, double** planes4, int* pitch, int* offset) {
+    if (!x || !planes4) return IBTFO_E_ARG;
+    Ctl c;
+    int rc = read_ctl_head(x, c);
+    if (rc) return rc;
+    for (int k = 0; k < 4; k++) planes4[k] = x->state[c.cur] + k * x->L.plane;
+    if (pitch) *pitch = x->L.pitch;
+    if (offset) *offset = x->L.off;
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_initialize(ibtfo_ctx* x, int kind, const double* prm, int nprm) {
+    if (!x) return IBTFO_E_ARG;
+    if (!x->have_geometry) { set_error("ibtfo_initialize needs ibtfo_geometry first"); return IBTFO_E_STATE; }
+    const Layout& L = x->L;
+    const size_t cells = (size_t)L.M * L.N;
+    std::vector<double> xs(cells), ys(cells), U(cells * 4);
+    std::vector<uint8_t> tp(L.plane), t(cells);
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(tp.data(), x->type, L.plane, hipMemcpyDeviceToHost));
+    for (int j = 0; j < L.N; j++)
+        for (int i = 0; i < L.M; i++) {
+            size_t q = (size_t)L.M * j + i;
+            xs[q] = x->Li + i * x->dxc;
+            ys[q] = x->Hi + j * x->dxc;
+            t[q] = tp[L.at(i, j)];
+        }
+    int rc = host_initialize(kind, prm, nprm, L.nx, L.ny, L.ng, x->cfg.gamma, xs.data(), ys.data(), t.data(), U.data());
+    if (rc) return rc;
+    return ibtfo_upload_state(x, U.data(), nullptr);
+}
+
+// -------------------------------------------------------------- time loop
+static void build_args(ibtfo_ctx* x) {
+    const ibtfo_config& c = x->cfg;
+    const Layout& L = x->L;
+    StepArgs& a = x->sa;
+    memset(&a, 0, sizeof a);
+    a.buf[0] = x->state[0]; a.buf[1] = x->state[1];
+    a.L = L;
+    a.type = x->type;
+    a.ctl = x->ctl;
+    a.mass_part = x->mass_part;
+    a.corner = x->corner;
+    a.gam = c.gamma; a.gm1 = c.gamma - 1; a.gp1 = c.gamma + 1; a.c1 = (c.gamma - 1) / 2 / c.gamma;
+    a.dx = (c.hi_x - c.lo_x) / c.N_x;   // MUSCL_Scheme.cpp:64-65
+    a.dy = (c.hi_y - c.lo_y) / c.N_y;
+    a.cfl = c.CFL_number;
+    a.t_end = c.Psy_time;
+    a.max_iter = c.max_iter;
+    a.quirks = c.quirks;
+    a.edge_fused = (c.edge == IBTFO_EDGE_REFERENCE && c.scheme != IBTFO_LAX) ? 1 : 0;
+    a.corner_from_input = c.boundary != IBTFO_REGULAR;
+    step_grid(L, &a.nbx, &a.nby, &a.rows);
+
+    FillArgs& f = x->fa;
+    memset(&f, 0, sizeof f);
+    f.buf[0] = x->state[0]; f.buf[1] = x->state[1];
+    f.L = L; f.ctl = x->ctl; f.ghosts = x->ghosts; f.nghost = x->nghost; f.out = x->gvals;
+    f.bc = c.boundary; f.gam = c.gamma; f.gm1 = c.gamma - 1;
+    f.dxg = (x->Li + 1 * x->dxc) - (x->Li + 0 * x->dxc);
+    f.dyg = (x->Hi + 2 * x->dxc) - (x->Hi + 1 * x->dxc);
+    f.use_cur = 1; f.require_ran = 1; f.ring_prev = 1;
+
+    EdgeArgs& e = x->ea;
+    memset(&e, 0, sizeof e);
+    e.buf[0] = x->state[0]; e.buf[1] = x->state[1];
+    e.L = L; e.ctl = x->ctl; e.kind = c.edge; e.gam = c.gamma; e.require_ran = 1;
+    double r = c.edge_state[0], u = c.edge_state[1], v = c.edge_state[2], p = c.edge_state[3];
+    e.inflow[0] = r; e.inflow[1] = r * u; e.inflow[2] = r * v;
+    e.inflow[3] = p / (c.gamma - 1) + 0.5 * r * (u * u + v * v);
+
+    ReduceArgs& q = x->ra;
+    memset(&q, 0, sizeof q);
+    q.L = L; q.type = x->type; q.ctl = x->ctl; q.mass_part = x->mass_part;
+    q.gam = c.gamma; q.gm1 = c.gamma - 1; q.dx = a.dx; q.dy = a.dy; q.cfl = c.CFL_number;
+    q.t_end = c.Psy_time; q.max_iter = c.max_iter;
+    q.nblocks = std::min(x->mass_cap, std::max(1, (L.nx * L.ny + 255) / 256));
+}
+
+static int enqueue_step(ibtfo_ctx* x) {
+    const ibtfo_config& c = x->cfg;
+    HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, x->sa, x->stream));
+    if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_fill(x->fa, x->stream));
+    if (c.edge == IBTFO_EDGE_REFERENCE && c.scheme == IBTFO_LAX)
+        HIPCHK(launch_edge_ref(x->ea, x->corner, x->sa.corner_from_input, x->stream));
+    if (c.edge != IBTFO_EDGE_REFERENCE) HIPCHK(launch_edge(x->ea, x->stream));
+    if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_scatter(x->fa, x->stream));
+    return IBTFO_OK;
+}
+
+static int run_reduce(ibtfo_ctx* x, int reset) {
+    Ctl c;
+    int rc = read_ctl_head(x, c);
+    if (rc) return rc;
+    x->ra.U = x->state[c.cur];
+    x->ra.reset = reset;
+    HIPCHK(launch_reduce(x->ra, x->stream));
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_boundary(ibtfo_ctx* x) {
+    if (!x) return IBTFO_E_ARG;
+    if (!x->have_geometry || !x->have_state) { set_error("boundary needs geometry and state"); return IBTFO_E_STATE; }
+    build_args(x);
+    if (x->cfg.boundary == IBTFO_REGULAR) return IBTFO_OK;  // U_OLD = U_NEW on the interior: already equal
+    FillArgs f = x->fa;
+    f.require_ran = 0;
+    f.ring_prev = 0;  // U_NEW = U_OLD here, ring included
+    HIPCHK(launch_fill(f, x->stream));
+    HIPCHK(launch_scatter(f, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_prepare(ibtfo_ctx* x, int apply_boundary) {
+    if (!x) return IBTFO_E_ARG;
+    if (!x->have_geometry || !x->have_state) { set_error("prepare needs geometry and state"); return IBTFO_E_STATE; }
+    build_args(x);
+    int rc;
+    if (apply_boundary && (rc = ibtfo_boundary(x))) return rc;
+    if ((rc = run_reduce(x, 1))) return rc;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->prepared = 1;
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_step(ibtfo_ctx* x, int nsteps) {
+    if (!x || nsteps < 0) return IBTFO_E_ARG;
+    if (!x->prepared) { set_error("ibtfo_step before ibtfo_prepare"); return IBTFO_E_STATE; }
+    for (int s = 0; s < nsteps; s++) {
+        int rc = enqueue_step(x);
+        if (rc) return rc;
+    }
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_step_graph(ibtfo_ctx* x, int nsteps, int chunk) {
+    if (!x || nsteps < 0 || chunk <= 0) return IBTFO_E_ARG;
+    if (!x->prepared) { set_error("ibtfo_step_graph before ibtfo_prepare"); return IBTFO_E_STATE; }
+    if (!x->graph || x->graph_chunk != chunk) {
+        if (x->graph) { hipGraphExecDestroy(x->graph); x->graph = nullptr; }
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+        int rc = IBTFO_OK;
+        for (int s = 0; s < chunk && rc == IBTFO_OK; s++) rc = enqueue_step(x);
+        hipError_t e = hipStreamEndCapture(x->stream, &g);
+        if (rc) return rc;
+        HIPCHK(e);
+        HIPCHK(hipGraphInstantiate(&x->graph, g, nullptr, nullptr, 0));
+        hipGraphDestroy(g);
+        x->graph_chunk = chunk;
+    }
+    int full = nsteps / chunk, rest = nsteps % chunk;
+    for (int r = 0; r < full; r++) HIPCHK(hipGraphLaunch(x->graph, x->stream));
+    return ibtfo_step(x, rest);
+}
+
+extern "C" int ibtfo_status(ibtfo_ctx* x, int* iter, double* now_t, double* dt) {
+    if (!x) return IBTFO_E_ARG;
+    Ctl c;
+    int rc = read_ctl_head(x, c);
+    if (rc) return rc;
+    if (iter) *iter = c.iter;
+    if (now_t) *now_t = c.now_t;
+    if (dt) *dt = c.dt;
+    return c.status ? IBTFO_E_NONFINITE : IBTFO_OK;
+}
+
+extern "C" int ibtfo_log(ibtfo_ctx* x, int count, double* dt, double* t, double* mass) {
+    if (!x || count < 0 || count > LOGCAP - 1) return IBTFO_E_ARG;
+    Ctl c;
+    int rc = read_ctl_head(x, c);
+    if (rc) return rc;
+    if (count > c.iter) { set_error("only %d steps logged", c.iter); return IBTFO_E_ARG; }
+    std::vector<double> ld(LOGCAP), lt(LOGCAP), lm(LOGCAP);
+    HIPCHK(hipMemcpy(ld.data(), x->ctl->log_dt, sizeof(double) * LOGCAP, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lt.data(), x->ctl->log_t, sizeof(double) * LOGCAP, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lm.data(), x->ctl->log_mass, sizeof(double) * LOGCAP, hipMemcpyDeviceToHost));
+    for (int q = 0; q < count; q++) {
+        int it = c.iter - count + q;
+        int s = it % LOGCAP;
+        if (dt) dt[q] = ld[s];
+        if (t) t[q] = lt[s];
+        if (mass) mass[q] = lm[s];
+    }
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_compute_dt(ibtfo_ctx* x, double* dt) {
+    if (!x || !dt) return IBTFO_E_ARG;
+    if (!x->have_geometry || !x->have_state) return IBTFO_E_STATE;
+    build_args(x);
+    int rc = run_reduce(x, 0);
+    if (rc) return rc;
+    Ctl c;
+    if ((rc = read_ctl_head(x, c))) return rc;
+    *dt = c.dt;
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_mass(ibtfo_ctx* x, double* mass) {
+    if (!x || !mass) return IBTFO_E_ARG;
+    if (!x->have_geometry || !x->have_state) return IBTFO_E_STATE;
+    build_args(x);
+    int rc = run_reduce(x, 0);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(mass, &x->ctl->mass_now, sizeof(double), hipMemcpyDeviceToHost));
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_time_steps(ibtfo_ctx* x, int nsteps, double* total_ms, double* step_kernel_ms) {
+    if (!x || nsteps <= 0) return IBTFO_E_ARG;
+    if (!x->prepared) return IBTFO_E_STATE;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipEventRecord(e0, x->stream));
+    int rc = ibtfo_step(x, nsteps);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(e1, x->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (total_ms) *total_ms = ms;
+    if (step_kernel_ms) {
+        std::vector<hipEvent_t> ev(2 * nsteps);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        const ibtfo_config& c = x->cfg;
+        for (int s = 0; s < nsteps; s++) {
+            HIPCHK(hipEventRecord(ev[2 * s], x->stream));
+            HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, x->sa, x->stream));
+            HIPCHK(hipEventRecord(ev[2 * s + 1], x->stream));
+            if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_fill(x->fa, x->stream));
+            if (c.edge == IBTFO_EDGE_REFERENCE && c.scheme == IBTFO_LAX)
+                HIPCHK(launch_edge_ref(x->ea, x->corner, x->sa.corner_from_input, x->stream));
+            if (c.edge != IBTFO_EDGE_REFERENCE) HIPCHK(launch_edge(x->ea, x->stream));
+            if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_scatter(x->fa, x->stream));
+        }
+        HIPCHK(hipEventSynchronize(ev[2 * nsteps - 1]));
+        double sum = 0;
+        for (int s = 0; s < nsteps; s++) {
+            float m = 0;
+            HIPCHK(hipEventElapsedTime(&m, ev[2 * s], ev[2 * s + 1]));
+            sum += m;
+        }
+        for (auto& e : ev) hipEventDestroy(e);
+        *step_kernel_ms = sum / nsteps;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return IBTFO_OK;
+}

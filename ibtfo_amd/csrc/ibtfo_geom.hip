@@ -1,0 +1,211 @@
+// Geometry kernels: cell centres, level-set hooks, normals, fluid/solid/ghost
+// classification and the ghost-fluid donor records, run once per solve.
+//
+// All of it is integer logic and IEEE +,-,*,/,sqrt in the reference's order
+// (no contraction), so flags, mirror points, donor indices and weights are bit
+// identical to the reference's Src/Geometry and Scheme_Index results.
+#include "ibtfo_geom.h"
+
+namespace ibtfo {
+
+// Psy_coord.cpp:25-35: x_i = Li + i*dx, y_j = Hi + j*dx (the reference spaces y
+// with dx as well).
+__device__ __forceinline__ double xc(const GeomArgs& g, int i) { return g.Li + i * g.dx; }
+__device__ __forceinline__ double yc(const GeomArgs& g, int j) { return g.Hi + j * g.dx; }
+
+__device__ double hook_phi(const GeomArgs& g, double x, double y, long long q) {
+    switch (g.kind) {
+        case 1: {  // Test/InclineSod/Level_Set_function.cpp:8-20, :48-51
+            double k = g.p[0], nrm = g.p[1], b1 = g.p[2], b2 = g.p[3];
+            double p1 = (y > k * x + b1) ? 1 * ib_fabs(k * x - y + b1) / nrm : -1 * ib_fabs(k * x - y + b1) / nrm;
+            double p2 = (y > k * x + b2) ? -1 * ib_fabs(k * x - y + b2) / nrm : 1 * ib_fabs(k * x - y + b2) / nrm;
+            return (p1 < p2) ? p1 : p2;
+        }
+        case 2: {  // Test/Vortex/Level_Set_function.cpp:31-38
+            double r = ib_sqrt((x - 0.50) * (x - 0.50) + (y - 0.50) * (y - 0.50));
+            double p1 = g.p[1] - r, p2 = r - g.p[0];
+            return (p1 < p2) ? p1 : p2;
+        }
+        case 3:
+            return g.phi_in[q];
+        case 4: {  // ellipse body: phi > 0 outside; |phi| approximates the distance
+            double cx = g.p[0], cy = g.p[1], a = g.p[2], b = g.p[3], ca = g.p[4], sa = g.p[5];
+            double dxp = x - cx, dyp = y - cy;
+            double xr = dxp * ca + dyp * sa, yr = -dxp * sa + dyp * ca;
+            double s = ib_sqrt((xr / a) * (xr / a) + (yr / b) * (yr / b));
+            // first-order distance: (s - 1) / |grad s|
+            double gx = xr / (a * a), gy = yr / (b * b);
+            double gn = ib_sqrt(gx * gx + gy * gy);
+            return (s > 0.0 && gn > 0.0) ? (s * s - 1.0) / (2.0 * gn) : -((a < b) ? a : b);
+        }
+        default:
+            return 1.0;
+    }
+}
+
+__global__ void k_geom_phi(GeomArgs g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= g.L.M) return;
+    long long o = g.L.at(i, j);
+    double x = xc(g, i), y = yc(g, j);
+    g.phi[o] = hook_phi(g, x, y, (long long)j * g.L.M + i);
+}
+
+// Level_Set.cpp:71-86 (normals, ring left at 0) and :90-158 (classification in
+// gather form: a solid cell is a ghost when a fluid cell lies within ng cells
+// along x or y; the domain ring is ghost).
+__global__ void k_geom_classify(GeomArgs g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const Layout L = g.L;
+    if (i >= L.M) return;
+    const long long o = L.at(i, j);
+    double nxv = 0.0, nyv = 0.0;
+    if (i >= 1 && i < L.M - 1 && j >= 1 && j < L.N - 1) {
+        nxv = (g.phi[L.at(i + 1, j)] - g.phi[L.at(i - 1, j)]) / ((xc(g, i + 1) - xc(g, i - 1)));
+        nyv = (g.phi[L.at(i, j + 1)] - g.phi[L.at(i, j - 1)]) / ((yc(g, j + 1) - yc(g, j - 1)));
+    }
+    g.nx[o] = nxv;
+    g.ny[o] = nyv;
+    const int ng = L.ng;
+    auto interior = [&](int ii, int jj) { return ii >= ng && ii < L.nx + ng && jj >= ng && jj < L.ny + ng; };
+    uint8_t t;
+    if (!interior(i, j)) t = 2;
+    else if (g.phi[o] >= 0) t = 0;
+    else {
+        t = 1;
+        for (int k = 1; k <= ng; k++) {
+            const int ci[4] = {i - k, i + k, i, i};
+            const int cj[4] = {j, j, j - k, j + k};
+            for (int q = 0; q < 4; q++)
+                if (interior(ci[q], cj[q]) && g.phi[L.at(ci[q], cj[q])] >= 0) t = 2;
+        }
+    }
+    g.type[o] = t;
+}
+
+// Mirror_IDX.cpp:176-206: the ii in [0, M-1) with x_ii <= xm < x_{ii+1}; the same
+// half-open test on the candidates around the closed-form estimate. No match
+// leaves 0 (the value a zero-initialised record keeps in the reference).
+__device__ int donor_idx(double v, double base, double h, int n) {
+    double t = __builtin_floor((v - base) / h);
+    int r = 0;
+    if (!(t == t)) return 0;
+    int t0 = (int)fmax(fmin(t, (double)n + 4.0), -4.0);
+    for (int ii = t0 - 2; ii <= t0 + 2; ii++)
+        if (ii >= 0 && ii < n - 1 && v < base + (ii + 1) * h && v >= base + ii * h) r = ii;
+    return r;
+}
+
+// Scheme_Index for one ghost: IDWBaseGFM/GFM_Method.cpp:126-208,
+// LinearBaseGFM/GFM_Method.cpp:19-44 + GFM_Bound.cpp:93-575 (donor choice),
+// BCGFM/BCGFM_Method.cpp:49-205, HBCGFM/HBCGFM_Method.cpp:56-91.
+__global__ void k_geom_ghosts(GeomArgs g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const Layout L = g.L;
+    const int ng = L.ng;
+    if (i < ng || i >= L.nx + ng || j < ng || j >= L.ny + ng) return;
+    const long long o = L.at(i, j);
+    if (g.type[o] != 2) return;
+    Ghost gh;
+    gh.cell = o;
+    gh.nx = g.nx[o];
+    gh.ny = g.ny[o];
+    double ph = ib_fabs(g.phi[o]);
+    gh.phi = ph;
+    double x = xc(g, i), y = yc(g, j);
+    const int bc = g.bc;
+    double dxg = xc(g, 1) - xc(g, 0), dyg = yc(g, 2) - yc(g, 1);
+    if (bc == 1 || bc == 2 || bc == 3) {
+        gh.xm = x + 2 * gh.nx * ph;
+        gh.ym = y + 2 * gh.ny * ph;
+        gh.xe = gh.ye = 0.0;
+    } else {
+        double delta = ib_sqrt(dxg * dxg + dyg * dyg) / 2;
+        if (bc == 5 && ph > delta) {
+            gh.xm = x + gh.nx * (2 * ph); gh.ym = y + gh.ny * (2 * ph);
+            gh.xe = x + gh.nx * (2 * ph); gh.ye = y + gh.ny * (2 * ph);
+        } else {
+            gh.xm = x + gh.nx * (ph + delta); gh.ym = y + gh.ny * (ph + delta);
+            gh.xe = x + gh.nx * (ph + 2.0 * delta); gh.ye = y + gh.ny * (ph + 2.0 * delta);
+        }
+    }
+    gh.idx = donor_idx(gh.xm, g.Li, g.dx, L.M);
+    gh.idy = donor_idx(gh.ym, g.Hi, g.dx, L.N);
+    gh.d1 = L.at(gh.idx, gh.idy);
+    const int di[4] = {0, 0, 1, 1}, dj[4] = {0, 1, 1, 0};
+    for (int q = 0; q < 4; q++) { gh.a[q] = 0.0; gh.ae[q] = 0.0; }
+    gh.idxe = gh.idye = 0;
+    gh.d1e = gh.d1;
+    if (bc == 1 || bc >= 4) {
+        for (int q = 0; q < 4; q++) {
+            int ii = gh.idx + di[q], jj = gh.idy + dj[q];
+            uint8_t t = g.type[L.at(ii, jj)];
+            bool keep = (bc == 1) ? (t < 1) : (t != 1);
+            if (keep) {
+                double ddx = xc(g, ii) - gh.xm, ddy = yc(g, jj) - gh.ym;
+                gh.a[q] = 1 / ib_sqrt(ddx * ddx + ddy * ddy);
+            }
+        }
+    }
+    if (bc >= 4) {
+        gh.idxe = donor_idx(gh.xe, g.Li, g.dx, L.M);
+        gh.idye = donor_idx(gh.ye, g.Hi, g.dx, L.N);
+        gh.d1e = L.at(gh.idxe, gh.idye);
+        for (int q = 0; q < 4; q++) {
+            int ii = gh.idxe + di[q], jj = gh.idye + dj[q];
+            if (g.type[L.at(ii, jj)] != 1) {
+                double ddx = xc(g, ii) - gh.xe, ddy = yc(g, jj) - gh.ye;
+                gh.ae[q] = 1 / ib_sqrt(ddx * ddx + ddy * ddy);
+            }
+        }
+    }
+    gh.ring = 0;
+    auto in_ring = [&](int ii, int jj) { return ii < ng || ii >= L.nx + ng || jj < ng || jj >= L.ny + ng; };
+    for (int q = 0; q < 4; q++) {
+        if (in_ring(gh.idx + di[q], gh.idy + dj[q])) gh.ring |= 1 << q;
+        if (bc >= 4 && in_ring(gh.idxe + di[q], gh.idye + dj[q])) gh.ring |= 1 << (4 + q);
+    }
+    gh.pick = -1;
+    if (bc == 2 || bc == 3) {
+        bool f[4];
+        for (int q = 0; q < 4; q++) f[q] = g.type[L.at(gh.idx + di[q], gh.idy + dj[q])] < 1;
+        bool f1 = f[0], f2 = f[1], f3 = f[2], f4 = f[3];
+        double x_d = gh.xm - xc(g, gh.idx);
+        double y_d = gh.xm - xc(g, gh.idx);  // the reference measures y_d with the x mirror too
+        bool xr = x_d > dxg / 2, yu = y_d > dyg / 2;
+        int pick = -1;
+        if (f1 && f4 && f2 && f3) pick = yu ? (xr ? 2 : 1) : (xr ? 3 : 0);
+        else if (!f1 && f4 && f2 && f3) pick = yu ? (xr ? 2 : 1) : 3;
+        else if (f1 && !f4 && f2 && f3) pick = yu ? (xr ? 2 : 1) : 0;
+        else if (f1 && f4 && !f2 && f3) pick = yu ? 2 : (xr ? 3 : 0);
+        else if (f1 && f4 && f2 && !f3) pick = yu ? 1 : (xr ? 3 : 0);
+        else if (f1 && f4 && !f2 && !f3) pick = xr ? 3 : 0;
+        else if (!f1 && !f4 && f2 && f3) pick = xr ? 2 : 1;
+        else if (f1 && !f4 && f2 && !f3) pick = yu ? 1 : 0;
+        else if (!f1 && f4 && !f2 && f3) pick = yu ? 2 : 3;
+        else if (f1 && !f4 && !f2 && f3) pick = yu ? 2 : 0;
+        else if (!f1 && f4 && f2 && !f3) pick = yu ? 1 : 3;
+        else if (f1 && !f4 && !f2 && !f3) pick = 0;
+        else if (!f1 && !f4 && f2 && !f3) pick = 1;
+        else if (!f1 && !f4 && !f2 && f3) pick = 2;
+        else if (!f1 && f4 && !f2 && !f3) pick = 3;
+        gh.pick = pick;
+        gh.a[0] = pick;
+    }
+    unsigned int slot = atomicAdd(g.count, 1u);
+    if (slot < (unsigned int)g.cap) g.ghosts[slot] = gh;
+}
+
+hipError_t geom_run(GeomArgs& g, hipStream_t s) {
+    dim3 blk(128), grd((g.L.M + 127) / 128, g.L.N);
+    hipLaunchKernelGGL(k_geom_phi, grd, blk, 0, s, g);
+    hipLaunchKernelGGL(k_geom_classify, grd, blk, 0, s, g);
+    return hipGetLastError();
+}
+hipError_t geom_ghosts(GeomArgs& g, hipStream_t s) {
+    dim3 blk(128), grd((g.L.M + 127) / 128, g.L.N);
+    hipLaunchKernelGGL(k_geom_ghosts, grd, blk, 0, s, g);
+    return hipGetLastError();
+}
+
+}  // namespace ibtfo

@@ -1,0 +1,123 @@
+// Internal layout and kernel interfaces of the IBTFO HIP library.
+//
+// Device layout (all FP64 state in SoA planes, x fastest):
+//   cell (i, j), 0 <= i < M = N_x + 2ng, 0 <= j < N = N_y + 2ng, lives at
+//   plane[off + j*pitch + i]; `off` puts the first interior column (i = ng) on a
+//   128-byte boundary and `pitch` is a multiple of 16 doubles, so every row's
+//   interior starts cache-line aligned. A state buffer is 4 planes (rho, rho u,
+//   rho v, E) spaced `plane` doubles apart; two buffers ping-pong between steps.
+//   Cell types are a uint8 plane with the same indexing.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "ibtfo_math.h"
+
+namespace ibtfo {
+
+constexpr int LOGCAP = 4096;  // per-step log ring (dt, time, mass)
+constexpr int STRIP_W = 63;   // interior columns updated by one 64-lane block
+
+struct Layout {
+    int nx, ny, ng, M, N;
+    int pitch, off;
+    long long plane;  // doubles between consecutive planes of one buffer
+    __host__ __device__ long long at(int i, int j) const { return (long long)off + (long long)j * pitch + i; }
+};
+
+// Device control block of one solve: the time loop runs on the device so that
+// steps can be queued (or replayed from a graph) without host round trips.
+struct Ctl {
+    double dt;      // dt for the next step (ComputeDt of the current state)
+    double now_t;   // physical time as the reference accumulates it
+    int iter;       // steps taken
+    int done;       // 1 once now_t >= Psy_time or iter >= max_iter
+    int cur;        // which state buffer holds the current state
+    int ran;        // 1 if the last launched step executed
+    int status;     // 0 ok; 1 non-finite value met in the dt reduction
+    int pad;
+    unsigned long long amax, umax, vmax;  // running maxima (bit patterns of non-negative doubles)
+    unsigned int arrive;                  // blocks finished in the current step
+    unsigned int pad2;
+    double mass_now;                      // CaculateMass of the state seen by the last reduction
+    double log_dt[LOGCAP], log_t[LOGCAP], log_mass[LOGCAP];
+};
+
+struct StepArgs {
+    double* buf[2];     // state buffers
+    Layout L;
+    const uint8_t* type;
+    Ctl* ctl;
+    double* mass_part;  // one partial sum per block
+    const double* corner;  // [2 sides][2ng rows][4]: U_NEW ring at columns ng / N_x+ng-1
+    double gam, gm1, gp1, c1, dx, dy, cfl, t_end;
+    int max_iter;
+    int quirks;
+    int edge_fused;         // write the reference Global_Boundary ring inside the step
+    int corner_from_input;  // corners from the input buffer's ring (GFM) or `corner` (Regular)
+    int rows;               // rows per block strip
+    int nbx, nby;           // grid
+};
+
+// Ghost-cell records (compacted list of interior cells of type 2), built once.
+struct Ghost {
+    long long cell;     // index of the ghost cell in a plane
+    long long d1;       // index of donor (IDX, IDY) of the image point; donors 2..4 = +pitch, +pitch+1, +1
+    long long d1e;      // same for the extra point (BCGFM/HBCGFM)
+    double nx, ny;      // level-set normal at the ghost
+    double phi;         // |phi| at the ghost
+    double a[4];        // image-point weights (IDW/BCGFM); Linear/RiemannBaseGFM: a[0] = chosen donor
+    double ae[4];       // extra-point weights
+    double xm, ym, xe, ye;  // image (mirror) and extra points
+    int idx, idy, idxe, idye, pick;
+    int ring;           // bit q (image) / 4+q (extra): donor q lies in the domain ring
+};
+
+struct FillArgs {
+    double* buf[2];
+    Layout L;
+    Ctl* ctl;
+    const Ghost* ghosts;
+    int nghost;
+    double* out;  // [nghost][4]
+    int bc;
+    double gam, gm1;
+    double dxg, dyg;   // spacing from the coordinates (BCGFM delta)
+    int use_cur;       // 1: operate on ctl->cur (after a step); 0: on buffer `fixed`
+    int fixed;
+    int require_ran;   // skip unless the preceding step executed
+    int ring_prev;     // read ring donors from the previous state (the ring before Global_Boundary)
+};
+
+struct EdgeArgs {
+    double* buf[2];
+    Layout L;
+    Ctl* ctl;
+    int kind;
+    double gam;
+    double inflow[4];  // conserved inflow state
+    int require_ran;
+};
+
+struct ReduceArgs {
+    const double* U;  // state buffer
+    Layout L;
+    const uint8_t* type;
+    Ctl* ctl;
+    double* mass_part;
+    double gam, gm1, dx, dy, cfl, t_end;
+    int max_iter;
+    int nblocks;
+    int reset;  // 1: set iter = 0, now_t = 0 (start of a run)
+};
+
+// kernel launchers (ibtfo_step.hip)
+hipError_t launch_step(int scheme, int rs, int lim, int prec, const StepArgs& a, hipStream_t s);
+hipError_t launch_fill(const FillArgs& a, hipStream_t s);
+hipError_t launch_scatter(const FillArgs& a, hipStream_t s);
+hipError_t launch_edge(const EdgeArgs& a, hipStream_t s);
+hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
+hipError_t launch_step_lax(const StepArgs& a, hipStream_t s);
+hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_from_input, hipStream_t s);
+
+}  // namespace ibtfo

@@ -1,0 +1,187 @@
+// Thin C++ shim with the reference's free-function signatures, so a driver
+// written against the reference (Test/*/<Case>.cpp) keeps its call sequence.
+// Each function finds the solve by the host array it is given (the OpenACC
+// present table's role) and forwards to the C ABI; a failing call prints the
+// library's message and aborts, since these signatures have no error channel.
+//
+// Mapping (reference header:line -> here):
+//   TimeAdvance   Src/Base/TimeAdvance/TimeAdvance.H:13-26  -> ibtfo_step(ctx, 1): the
+//                 fused step also applies Boundary, Global_Boundary and ComputeDt
+//   Boundary      Src/BoundaryCondition/Boundary.H:11-20    -> before the first step:
+//                 ibtfo_boundary; after a TimeAdvance: already applied
+//   Global_Boundary Src/BoundaryCondition/Boundary.H:22-29  -> applied by the step
+//   ComputeDt     Src/Base/Common/ComputeDt.H:12-23         -> dt kept by the step
+//   CaculateMass  Src/Utils/CaculateMass.H:14-23            -> mass kept by the step
+//   Scheme_Index  Src/BoundaryCondition/Scheme_Index.H:13-19 -> built with the geometry
+//   Level_Set     Src/Geometry/Level_Set.H:12-18             -> the driver's own
+//                 Level_Set_function hook fills phi on the host; classification,
+//                 normals and ghost records run on the device
+// `ndevices` must be 1 and `device` 0: the whole domain lives on one GPU.
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <vector>
+
+#include "../../include/ibtfo.h"
+
+namespace {
+struct Bound {
+    ibtfo_ctx* ctx;
+    ibtfo_config cfg;
+    double *U_OLD, *U_NEW, *XY, *SCH;
+    bool stepped;  // a TimeAdvance has run since the last host-side change
+};
+std::map<const double*, Bound*> g_present;
+
+Bound* find(const double* p, const char* who) {
+    auto it = g_present.find(p);
+    if (it == g_present.end()) {
+        fprintf(stderr, "ibtfo: %s: array not registered with ibtfo_ref_enter\n", who);
+        abort();
+    }
+    return it->second;
+}
+void check(int rc, const char* who) {
+    if (rc != IBTFO_OK) {
+        fprintf(stderr, "ibtfo: %s failed (%d): %s\n", who, rc, ibtfo_last_error());
+        abort();
+    }
+}
+void one_device(int ndevices, int device, const char* who) {
+    if (ndevices != 1 || device != 0) {
+        fprintf(stderr, "ibtfo: %s: ndevices=%d device=%d; the domain is resident on one GPU\n", who, ndevices,
+                device);
+        abort();
+    }
+}
+}  // namespace
+
+// the driver's level-set hook (Src/Geometry/Level_Set_function.H:13-19), linked from its test directory
+void Level_Set_function(char* filename, const int N_x, const int N_y, const int num_ghost_cell, double* XYCOORD,
+                        const int ndevices, const int device) __attribute__((weak));
+
+extern "C" {
+
+// replaces `acc enter data create(...)`: one solve per .inp file and host array set
+int ibtfo_ref_enter(const char* inp, double* U_OLD, double* U_NEW, double* XYCOORD, double* SCHEME_IDX) {
+    Bound* b = new Bound();
+    int rc = ibtfo_config_from_inp(inp, &b->cfg);
+    if (rc) { delete b; return rc; }
+    if ((rc = ibtfo_create(&b->cfg, &b->ctx))) { delete b; return rc; }
+    b->U_OLD = U_OLD; b->U_NEW = U_NEW; b->XY = XYCOORD; b->SCH = SCHEME_IDX;
+    for (const double* p : {(const double*)U_OLD, (const double*)U_NEW, (const double*)XYCOORD, (const double*)SCHEME_IDX})
+        if (p) g_present[p] = b;
+    return IBTFO_OK;
+}
+// replaces `acc update device(U_OLD, U_NEW)`
+int ibtfo_ref_update_device(double* U_OLD) {
+    Bound* b = find(U_OLD, "ibtfo_ref_update_device");
+    b->stepped = false;
+    return ibtfo_upload_state(b->ctx, b->U_OLD, b->U_NEW);
+}
+// replaces `acc update self(U_OLD)`
+int ibtfo_ref_update_self(double* U_OLD) {
+    Bound* b = find(U_OLD, "ibtfo_ref_update_self");
+    return ibtfo_download_state(b->ctx, b->U_OLD);
+}
+// replaces `acc exit data`
+int ibtfo_ref_exit(double* U_OLD) {
+    Bound* b = find(U_OLD, "ibtfo_ref_exit");
+    for (auto it = g_present.begin(); it != g_present.end();) it = (it->second == b) ? g_present.erase(it) : ++it;
+    int rc = ibtfo_destroy(b->ctx);
+    delete b;
+    return rc;
+}
+
+}  // extern "C"
+
+void Psy_coord(const double, const double, const double, const double, const int, const int, const int,
+               double* XYCOORD, const int ndevices, const int device) {
+    one_device(ndevices, device, "Psy_coord");
+    Bound* b = find(XYCOORD, "Psy_coord");
+    int M, N;
+    ibtfo_dims(b->ctx, &M, &N);
+    const double dx = (b->cfg.hi_x - b->cfg.lo_x) / b->cfg.N_x;
+    const double Li = -dx * b->cfg.num_ghost_cell + b->cfg.lo_x + dx / 2;
+    const double Hi = -dx * b->cfg.num_ghost_cell + b->cfg.lo_y + dx / 2;
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < M; i++) {
+            XYCOORD[((size_t)M * j + i) * 6] = Li + i * dx;
+            XYCOORD[((size_t)M * j + i) * 6 + 1] = Hi + j * dx;
+        }
+}
+
+void Level_Set(char* filename, const int N_x, const int N_y, const int ng, double* XYCOORD, const int ndevices,
+               const int device) {
+    one_device(ndevices, device, "Level_Set");
+    Bound* b = find(XYCOORD, "Level_Set");
+    if (!Level_Set_function) {
+        fprintf(stderr, "ibtfo: Level_Set: no Level_Set_function hook linked\n");
+        abort();
+    }
+    Level_Set_function(filename, N_x, N_y, ng, XYCOORD, 1, 0);
+    const size_t cells = (size_t)(N_x + 2 * ng) * (N_y + 2 * ng);
+    std::vector<double> phi(cells);
+    for (size_t q = 0; q < cells; q++) phi[q] = XYCOORD[q * 6 + 2];
+    check(ibtfo_geometry(b->ctx, IBTFO_LS_FIELD, nullptr, 0, phi.data()), "Level_Set");
+    check(ibtfo_download_geometry(b->ctx, XYCOORD), "Level_Set");
+}
+
+void Scheme_Index(const int, const int, const int, double* XYCOORD, double* SCH, const int ndevices, const int device) {
+    one_device(ndevices, device, "Scheme_Index");
+    Bound* b = find(XYCOORD, "Scheme_Index");
+    if (SCH) check(ibtfo_download_scheme_index(b->ctx, SCH), "Scheme_Index");
+}
+
+void Boundary(const int, const int, const int, const double, double* U_OLD, double*, double*, double*,
+              const int ndevices, const int device) {
+    one_device(ndevices, device, "Boundary");
+    Bound* b = find(U_OLD, "Boundary");
+    if (b->stepped) return;  // applied by the fused step
+    check(ibtfo_upload_state(b->ctx, b->U_OLD, b->U_NEW), "Boundary");
+    check(ibtfo_boundary(b->ctx), "Boundary");
+    check(ibtfo_download_state(b->ctx, b->U_OLD), "Boundary");
+}
+
+void Global_Boundary(const int, const int, const int, const double, double* U_OLD, double*, const int ndevices,
+                     const int device) {
+    one_device(ndevices, device, "Global_Boundary");
+    (void)find(U_OLD, "Global_Boundary");  // applied by the fused step
+}
+
+static void ensure_prepared(Bound* b) {
+    if (b->stepped) return;
+    check(ibtfo_upload_state(b->ctx, b->U_OLD, b->U_NEW), "prepare");
+    check(ibtfo_prepare(b->ctx, 0), "prepare");
+}
+
+void TimeAdvance(const double, const double, const int, const int, const int, const double, double, double* U_OLD,
+                 double*, double*, double*, double*, const int ndevices, const int device) {
+    one_device(ndevices, device, "TimeAdvance");
+    Bound* b = find(U_OLD, "TimeAdvance");
+    ensure_prepared(b);
+    check(ibtfo_step(b->ctx, 1), "TimeAdvance");
+    b->stepped = true;
+}
+
+void ComputeDt(const double, const double, const int, const int, const int, const double, const double,
+               double* U_OLD, double*, double* TMP_DT, const int ndevices, const int device) {
+    one_device(ndevices, device, "ComputeDt");
+    Bound* b = find(U_OLD, "ComputeDt");
+    double dt = 0;
+    if (b->stepped) check(ibtfo_status(b->ctx, nullptr, nullptr, &dt), "ComputeDt");
+    else {
+        ensure_prepared(b);
+        check(ibtfo_compute_dt(b->ctx, &dt), "ComputeDt");
+    }
+    TMP_DT[device] = dt;
+}
+
+void CaculateMass(const double, const double, const int, const int, const int, double* U_OLD, double*,
+                  double* MASS_DEVICE, const int ndevices, const int device) {
+    one_device(ndevices, device, "CaculateMass");
+    Bound* b = find(U_OLD, "CaculateMass");
+    double m = 0;
+    check(ibtfo_mass(b->ctx, &m), "CaculateMass");
+    MASS_DEVICE[device] = m;
+}

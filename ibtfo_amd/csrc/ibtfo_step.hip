@@ -1,0 +1,650 @@
+// Fused per-timestep kernels of the IBTFO Euler update for gfx950.
+//
+// k_step: one launch = TimeAdvance (reconstruction + Riemann flux + conservative
+// update, x then y) + the reference Global_Boundary ring + the ComputeDt and
+// CaculateMass reductions of the new state + the host loop's bookkeeping
+// (now_t += dt, iter++, stop test), for MUSCL / MUSCL-Hancock / Godunov with
+// HLL(=HLLC) / PVRS / Steger-Warming and minmod / VanLeer / VanAlbada.
+//
+// Work decomposition: a 64-lane block owns a strip of 63 interior columns and
+// `rows` interior rows and marches upward through it. Lane t holds column
+// c = cb - 1 + t: it keeps a 4-row window of its column in registers (for the
+// y reconstruction), stages its row value in LDS for its x neighbours, computes
+// the x-face flux at c+1/2 and the y-face flux at j+1/2 once each, takes the
+// x-face at c-1/2 from lane t-1 through LDS and the y-face at j-1/2 from its own
+// previous iteration. Each state value is read from HBM once per step (halo
+// re-reads hit L2) and written once.
+#include "ibtfo_internal.h"
+
+namespace ibtfo {
+
+__device__ __forceinline__ double ld_agent(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_agent(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double w = __shfl_xor(v, o, 64);
+        v = (v < w) ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ComputeDt.cpp:55-63 contribution of one fluid cell (std::max keeps the old
+// maximum when the candidate is NaN).
+__device__ __forceinline__ void dt_terms(double gam, double gm1, const double* U, double& amax, double& umax,
+                                         double& vmax, int& bad) {
+    double rho = U[0], u = U[1] / U[0], v = U[2] / U[0];
+    double p = gm1 * (U[3] - 0.5 * rho * (u * u + v * v));
+    double a = ib_sqrt(gam * p / rho);
+    amax = (amax < a) ? a : amax;
+    double au = ib_fabs(u), av = ib_fabs(v);
+    umax = (umax < au) ? au : umax;
+    vmax = (vmax < av) ? av : vmax;
+    bad |= !(rho > 0.0 && p > 0.0);
+}
+
+// Per-block reduction and the last block's finish: dt for the next step,
+// time / iteration / stop flag, mass, log entry. `advance` = 0 for the
+// stand-alone ComputeDt (no time step taken).
+__device__ void finish_block(Ctl* ctl, double* mass_part, int nblocks, int bid, double amax, double umax,
+                             double vmax, double msum, int bad, double cfl, double dx, double t_end, int max_iter,
+                             int advance, int reset) {
+    __shared__ int s_last;
+    amax = wave_max(amax);
+    umax = wave_max(umax);
+    vmax = wave_max(vmax);
+    msum = wave_sum(msum);
+    bad = __any(bad);
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+        if (amax > 0.0) atomicMax(&ctl->amax, (unsigned long long)__double_as_longlong(amax));
+        if (umax > 0.0) atomicMax(&ctl->umax, (unsigned long long)__double_as_longlong(umax));
+        if (vmax > 0.0) atomicMax(&ctl->vmax, (unsigned long long)__double_as_longlong(vmax));
+        if (bad) atomicOr(&ctl->status, 1);
+        mass_part[bid] = msum;
+        __threadfence();
+        unsigned int prev = atomicAdd(&ctl->arrive, 1u);
+        s_last = (prev == (unsigned int)nblocks - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    double m = 0.0;
+    for (int q = lane; q < nblocks; q += 64) m += ld_agent(mass_part + q);
+    m = wave_sum(m);
+    if (lane == 0) {
+        double A = __longlong_as_double((long long)ld_agent(&ctl->amax));
+        double Um = __longlong_as_double((long long)ld_agent(&ctl->umax));
+        double Vm = __longlong_as_double((long long)ld_agent(&ctl->vmax));
+        double dt = cfl * dx / (A + ((Um < Vm) ? Vm : Um));
+        ctl->dt = dt;
+        if (advance) {
+            double t = ctl->now_t + dt;
+            int it = ctl->iter + 1;
+            int slot = ctl->iter % LOGCAP;
+            ctl->log_dt[slot] = dt;
+            ctl->log_t[slot] = t;
+            ctl->log_mass[slot] = m;
+            ctl->mass_now = m;
+            ctl->now_t = t;
+            ctl->iter = it;
+            ctl->done = !(t < t_end && it < max_iter);
+            ctl->cur ^= 1;
+            ctl->ran = 1;
+        } else {
+            ctl->mass_now = m;
+            if (reset) {
+                ctl->now_t = 0.0;
+                ctl->iter = 0;
+                ctl->done = !(0.0 < t_end && 0 < max_iter);
+                ctl->ran = 0;
+            }
+        }
+        ctl->amax = 0ull;
+        ctl->umax = 0ull;
+        ctl->vmax = 0ull;
+        __threadfence();
+        ctl->arrive = 0u;
+    }
+}
+
+template <int SCH, int RS, int LIM>
+__device__ __forceinline__ void xface(const GasT<double>& g, double dt, double dx, int quirks, const double* u0,
+                                      const double* u1, const double* u2, const double* u3, double* F) {
+    double UL[4], UR[4];
+    if (SCH == SCH_GODUNOV) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) { UL[k] = u1[k]; UR[k] = u2[k]; }
+    } else {
+        recon<LIM>(u0, u1, u2, u3, UL, UR, quirks != 0);
+    }
+    if (SCH == SCH_MHM) {  // MHM_Scheme.cpp:61-96 (same increment on both sides of the face)
+        double FL[4], FR[4];
+        flux_of<false>(g.gm1, UL, FL);
+        flux_of<false>(g.gm1, UR, FR);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double inc = 0.5 * (FL[k] - FR[k]) * dt / dx;
+            UL[k] = UL[k] + inc;
+            UR[k] = UR[k] + inc;
+        }
+    }
+    if (RS == RS_SW) sw_face<false>(g, UL, UR, F);
+    else if (RS == RS_PVRS) { bool s; pvrs_face<false>(g, UL, UR, F, s); }
+    else hll_face<false>(g, UL, UR, F);
+}
+
+// y face at j+1/2 of column c; Fx is this cell's x-face flux in the same row,
+// which is what the reference leaves in the aliased G_OLD slot when its y solver
+// stores nothing (HLL/Riemann_solver.cpp:224-226, PVRS/Riemann_solver.cpp:550).
+template <int SCH, int RS, int LIM>
+__device__ __forceinline__ void yface(const GasT<double>& g, double dt, double dy, int quirks, const double* u0,
+                                      const double* u1, const double* u2, const double* u3, const double* Fx,
+                                      double* G) {
+    double UD[4], UU[4];
+    if (SCH == SCH_GODUNOV) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) { UD[k] = u1[k]; UU[k] = u2[k]; }
+    } else {
+        recon<LIM>(u0, u1, u2, u3, UD, UU, quirks != 0);
+    }
+    if (SCH == SCH_MHM) {  // MHM_Scheme.cpp:148-183
+        double GD[4], GU[4];
+        flux_of<true>(g.gm1, UD, GD);
+        flux_of<true>(g.gm1, UU, GU);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double inc = 0.5 * (GD[k] - GU[k]) * dt / dy;
+            UD[k] = UD[k] + inc;
+            UU[k] = UU[k] + inc;
+        }
+    }
+    if (RS == RS_SW) {
+        sw_face<true>(g, UD, UU, G);
+    } else if (RS == RS_PVRS) {
+        bool stray;
+        pvrs_face<true>(g, UD, UU, G, stray);
+        if (stray && quirks) G[3] = Fx[3];
+    } else {
+        bool up = hll_face<true>(g, UD, UU, G);
+        if (up && quirks) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) G[k] = Fx[k];
+        }
+    }
+}
+
+template <int SCH, int RS, int LIM>
+__global__ __launch_bounds__(64) void k_step(StepArgs a) {
+    Ctl* ctl = a.ctl;
+    const int lane = threadIdx.x;
+    if (ctl->done) {
+        if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0) ctl->ran = 0;
+        return;
+    }
+    const Layout L = a.L;
+    const int cur = ctl->cur;
+    const double dt = ctl->dt;
+    const double* __restrict__ Ui = a.buf[cur];
+    double* __restrict__ Uo = a.buf[cur ^ 1];
+    const long long PL = L.plane;
+    GasT<double> g{a.gam, a.gm1, a.gp1, a.c1};
+    const double dx = a.dx, dy = a.dy;
+    const int ng = L.ng, nx = L.nx, ny = L.ny;
+
+    const int cb = ng + STRIP_W * blockIdx.x;  // first interior column of the strip
+    const int c = cb - 1 + lane;
+    const int cl = min(max(c, 0), L.M - 1);
+    const int j0 = ng + a.rows * blockIdx.y;
+    const int j1 = min(j0 + a.rows, ny + ng);
+    // lanes 0..2 also stage the strip's outer halo columns cb-2, cb+63, cb+64
+    const int ecol = min(max(lane == 0 ? cb - 2 : cb + 62 + lane, 0), L.M - 1);
+    const int eidx = lane == 0 ? 0 : 64 + lane;
+
+    __shared__ double rb[2][4][68];
+    __shared__ double fb[4][64];
+
+    // reference Global_Boundary corners (Test/Sod/Global_Boudary.cpp:62-75 applied to
+    // ring rows): U_OLD(i<ng or i>=N_x+ng, j ring) = U_NEW(ng or N_x+ng-1, j)
+    if (a.edge_fused && blockIdx.x == 0 && blockIdx.y == 0 && lane < 4 * ng * ng) {
+        int side = lane / (2 * ng * ng);
+        int rem = lane % (2 * ng * ng);
+        int ri = rem / ng, cc = rem % ng;
+        int row = ri < ng ? ri : ny + ri;
+        int col = side == 0 ? cc : nx + ng + cc;
+        int scol = side == 0 ? ng : nx + ng - 1;
+        long long o = L.at(col, row);
+        for (int k = 0; k < 4; k++) {
+            double v = a.corner_from_input ? Ui[L.at(scol, row) + k * PL] : a.corner[(side * 2 * ng + ri) * 4 + k];
+            Uo[o + k * PL] = v;
+        }
+    }
+
+    double w[4][4];
+    double ex[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        long long o = L.at(cl, j0 - 2 + r);
+#pragma unroll
+        for (int k = 0; k < 4; k++) w[r][k] = Ui[o + k * PL];
+    }
+    if (lane < 3) {
+        long long o = L.at(ecol, j0 - 1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) ex[k] = Ui[o + k * PL];
+    }
+
+    double Gp[4] = {0, 0, 0, 0};
+    double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
+    int bad = 0;
+    const bool upd_col = lane >= 1 && c < nx + ng;
+
+    for (int j = j0 - 1; j < j1; j++) {
+        const int b = j & 1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) rb[b][k][lane + 1] = w[1][k];
+        if (lane < 3) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) rb[b][k][eidx] = ex[k];
+        }
+        __syncthreads();
+        // prefetch the next window row and next extra column
+        double nw[4], nex[4] = {0, 0, 0, 0};
+        {
+            long long o = L.at(cl, min(j + 3, L.N - 1));
+#pragma unroll
+            for (int k = 0; k < 4; k++) nw[k] = Ui[o + k * PL];
+            if (lane < 3) {
+                long long oe = L.at(ecol, min(j + 1, L.N - 1));
+#pragma unroll
+                for (int k = 0; k < 4; k++) nex[k] = Ui[oe + k * PL];
+            }
+        }
+        double xm[4], xp[4], xp2[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            xm[k] = rb[b][k][lane];
+            xp[k] = rb[b][k][lane + 2];
+            xp2[k] = rb[b][k][lane + 3];
+        }
+        double F[4], G[4];
+        xface<SCH, RS, LIM>(g, dt, dx, a.quirks, xm, w[1], xp, xp2, F);
+        yface<SCH, RS, LIM>(g, dt, dy, a.quirks, w[0], w[1], w[2], w[3], F, G);
+
+        if (j >= j0) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) fb[k][lane] = F[k];
+            __syncthreads();
+            if (upd_col) {
+                double un[4];
+                const long long o = L.at(c, j);
+                const bool fluid = a.type[o] == 0;
+                if (fluid) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        double Fm = fb[k][lane - 1];
+                        double v = w[1][k] - dt * (F[k] - Fm) / dx;
+                        un[k] = v - dt * (G[k] - Gp[k]) / dy;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) un[k] = w[1][k];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) Uo[o + k * PL] = un[k];
+                if (a.edge_fused) {
+                    // Test/Sod/Global_Boudary.cpp:24-59 (rows) and :62-75 (columns)
+                    if (j < 2 * ng) Uo[L.at(c, 2 * ng - 1 - j) + 2 * PL] = -un[2];
+                    if (j == 2 * ng - 1)
+                        for (int r = 0; r < ng; r++) {
+                            long long q = L.at(c, r);
+                            Uo[q] = un[0]; Uo[q + PL] = un[1]; Uo[q + 3 * PL] = un[3];
+                        }
+                    if (j >= ny) Uo[L.at(c, 2 * (ny + ng) - 1 - j) + 2 * PL] = -un[2];
+                    if (j == ny + ng - 1)
+                        for (int r = 0; r < ng; r++) {
+                            long long q = L.at(c, ny + ng + r);
+                            Uo[q] = un[0]; Uo[q + PL] = un[1]; Uo[q + 3 * PL] = un[3];
+                        }
+                    if (c == ng)
+                        for (int r = 0; r < ng; r++) {
+                            long long q = L.at(r, j);
+                            for (int k = 0; k < 4; k++) Uo[q + k * PL] = un[k];
+                        }
+                    if (c == nx + ng - 1)
+                        for (int r = 0; r < ng; r++) {
+                            long long q = L.at(nx + ng + r, j);
+                            for (int k = 0; k < 4; k++) Uo[q + k * PL] = un[k];
+                        }
+                }
+                if (fluid) {
+                    dt_terms(a.gam, a.gm1, un, amax, umax, vmax, bad);
+                    msum += un[0] * dx * dy;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            Gp[k] = G[k];
+            w[0][k] = w[1][k];
+            w[1][k] = w[2][k];
+            w[2][k] = w[3][k];
+            w[3][k] = nw[k];
+            ex[k] = nex[k];
+        }
+    }
+    finish_block(ctl, a.mass_part, a.nbx * a.nby, blockIdx.y * a.nbx + blockIdx.x, amax, umax, vmax, msum, bad,
+                 a.cfl, dx, a.t_end, a.max_iter, 1, 0);
+}
+
+// Lax-Friedrichs, Lax_Scheme.cpp:27-62: U_new = average of the 4 neighbours minus
+// central flux differences. Same strip decomposition; lane t owns column cb-1+t.
+__global__ __launch_bounds__(64) void k_step_lax(StepArgs a) {
+    Ctl* ctl = a.ctl;
+    const int lane = threadIdx.x;
+    if (ctl->done) {
+        if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0) ctl->ran = 0;
+        return;
+    }
+    const Layout L = a.L;
+    const int cur = ctl->cur;
+    const double dt = ctl->dt;
+    const double* __restrict__ Ui = a.buf[cur];
+    double* __restrict__ Uo = a.buf[cur ^ 1];
+    const long long PL = L.plane;
+    const int ng = L.ng, nx = L.nx, ny = L.ny;
+    const int cb = ng + STRIP_W * blockIdx.x;
+    const int c = cb - 1 + lane;
+    const int j0 = ng + a.rows * blockIdx.y;
+    const int j1 = min(j0 + a.rows, ny + ng);
+    double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
+    int bad = 0;
+    if (lane >= 1 && c < nx + ng) {
+        for (int j = j0; j < j1; j++) {
+            const long long o = L.at(c, j);
+            double un[4];
+            if (a.type[o] == 0) {
+                double U[4][4], Fxm[4], Fxp[4], Gym[4], Gyp[4];
+                const long long nb[4] = {L.at(c - 1, j), L.at(c + 1, j), L.at(c, j + 1), L.at(c, j - 1)};
+                for (int q = 0; q < 4; q++)
+                    for (int k = 0; k < 4; k++) U[q][k] = Ui[nb[q] + k * PL];
+                flux_of<false>(a.gm1, U[0], Fxm);
+                flux_of<false>(a.gm1, U[1], Fxp);
+                flux_of<true>(a.gm1, U[2], Gyp);
+                flux_of<true>(a.gm1, U[3], Gym);
+                for (int k = 0; k < 4; k++)
+                    un[k] = 0.25 * (U[0][k] + U[1][k] + U[2][k] + U[3][k]) - 0.5 * dt * (Fxp[k] - Fxm[k]) / a.dx -
+                            0.5 * dt * (Gyp[k] - Gym[k]) / a.dy;
+                dt_terms(a.gam, a.gm1, un, amax, umax, vmax, bad);
+                msum += un[0] * a.dx * a.dy;
+            } else {
+                for (int k = 0; k < 4; k++) un[k] = Ui[o + k * PL];
+            }
+            for (int k = 0; k < 4; k++) Uo[o + k * PL] = un[k];
+        }
+    }
+    // the reference edge ring for Lax runs as the separate edge kernel
+    finish_block(ctl, a.mass_part, a.nbx * a.nby, blockIdx.y * a.nbx + blockIdx.x, amax, umax, vmax, msum, bad,
+                 a.cfl, a.dx, a.t_end, a.max_iter, 1, 0);
+}
+
+// ComputeDt + CaculateMass of the current state (ComputeDt.cpp:46-67,
+// CaculateMass.cpp:27-38); with `reset`, also starts the run clock.
+__global__ __launch_bounds__(64) void k_reduce(ReduceArgs a) {
+    const Layout L = a.L;
+    const int nint = L.nx * L.ny;
+    double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
+    int bad = 0;
+    for (int q = blockIdx.x * 64 + threadIdx.x; q < nint; q += a.nblocks * 64) {
+        int i = L.ng + q % L.nx, j = L.ng + q / L.nx;
+        long long o = L.at(i, j);
+        if (a.type[o] == 0) {
+            double U[4] = {a.U[o], a.U[o + L.plane], a.U[o + 2 * L.plane], a.U[o + 3 * L.plane]};
+            dt_terms(a.gam, a.gm1, U, amax, umax, vmax, bad);
+            msum += U[0] * a.dx * a.dy;
+        }
+    }
+    finish_block(a.ctl, a.mass_part, a.nblocks, blockIdx.x, amax, umax, vmax, msum, bad, a.cfl, a.dx, a.t_end,
+                 a.max_iter, 0, a.reset);
+}
+
+// Ghost-cell fill, phase 1: values from the donors of U_NEW (the state right
+// after the update, before any ghost is rewritten) into a compact buffer.
+// IDWBaseGFM/GFM_Bound.cpp:65-106, LinearBaseGFM/GFM_Bound.cpp:58-575,
+// BCGFM/BCGFM_Bound.cpp:81-175, HBCGFM/HBCGFM_Bound.cpp:163-178.
+__global__ void k_fill(FillArgs a) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.require_ran && !a.ctl->ran) return;
+    if (gi >= a.nghost) return;
+    const Ghost gh = a.ghosts[gi];
+    const int cur = a.use_cur ? a.ctl->cur : a.fixed;
+    const double* U = a.buf[cur];
+    // Boundary runs before Global_Boundary in the reference, so a donor in the
+    // domain ring still holds the ring of the previous state
+    const double* Uring = a.ring_prev ? a.buf[cur ^ 1] : U;
+    const long long PL = a.L.plane, P = a.L.pitch;
+    const double gm1 = a.gm1;
+    auto prim = [&](long long d, bool ring, double& r, double& u, double& v, double& p) {
+        const double* S = ring ? Uring : U;
+        double q[4] = {S[d], S[d + PL], S[d + 2 * PL], S[d + 3 * PL]};
+        primitives(gm1, q, r, u, v, p);
+    };
+    auto idw = [&](long long d1, int rbits, const double* w, double& R, double& Uu, double& V, double& Pp) {
+        double r[4], u[4], v[4], p[4];
+        const long long dd[4] = {d1, d1 + P, d1 + P + 1, d1 + 1};
+        for (int q = 0; q < 4; q++) prim(dd[q], (rbits >> q) & 1, r[q], u[q], v[q], p[q]);
+        double s = w[0] + w[1] + w[2] + w[3];
+        R = (w[0] * r[0] + w[1] * r[1] + w[2] * r[2] + w[3] * r[3]) / s;
+        Uu = (w[0] * u[0] + w[1] * u[1] + w[2] * u[2] + w[3] * u[3]) / s;
+        V = (w[0] * v[0] + w[1] * v[1] + w[2] * v[2] + w[3] * v[3]) / s;
+        Pp = (w[0] * p[0] + w[1] * p[1] + w[2] * p[2] + w[3] * p[3]) / s;
+    };
+    const double n_x = gh.nx, n_y = gh.ny;
+    double rw, uw, vw, pw;
+    if (a.bc == 1) {  // IDWBaseGFM
+        idw(gh.d1, gh.ring & 15, gh.a, rw, uw, vw, pw);
+        double u_n = uw * n_x + vw * n_y;
+        double u_t = uw * n_y - vw * n_x;
+        u_n = -u_n;
+        uw = u_n * n_x + u_t * n_y;
+        vw = u_n * n_y - u_t * n_x;
+    } else if (a.bc == 2 || a.bc == 3) {  // Linear/RiemannBaseGFM: one donor
+        int pick = (int)gh.a[0];
+        if (pick >= 0) {
+            const long long dd[4] = {gh.d1, gh.d1 + P, gh.d1 + P + 1, gh.d1 + 1};
+            prim(dd[pick], (gh.ring >> pick) & 1, rw, uw, vw, pw);
+        } else {
+            rw = 0; uw = 0; vw = 0; pw = 0;
+        }
+        double u_n = uw * n_x + vw * n_y;
+        double u_t = uw * n_y - vw * n_x;
+        u_n = -u_n;
+        uw = u_n * n_x + u_t * n_y;
+        vw = u_n * n_y - u_t * n_x;
+    } else {  // BCGFM / HBCGFM
+        double delta = ib_sqrt(a.dxg * a.dxg + a.dyg * a.dyg) / 2;
+        double R_E = delta, R_G = gh.phi;
+        double ri, ui, vi, pi_, re, ue, ve, pe;
+        idw(gh.d1, gh.ring & 15, gh.a, ri, ui, vi, pi_);
+        double uni = ui * n_x + vi * n_y, uti = ui * n_y - vi * n_x;
+        idw(gh.d1e, (gh.ring >> 4) & 15, gh.ae, re, ue, ve, pe);
+        double une = ue * n_x + ve * n_y, ute = ue * n_y - ve * n_x;
+        double un, ut;
+        if (a.bc == 5 && R_G > delta) {
+            rw = ri; pw = pi_; ut = uti; un = -uni;
+        } else {
+            double A = ((delta + R_E) * (delta + R_E) - R_G * R_G), B = (delta * delta - R_G * R_G);
+            double D = (R_E * R_E + 2 * delta * R_E);
+            rw = (ri * A - re * B) / D;
+            pw = (pi_ * A - pe * B) / D;
+            ut = (uti * A - ute * B) / D;
+            un = -(uni * A - une * B) / D;
+        }
+        uw = un * n_x + ut * n_y;
+        vw = un * n_y - ut * n_x;
+    }
+    double* o = a.out + 4 * (long long)gi;
+    o[0] = rw;
+    o[1] = rw * uw;
+    o[2] = rw * vw;
+    o[3] = pw / gm1 + 0.5 * rw * (uw * uw + vw * vw);
+}
+
+// Ghost-cell fill, phase 2: write the values into the current state.
+__global__ void k_scatter(FillArgs a) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.require_ran && !a.ctl->ran) return;
+    if (gi >= a.nghost) return;
+    double* U = a.buf[a.use_cur ? a.ctl->cur : a.fixed];
+    const long long c = a.ghosts[gi].cell, PL = a.L.plane;
+    const double* v = a.out + 4 * (long long)gi;
+    U[c] = v[0];
+    U[c + PL] = v[1];
+    U[c + 2 * PL] = v[2];
+    U[c + 3 * PL] = v[3];
+}
+
+// Ring of the current state for the edge policies that are not fused into the
+// step (the reference policy for Lax; periodic; inflow). Reads the interior of
+// the current state, writes its ring; one thread per ring cell.
+__global__ void k_edge(EdgeArgs a) {
+    if (a.require_ran && !a.ctl->ran) return;
+    const Layout L = a.L;
+    const int ng = L.ng, nx = L.nx, ny = L.ny;
+    const int nring = L.M * L.N - nx * ny;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nring) return;
+    // enumerate ring cells: bottom rows, top rows, then left/right columns of interior rows
+    int i, j;
+    int nb = L.M * ng;
+    if (q < nb) { j = q / L.M; i = q % L.M; }
+    else if (q < 2 * nb) { int r = q - nb; j = ny + ng + r / L.M; i = r % L.M; }
+    else { int r = q - 2 * nb; j = ng + r / (2 * ng); int s = r % (2 * ng); i = s < ng ? s : nx + s; }
+    double* U = a.buf[a.ctl->cur];
+    const long long PL = L.plane;
+    double v[4];
+    if (a.kind == 1) {  // periodic
+        int si = i < ng ? i + nx : (i >= nx + ng ? i - nx : i);
+        int sj = j < ng ? j + ny : (j >= ny + ng ? j - ny : j);
+        long long s = L.at(si, sj);
+        for (int k = 0; k < 4; k++) v[k] = U[s + k * PL];
+    } else if (a.kind == 2) {  // inflow left, zero gradient elsewhere
+        if (i < ng) {
+            for (int k = 0; k < 4; k++) v[k] = a.inflow[k];
+        } else {
+            int si = i >= nx + ng ? nx + ng - 1 : i;
+            int sj = j < ng ? ng : (j >= ny + ng ? ny + ng - 1 : j);
+            long long s = L.at(si, sj);
+            for (int k = 0; k < 4; k++) v[k] = U[s + k * PL];
+        }
+    } else {
+        return;  // the reference policy is written by the step kernel
+    }
+    long long o = L.at(i, j);
+    for (int k = 0; k < 4; k++) U[o + k * PL] = v[k];
+}
+
+// Reference Global_Boundary as a separate pass (used with Lax, whose step
+// kernel does not write the ring): reads U_NEW = the new state (ring of U_NEW
+// taken from the previous state for the corners), writes the ring.
+__global__ void k_edge_ref(EdgeArgs a, const double* corner, int corner_from_input) {
+    if (a.require_ran && !a.ctl->ran) return;
+    const Layout L = a.L;
+    const int ng = L.ng, nx = L.nx, ny = L.ny;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nring = L.M * L.N - nx * ny;
+    if (q >= nring) return;
+    int i, j;
+    int nb = L.M * ng;
+    if (q < nb) { j = q / L.M; i = q % L.M; }
+    else if (q < 2 * nb) { int r = q - nb; j = ny + ng + r / L.M; i = r % L.M; }
+    else { int r = q - 2 * nb; j = ng + r / (2 * ng); int s = r % (2 * ng); i = s < ng ? s : nx + s; }
+    const int cur = a.ctl->cur;
+    double* U = a.buf[cur];
+    const double* Uprev = a.buf[cur ^ 1];
+    const long long PL = L.plane;
+    const long long o = L.at(i, j);
+    if (i >= ng && i < nx + ng) {
+        long long s = (j < ng) ? L.at(i, 2 * ng - 1) : L.at(i, ny + ng - 1);
+        long long s2 = (j < ng) ? L.at(i, 2 * ng - j - 1) : L.at(i, 2 * (ny + ng) - 1 - j);
+        U[o] = U[s]; U[o + PL] = U[s + PL]; U[o + 3 * PL] = U[s + 3 * PL];
+        U[o + 2 * PL] = -U[s2 + 2 * PL];
+    } else {
+        int scol = i < ng ? ng : nx + ng - 1;
+        bool ringrow = j < ng || j >= ny + ng;
+        if (!ringrow) {
+            long long s = L.at(scol, j);
+            for (int k = 0; k < 4; k++) U[o + k * PL] = U[s + k * PL];
+        } else {
+            int side = i < ng ? 0 : 1, ri = j < ng ? j : j - ny;
+            for (int k = 0; k < 4; k++)
+                U[o + k * PL] = corner_from_input ? Uprev[L.at(scol, j) + k * PL] : corner[(side * 2 * ng + ri) * 4 + k];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+template <int SCH, int RS, int LIM>
+static hipError_t go(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((k_step<SCH, RS, LIM>), dim3(a.nbx, a.nby), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+template <int SCH, int RS>
+static hipError_t go_l(int lim, const StepArgs& a, hipStream_t s) {
+    if (lim == LIM_MINMOD) return go<SCH, RS, LIM_MINMOD>(a, s);
+    if (lim == LIM_VANLEER) return go<SCH, RS, LIM_VANLEER>(a, s);
+    return go<SCH, RS, LIM_VANALBADA>(a, s);
+}
+template <int SCH>
+static hipError_t go_r(int rs, int lim, const StepArgs& a, hipStream_t s) {
+    if (rs == RS_SW) return go_l<SCH, RS_SW>(lim, a, s);
+    if (rs == RS_PVRS) return go_l<SCH, RS_PVRS>(lim, a, s);
+    return go_l<SCH, RS_HLL>(lim, a, s);
+}
+hipError_t launch_step(int scheme, int rs, int lim, int prec, const StepArgs& a, hipStream_t s) {
+    (void)prec;
+    if (scheme == SCH_LAX) return launch_step_lax(a, s);
+    if (scheme == SCH_MHM) return go_r<SCH_MHM>(rs, lim, a, s);
+    if (scheme == SCH_GODUNOV) return go_r<SCH_GODUNOV>(rs, LIM_MINMOD, a, s);
+    return go_r<SCH_MUSCL>(rs, lim, a, s);
+}
+hipError_t launch_step_lax(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_step_lax, dim3(a.nbx, a.nby), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
+    if (a.nghost == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3((a.nghost + 127) / 128), dim3(128), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_scatter(const FillArgs& a, hipStream_t s) {
+    if (a.nghost == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3((a.nghost + 127) / 128), dim3(128), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_edge(const EdgeArgs& a, hipStream_t s) {
+    int nring = a.L.M * a.L.N - a.L.nx * a.L.ny;
+    hipLaunchKernelGGL(k_edge, dim3((nring + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_from_input, hipStream_t s) {
+    int nring = a.L.M * a.L.N - a.L.nx * a.L.ny;
+    hipLaunchKernelGGL(k_edge_ref, dim3((nring + 255) / 256), dim3(256), 0, s, a, corner, corner_from_input);
+    return hipGetLastError();
+}
+hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce, dim3(a.nblocks), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace ibtfo

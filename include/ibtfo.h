@@ -1,0 +1,169 @@
+/*
+ * ibtfo.h -- C ABI of the MI355X-native IBTFO per-timestep Euler update.
+ *
+ * One opaque context per solve. All state lives in device memory (SoA planes,
+ * one FP64 plane per conserved variable, plus a uint8 cell-type plane); the
+ * host exchanges it in the reference's AoS layout. Every call returns an int
+ * status (IBTFO_OK = 0). No call allocates unless its comment says so.
+ *
+ * Reference interfaces replaced (reference-relative paths):
+ *   ibtfo_config_from_inp   ParamReader::open/numbers/strings   Src/Utils/ParamReader.cpp:14-68
+ *                           + Makefile slot selection            Test/Sod/Makefile:1-9
+ *   ibtfo_create/destroy    acc enter/exit data create/copyout   Test/Sod/Sod.cpp:74-78, :187-191
+ *   ibtfo_upload_state      acc update device                    Test/Sod/Sod.cpp:88-92
+ *   ibtfo_download_*        acc update self                      Test/Sod/Sod.cpp:100-104
+ *   ibtfo_geometry          Psy_coord + Level_Set(+hook) + Scheme_Index
+ *                                       Src/Geometry/Psy_coord.H:11-20, Src/Geometry/Level_Set.H:12-18,
+ *                                       Src/Geometry/Level_Set_function.H:13-19,
+ *                                       Src/BoundaryCondition/Scheme_Index.H:13-19
+ *   ibtfo_initialize        Initialize (user hook)               Src/Base/Common/Initialize.H:12-23
+ *   ibtfo_boundary          Boundary                             Src/BoundaryCondition/Boundary.H:11-20
+ *   ibtfo_compute_dt        ComputeDt                            Src/Base/Common/ComputeDt.H:12-23
+ *   ibtfo_step              TimeAdvance + Boundary + Global_Boundary + ComputeDt (+ CaculateMass)
+ *                                       Src/Base/TimeAdvance/TimeAdvance.H:13-26, Src/BoundaryCondition/Boundary.H:11-29,
+ *                                       Test/Sod/Sod.cpp:154-177
+ *   ibtfo_mass              CaculateMass                         Src/Utils/CaculateMass.H:14-23
+ */
+#ifndef IBTFO_H_
+#define IBTFO_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define IBTFO_OK 0
+#define IBTFO_E_ARG 1       /* bad argument or unsupported combination */
+#define IBTFO_E_HIP 2       /* HIP runtime error */
+#define IBTFO_E_NOMEM 3     /* device or host allocation failed */
+#define IBTFO_E_IO 4        /* input file could not be read */
+#define IBTFO_E_STATE 5     /* call out of order (e.g. step before geometry) */
+#define IBTFO_E_NONFINITE 6 /* non-finite or non-positive density/pressure seen */
+
+/* slot values use the reference Makefile's names (Test/Sod/Makefile:1-5) */
+enum ibtfo_scheme { IBTFO_LAX = 0, IBTFO_MUSCL = 1, IBTFO_MHM = 2, IBTFO_GODUNOV = 3 };
+enum ibtfo_riemann { IBTFO_HLL = 0, IBTFO_HLLC = 1, IBTFO_PVRS = 2, IBTFO_STEGER_WARMING = 3 };
+enum ibtfo_limiter { IBTFO_MINMOD = 0, IBTFO_VANLEER = 1, IBTFO_VANALBADA = 2 };
+enum ibtfo_boundary {
+    IBTFO_REGULAR = 0, IBTFO_IDW_GFM = 1, IBTFO_LINEAR_GFM = 2, IBTFO_RIEMANN_GFM = 3,
+    IBTFO_BCGFM = 4, IBTFO_HBCGFM = 5
+};
+enum ibtfo_time { IBTFO_SINGLESTEP = 0, IBTFO_RK2 = 1 };
+/* Global_Boundary policies: the reference hook (Test/Sod/Global_Boudary.cpp:24-75),
+ * periodic in x and y, supersonic inflow at x = lo_x with zero-gradient elsewhere. */
+enum ibtfo_edge { IBTFO_EDGE_REFERENCE = 0, IBTFO_EDGE_PERIODIC = 1, IBTFO_EDGE_INFLOW = 2 };
+enum ibtfo_precision { IBTFO_FP64 = 0, IBTFO_MIXED = 1 };
+
+/* level-set hooks (Level_Set_function user hook) */
+enum ibtfo_levelset_kind {
+    IBTFO_LS_NONE = 0,     /* phi = 1 everywhere: Test/Sod/Level_Set_function.cpp:26-35 */
+    IBTFO_LS_CHANNEL = 1,  /* inclined channel, prm {angle_deg, AxisD}: Test/InclineSod/Level_Set_function.cpp:8-56 */
+    IBTFO_LS_ANNULUS = 2,  /* annulus about (0.5,0.5), prm {f0, f1}: Test/Vortex/Level_Set_function.cpp:15-40 */
+    IBTFO_LS_FIELD = 3,    /* phi given per cell (host array, reference AoS cell order) */
+    IBTFO_LS_ELLIPSE = 4   /* solid ellipse, prm {xc, yc, a, b, angle_deg}; phi = signed distance estimate */
+};
+
+/* initialisers (Initialize user hook) */
+enum ibtfo_init_kind {
+    IBTFO_INIT_SOD_ROWS = 0,  /* Test/Sod/Initialize.cpp:29-52, prm {rho,u,v,p}_L {rho,u,v,p}_R */
+    IBTFO_INIT_INCLINE = 1,   /* Test/InclineSod/Initialize.cpp:21-72, prm {angle_deg, L state, R state} */
+    IBTFO_INIT_VORTEX = 2,    /* Test/Vortex/Initialize.cpp:15-98, prm {fa, fw} */
+    IBTFO_INIT_SOD_X = 3,     /* Sod split at x = prm[0], prm[1..8] = L, R states */
+    IBTFO_INIT_UNIFORM = 4,   /* prm {rho,u,v,p} */
+    IBTFO_INIT_SHU_VORTEX = 5 /* isentropic vortex, prm {x0, y0, beta, rho_inf, u_inf, v_inf, p_inf} */
+};
+
+typedef struct ibtfo_config {
+    /* .inp numbers (Test/Sod/Sod.cpp:34-48) */
+    int N_x, N_y, num_ghost_cell, max_iter;
+    double lo_x, lo_y, hi_x, hi_y;
+    double Psy_time, CFL_number, gamma;
+    /* slot selection */
+    int scheme, riemann, limiter, boundary, time, edge, precision;
+    /* 1: reproduce the reference arithmetic including its defects (HLL y-branch
+     * aliasing, VanLeer macro expansion, PVRS y store); 0: corrected forms */
+    int quirks;
+    double edge_state[4]; /* IBTFO_EDGE_INFLOW: inflow (rho, u, v, p) */
+} ibtfo_config;
+
+typedef struct ibtfo_ctx ibtfo_ctx;
+
+/* Fill *cfg with defaults (the reference driver's Params::get defaults) and then
+ * the numbers and slot strings found in an .inp file. */
+int ibtfo_config_default(ibtfo_config* cfg);
+int ibtfo_config_from_inp(const char* path, ibtfo_config* cfg);
+/* Numeric value of one key as the reference's ParamReader::numbers() parses it;
+ * returns 1 if present, 0 if missing (then *value is untouched), <0 on I/O error. */
+int ibtfo_inp_number(const char* path, const char* key, double* value);
+/* String value of one key as ParamReader::strings() returns it (inline comments
+ * included); returns length, 0 if missing, <0 on error. */
+int ibtfo_inp_string(const char* path, const char* key, char* buf, int cap);
+
+/* Allocates all device memory of the solve (state x2, cell types, geometry,
+ * scheme index, ghost list, control block) on the current HIP device. */
+int ibtfo_create(const ibtfo_config* cfg, ibtfo_ctx** out);
+int ibtfo_destroy(ibtfo_ctx* ctx);
+/* Run on this HIP stream (hipStream_t, default priority); NULL = the context's own stream. */
+int ibtfo_set_stream(ibtfo_ctx* ctx, void* stream);
+/* sizes: M = N_x + 2 ng columns, N = N_y + 2 ng rows */
+int ibtfo_dims(const ibtfo_ctx* ctx, int* M, int* N);
+/* device footprint in bytes of a configuration (before create) */
+long long ibtfo_footprint(const ibtfo_config* cfg);
+
+/* Geometry on the device: Psy_coord + level-set hook + normals + classification
+ * + Scheme_Index (mirror points, donor indices, weights) + compacted ghost list.
+ * phi_host: IBTFO_LS_FIELD only (M*N doubles, reference cell order), else NULL. */
+int ibtfo_geometry(ibtfo_ctx* ctx, int ls_kind, const double* prm, int nprm, const double* phi_host);
+/* Initialize hook; evaluated on the host, uploaded as both U_OLD and U_NEW. */
+int ibtfo_initialize(ibtfo_ctx* ctx, int init_kind, const double* prm, int nprm);
+/* Boundary once (the drivers' call before the loop for GFM variants,
+ * Test/InclineSod/InclineSod.cpp:81), then ComputeDt; resets time/iteration. */
+int ibtfo_prepare(ibtfo_ctx* ctx, int apply_boundary);
+
+/* state exchange in the reference's AoS layout: U[(M*j+i)*4+k] (M*N*4 doubles).
+ * U_NEW may be NULL on upload (then U_NEW = U_OLD, as every Initialize hook does). */
+int ibtfo_upload_state(ibtfo_ctx* ctx, const double* U_OLD, const double* U_NEW);
+int ibtfo_download_state(ibtfo_ctx* ctx, double* U_OLD);
+/* XYCOORD[(M*j+i)*6+c], c = {x, y, phi, n_x, n_y, type}; ring normals are 0 */
+int ibtfo_download_geometry(ibtfo_ctx* ctx, double* XYCOORD);
+/* SCHEME_IDX in the selected Boundary's record width (1, 4, 8 or 16 doubles/cell) */
+int ibtfo_download_scheme_index(ibtfo_ctx* ctx, double* SCHEME_IDX);
+int ibtfo_sch_width(const ibtfo_ctx* ctx);
+int ibtfo_num_ghosts(const ibtfo_ctx* ctx);
+
+/* Up to nsteps steps of the reference loop body (Test/Sod/Sod.cpp:154-177):
+ * TimeAdvance(dt); Boundary; Global_Boundary; dt = ComputeDt; mass; now_t += dt;
+ * iter++ -- stopping, on the device, when now_t >= Psy_time or iter >= max_iter.
+ * Asynchronous on the context stream; dt never leaves the device. */
+int ibtfo_step(ibtfo_ctx* ctx, int nsteps);
+/* Same, replayed from a captured HIP graph of `chunk` steps (captured on first use). */
+int ibtfo_step_graph(ibtfo_ctx* ctx, int nsteps, int chunk);
+/* Synchronises; iteration count, physical time and the current dt. */
+int ibtfo_status(ibtfo_ctx* ctx, int* iter, double* now_t, double* dt);
+/* Per-step log of the last `count` (<= 4096) steps ending at the current step:
+ * dt used after the step, time, fluid mass (CaculateMass). Any pointer may be NULL. */
+int ibtfo_log(ibtfo_ctx* ctx, int count, double* dt, double* t, double* mass);
+/* ComputeDt on the current state (synchronises). */
+int ibtfo_compute_dt(ibtfo_ctx* ctx, double* dt);
+/* CaculateMass on the current state (synchronises). */
+int ibtfo_mass(ibtfo_ctx* ctx, double* mass);
+/* Boundary (the selected variant) once on the current state. */
+int ibtfo_boundary(ibtfo_ctx* ctx);
+
+/* Timing: runs nsteps steps between two HIP events recorded on the context
+ * stream and returns the elapsed milliseconds; *step_kernel_ms gets the mean
+ * duration of the fused step kernel alone measured with events around each
+ * launch in a second pass of `nsteps` steps (may be NULL to skip). */
+int ibtfo_time_steps(ibtfo_ctx* ctx, int nsteps, double* total_ms, double* step_kernel_ms);
+
+/* Device pointers of the current state planes (rho, rho u, rho v, E), their
+ * row pitch in doubles and the offset of cell (0,0) in each plane. */
+int ibtfo_device_state(ibtfo_ctx* ctx, double** planes4, int* pitch, int* offset);
+
+/* Last error message of this thread (static storage). */
+const char* ibtfo_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IBTFO_H_ */

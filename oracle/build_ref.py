@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""Build the reference's own translation units into oracle/_ref/ (test infrastructure).
+
+The reference selects Scheme / Riemann / Limiter / Boundary at link time
+(Test/Sod/Makefile:1-45). This recipe compiles, for each combination the tests
+use, the reference sources where they lie under /root/reference together with
+oracle/ref_harness.cpp, oracle/ref_limiter.cpp and the chosen problem hooks, into
+oracle/_ref/libref_<case>_<scheme>_<riemann>_<limiter>_<boundary>.so.
+
+Compiler: g++ -O2 -fopenacc (OpenACC host fallback, sequential) with
+-ffp-contract=off, so the fixture arithmetic is plain IEEE double in source order.
+Nothing is copied out of /root/reference; when it is absent this script exits
+without building (the GPU box only uses prebuilt files).
+
+Usage: python3 oracle/build_ref.py [-j N] [--list]
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "_ref")
+
+SRC = REF + "/Src"
+TEST = REF + "/Test"
+
+CASE_HOOKS = {
+    # case -> hook sources
+    "sodx": [os.path.join(HERE, "ref_hooks_sodx.cpp"), TEST + "/Sod/Level_Set_function.cpp",
+             TEST + "/Sod/Global_Boudary.cpp"],
+    "sod": [TEST + "/Sod/Initialize.cpp", TEST + "/Sod/Level_Set_function.cpp", TEST + "/Sod/Global_Boudary.cpp"],
+    "incline": [TEST + "/InclineSod/Initialize.cpp", TEST + "/InclineSod/Level_Set_function.cpp",
+                TEST + "/InclineSod/Global_Boudary.cpp"],
+    "vortex": [TEST + "/Vortex/Initialize.cpp", TEST + "/Vortex/Level_Set_function.cpp",
+               TEST + "/Vortex/Global_Boudary.cpp"],
+}
+NUM_SCH = {"Regular": 1, "IDWBaseGFM": 8, "LinearBaseGFM": 4, "RiemannBaseGFM": 4, "BCGFM": 16, "HBCGFM": 16}
+
+SCHEMES = ["MUSCL", "MHM", "Godunov", "Lax"]
+RIEMANNS = ["HLL", "HLLC", "PVRS", "StegerWarming"]
+LIMITERS = ["minmod", "VanLeer"]
+
+
+def combos():
+    out = []
+    for s in SCHEMES:
+        for r in ["HLL", "PVRS", "StegerWarming"]:
+            for l in LIMITERS:
+                if s == "Lax" and (r != "HLL" or l != "minmod"):
+                    continue
+                out.append(("sodx", s, r, l, "Regular"))
+    out.append(("sodx", "MUSCL", "HLLC", "minmod", "Regular"))
+    out.append(("sod", "MUSCL", "HLL", "minmod", "Regular"))
+    for b in ["IDWBaseGFM", "LinearBaseGFM", "RiemannBaseGFM", "BCGFM", "HBCGFM"]:
+        out.append(("incline", "MUSCL", "HLL", "minmod", b))
+    out.append(("incline", "MHM", "HLL", "minmod", "IDWBaseGFM"))
+    out.append(("incline", "MUSCL", "StegerWarming", "minmod", "IDWBaseGFM"))
+    out.append(("vortex", "MUSCL", "HLL", "minmod", "IDWBaseGFM"))
+    return out
+
+
+def lib_name(c):
+    return "libref_%s_%s_%s_%s_%s.so" % c
+
+
+def sources(c):
+    case, scheme, riemann, limiter, bound = c
+    srcs = [SRC + "/Geometry/Psy_coord.cpp", SRC + "/Geometry/Level_Set.cpp", SRC + "/Geometry/Mirror_IDX.cpp",
+            SRC + "/Utils/ParamReader.cpp", SRC + "/Utils/CaculateMass.cpp",
+            SRC + "/Base/Common/ComputeDt.cpp", SRC + "/Base/Common/Conserve2Flux.cpp",
+            SRC + "/Base/TimeAdvance/SingleStep/SingleStep.cpp"]
+    bdir = SRC + "/BoundaryCondition/" + bound
+    srcs += sorted(os.path.join(bdir, f) for f in os.listdir(bdir) if f.endswith(".cpp"))
+    adir = SRC + "/Base/Euler/Advection/" + scheme
+    srcs += sorted(os.path.join(adir, f) for f in os.listdir(adir) if f.endswith(".cpp"))
+    if scheme != "Lax":
+        rdir = SRC + "/Base/Euler/Riemann/" + riemann
+        srcs += sorted(os.path.join(rdir, f) for f in os.listdir(rdir) if f.endswith(".cpp"))
+    srcs += CASE_HOOKS[case]
+    srcs += [os.path.join(HERE, "ref_harness.cpp"), os.path.join(HERE, "ref_limiter.cpp")]
+    return srcs
+
+
+def includes(c):
+    _, scheme, _, limiter, bound = c
+    dirs = [SRC + "/Geometry", SRC + "/Utils", SRC + "/BoundaryCondition", SRC + "/Base/Euler/Advection",
+            SRC + "/Base/Common", SRC + "/Base/Euler/Riemann", SRC + "/Base/Slope_limiter/" + limiter,
+            SRC + "/Base/TimeAdvance", SRC + "/BoundaryCondition/" + bound,
+            SRC + "/Base/Euler/Advection/" + scheme]
+    return ["-I" + d for d in dirs]
+
+
+def build(c):
+    out = os.path.join(OUT, lib_name(c))
+    srcs = sources(c)
+    if os.path.exists(out) and os.path.getmtime(out) >= max(os.path.getmtime(s) for s in srcs):
+        return out, "up-to-date"
+    defs = ["-DREF_NUM_SCH=%d" % NUM_SCH[c[4]], "-DREF_TMP_W=%d" % (16 if c[1] == "Lax" else 28),
+            "-DREF_HAS_SCHEME_INDEX=%d" % (0 if c[4] == "Regular" else 1)]
+    cmd = ["g++", "-O2", "-fopenacc", "-ffp-contract=off", "-fPIC", "-shared", "-w", "-o", out + ".tmp"] + \
+        defs + includes(c) + srcs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        return out, "FAILED\n" + r.stderr[-3000:]
+    os.replace(out + ".tmp", out)
+    return out, "built"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", type=int, default=min(8, os.cpu_count() or 1))
+    ap.add_argument("--list", action="store_true")
+    a = ap.parse_args()
+    if a.list:
+        for c in combos():
+            print(lib_name(c))
+        return 0
+    if not os.path.isdir(REF):
+        print("build_ref: %s absent; nothing built" % REF)
+        return 0
+    os.makedirs(OUT, exist_ok=True)
+    bad = 0
+    with cf.ThreadPoolExecutor(a.j) as ex:
+        for out, msg in ex.map(build, combos()):
+            if msg.startswith("FAILED"):
+                bad += 1
+                print(os.path.basename(out), msg)
+    print("build_ref: %d libraries, %d failed" % (len(combos()), bad))
+    return 1 if bad else 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

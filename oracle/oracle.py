@@ -1,0 +1,273 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as
+the checker. The product (ibtfo_amd/) never imports this module.
+
+`Case` describes one problem the way the reference's .inp file + Makefile slots
+do; `run_oracle` drives liboracle.so (the C++ restatement) in the order of
+Test/Sod/Sod.cpp:94-181 / Test/InclineSod/InclineSod.cpp:73-165, and `run_ref`
+drives a reference library built by build_ref.py from the reference's sources.
+"""
+import ctypes as C
+import os
+import tempfile
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+SCHEMES = {"Lax": 0, "MUSCL": 1, "MHM": 2, "Godunov": 3}
+RIEMANNS = {"HLL": 0, "HLLC": 1, "PVRS": 2, "StegerWarming": 3}
+LIMITERS = {"minmod": 0, "VanLeer": 1, "VanAlbada": 2}
+BOUNDS = {"Regular": 0, "IDWBaseGFM": 1, "LinearBaseGFM": 2, "RiemannBaseGFM": 3, "BCGFM": 4, "HBCGFM": 5}
+EDGES = {"reference": 0, "periodic": 1, "inflow": 2}
+# level-set hooks and initialisers (see or_level_set_hook / or_initialize)
+LS = {"sod": 0, "sodx": 0, "incline": 1, "vortex": 2, "phi": 3, "none": 0}
+INIT = {"sod": 0, "incline": 1, "vortex": 2, "sodx": 3, "uniform": 4, "shu": 5}
+
+_D = C.POINTER(C.c_double)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle library missing: build it with `make -C oracle`")
+        L = C.CDLL(LIB)
+        L.or_compute_dt.restype = C.c_double
+        L.or_mass.restype = C.c_double
+        L.or_limiter.restype = C.c_double
+        L.or_limiter.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.or_run.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+@dataclass
+class Case:
+    """A problem in the reference's terms: .inp numbers + slot selection + hooks."""
+    N_x: int = 40
+    N_y: int = 10
+    lo_x: float = 0.0
+    lo_y: float = 0.0
+    hi_x: float = 1.0
+    hi_y: float = 0.25
+    Psy_time: float = 0.2
+    max_iter: int = 100000
+    CFL_number: float = 0.5
+    num_ghost_cell: int = 2
+    gamma: float = 1.4
+    Scheme: str = "MUSCL"
+    Riemann: str = "HLL"
+    Limiter: str = "minmod"
+    Boundary: str = "Regular"
+    case: str = "sodx"          # hook set: sod | sodx | incline | vortex | uniform | shu
+    edge: str = "reference"     # Global_Boundary policy
+    quirks: int = 1             # 1 = reference arithmetic incl. its defects, 0 = corrected
+    extra: dict = field(default_factory=dict)  # problem keys (rho_L.., angle, AxisD, fa, f0, ...)
+    phi: object = None          # per-cell level set when case uses a given field
+
+    @property
+    def L(self):
+        return self.hi_x - self.lo_x
+
+    @property
+    def H(self):
+        return self.hi_y - self.lo_y
+
+    @property
+    def M(self):
+        return self.N_x + 2 * self.num_ghost_cell
+
+    @property
+    def N(self):
+        return self.N_y + 2 * self.num_ghost_cell
+
+    def inp_text(self):
+        keys = dict(max_iter=self.max_iter, N_x=self.N_x, N_y=self.N_y, Psy_time=self.Psy_time,
+                    lo_x=self.lo_x, lo_y=self.lo_y, hi_x=self.hi_x, hi_y=self.hi_y,
+                    CFL_number=self.CFL_number, num_ghost_cell=self.num_ghost_cell, gamma=self.gamma)
+        keys.update(self.extra)
+        lines = ["#generated"] + ["%s=%s" % (k, _num(v)) for k, v in keys.items()]
+        lines += ["Scheme=%s" % self.Scheme, "Riemann=%s" % self.Riemann, "Limiter=%s" % self.Limiter,
+                  "Boundary=%s" % self.Boundary]
+        return "\n".join(lines) + "\n"
+
+    def ls_prm(self):
+        e = self.extra
+        if self.case == "incline":
+            return [e.get("angle", 30.0), e.get("AxisD", 0.4)]
+        if self.case == "vortex":
+            return [e.get("f0", 0.0), e.get("f1", 1.0)]
+        return [0.0]
+
+    def init_prm(self):
+        e = self.extra
+        left = [e.get("rho_L", 1), e.get("u_L", 0), e.get("v_L", 0), e.get("p_L", 1)]
+        right = [e.get("rho_R", 0.125), e.get("u_R", 0), e.get("v_R", 0), e.get("p_R", 0.1)]
+        if self.case == "sod":
+            return left + right
+        if self.case == "incline":
+            return [e.get("angle", 30.0)] + left + right
+        if self.case == "vortex":
+            return [e.get("fa", 0.0), e.get("fw", 0.0)]
+        if self.case == "sodx":
+            return [e.get("x0", 0.5)] + left + right
+        if self.case == "uniform":
+            return [e.get("rho", 1.0), e.get("u", 0.0), e.get("v", 0.0), e.get("p", 1.0)]
+        if self.case == "shu":
+            return [e.get("x0", 5.0), e.get("y0", 5.0), e.get("beta", 5.0), 1.0, e.get("u", 1.0),
+                    e.get("v", 1.0), 1.0]
+        raise ValueError(self.case)
+
+    def edge_prm(self):
+        e = self.extra
+        return [e.get("rho", 1.0), e.get("u", 0.0), e.get("v", 0.0), e.get("p", 1.0)]
+
+    def init_boundary(self):
+        # Sod.cpp applies no Boundary before the loop; InclineSod.cpp:81 and Vortex.cpp do
+        return 0 if self.Boundary == "Regular" else 1
+
+
+def _num(v):
+    if isinstance(v, (int, np.integer)):
+        return str(int(v))
+    return repr(float(v))
+
+
+def setup(case):
+    """Psy_coord + Level_Set (+ hook) + Scheme_Index + Initialize, on the CPU."""
+    L = lib()
+    g = case
+    cells = g.M * g.N
+    XY = np.zeros(cells * 6)
+    U_OLD = np.zeros(cells * 4)
+    U_NEW = np.zeros(cells * 4)
+    bc = BOUNDS[g.Boundary]
+    S = np.zeros(cells * L.or_sch_width(bc))
+    L.or_psy_coord(C.c_double(g.lo_x), C.c_double(g.lo_y), C.c_double(g.L), C.c_double(g.H),
+                   g.N_x, g.N_y, g.num_ghost_cell, ptr(XY))
+    ls_kind = LS.get(g.case, 0)
+    phi = None
+    if g.phi is not None:
+        ls_kind = 3
+        phi = np.ascontiguousarray(g.phi, dtype=np.float64)
+    prm = np.array(g.ls_prm(), dtype=np.float64)
+    L.or_level_set_hook(ls_kind, ptr(prm), g.N_x, g.N_y, g.num_ghost_cell, ptr(XY), ptr(phi))
+    L.or_classify(g.N_x, g.N_y, g.num_ghost_cell, ptr(XY))
+    L.or_scheme_index(bc, g.N_x, g.N_y, g.num_ghost_cell, ptr(XY), ptr(S))
+    ip = np.array(g.init_prm(), dtype=np.float64)
+    L.or_initialize(INIT[g.case], ptr(ip), g.N_x, g.N_y, g.num_ghost_cell, C.c_double(g.gamma),
+                    ptr(U_OLD), ptr(U_NEW), ptr(XY))
+    return dict(XY=XY, U_OLD=U_OLD, U_NEW=U_NEW, S=S)
+
+
+def run_oracle(case, nsteps, root_mode=0, state=None):
+    """Whole run like the reference driver, at most nsteps steps. Returns a dict."""
+    L = lib()
+    L.or_set_root_mode(int(root_mode))
+    g = case
+    st = setup(g) if state is None else state
+    T = np.zeros(g.M * g.N * L.or_tmp_width(SCHEMES[g.Scheme]))
+    dt_log = np.zeros(max(nsteps, 1))
+    t_log = np.zeros(max(nsteps, 1))
+    m_log = np.zeros(max(nsteps, 1))
+    ep = np.array(g.edge_prm(), dtype=np.float64)
+    n = L.or_run(SCHEMES[g.Scheme], RIEMANNS[g.Riemann], LIMITERS[g.Limiter], BOUNDS[g.Boundary],
+                 EDGES[g.edge], ptr(ep), g.quirks, C.c_double(g.L), C.c_double(g.H), g.N_x, g.N_y,
+                 g.num_ghost_cell, C.c_double(g.gamma), C.c_double(g.CFL_number), C.c_double(g.Psy_time),
+                 min(g.max_iter, nsteps), g.init_boundary(), ptr(st["U_OLD"]), ptr(st["U_NEW"]), ptr(st["XY"]),
+                 ptr(st["S"]), ptr(T), ptr(dt_log), ptr(t_log), ptr(m_log))
+    L.or_set_root_mode(0)
+    st.update(steps=n, dt=dt_log[:n], t=t_log[:n], mass=m_log[:n])
+    return st
+
+
+def compute_dt(case, U, XY, root_mode=0):
+    L = lib()
+    L.or_set_root_mode(int(root_mode))
+    v = L.or_compute_dt(C.c_double(case.L), C.c_double(case.H), case.N_x, case.N_y, case.num_ghost_cell,
+                        C.c_double(case.gamma), C.c_double(case.CFL_number), ptr(U), ptr(XY))
+    L.or_set_root_mode(0)
+    return v
+
+
+# ------------------------------------------------------------ reference libs
+def ref_lib_name(case):
+    return "libref_%s_%s_%s_%s_%s.so" % (case.case, case.Scheme, case.Riemann, case.Limiter, case.Boundary)
+
+
+def ref_available(case=None):
+    if case is None:
+        return os.path.isdir(REF_DIR) and any(f.endswith(".so") for f in os.listdir(REF_DIR))
+    return os.path.exists(os.path.join(REF_DIR, ref_lib_name(case)))
+
+
+_ref_cache = {}
+
+
+def ref_lib(name):
+    if name not in _ref_cache:
+        L = C.CDLL(os.path.join(REF_DIR, name), mode=os.RTLD_LOCAL | os.RTLD_NOW)
+        L.ref_run.restype = C.c_int
+        L.ref_lim_L.restype = C.c_double
+        L.ref_lim_L.argtypes = [C.c_double] * 3
+        L.ref_lim_R.restype = C.c_double
+        L.ref_lim_R.argtypes = [C.c_double] * 3
+        _ref_cache[name] = L
+    return _ref_cache[name]
+
+
+def write_inp(case, path=None):
+    if path is None:
+        fd, path = tempfile.mkstemp(suffix=".inp")
+        os.close(fd)
+    with open(path, "w") as f:
+        f.write(case.inp_text())
+    return path
+
+
+def run_ref(case, nsteps):
+    """Run the reference translation units on the same case (edge policy and
+    hooks are the reference's own)."""
+    L = ref_lib(ref_lib_name(case))
+    path = write_inp(case)
+    try:
+        cells = case.M * case.N
+        out = dict(U_OLD=np.zeros(cells * 4), U_NEW=np.zeros(cells * 4), XY=np.zeros(cells * 6),
+                   S=np.zeros(cells * L.ref_sch_width()))
+        dt0 = C.c_double(0)
+        dt_log, t_log, m_log = np.zeros(max(nsteps, 1)), np.zeros(max(nsteps, 1)), np.zeros(max(nsteps, 1))
+        n = L.ref_run(path.encode(), nsteps, case.init_boundary(), ptr(out["U_OLD"]), ptr(out["U_NEW"]),
+                      ptr(out["XY"]), ptr(out["S"]), C.byref(dt0), ptr(dt_log), ptr(t_log), ptr(m_log))
+        out.update(steps=n, dt0=dt0.value, dt=dt_log[:n], t=t_log[:n], mass=m_log[:n])
+        return out
+    finally:
+        os.unlink(path)
+
+
+def parse_ref(path, strings=False):
+    """The reference ParamReader's numbers()/strings() map for a file."""
+    names = [f for f in os.listdir(REF_DIR) if f.endswith(".so")]
+    L = ref_lib(sorted(names)[0])
+    buf = C.create_string_buffer(1 << 16)
+    n = L.ref_parse(path.encode(), int(strings), buf, len(buf))
+    assert n >= 0
+    out = {}
+    for line in buf.value.decode("utf-8", "surrogateescape").split("\n"):
+        if not line:
+            continue
+        k, v = line.split("\t", 1)
+        out[k] = v if strings else float(v)
+    return out
