@@ -20,7 +20,7 @@ BOUNDARY = {"Regular": 0, "IDWBaseGFM": 1, "LinearBaseGFM": 2, "RiemannBaseGFM":
 TIME = {"SingleStep": 0, "RK2": 1}
 EDGE = {"reference": 0, "periodic": 1, "inflow": 2}
 PRECISION = {"FP64": 0, "mixed": 1}
-LEVELSET = {"none": 0, "channel": 1, "annulus": 2, "field": 3, "ellipse": 4}
+LEVELSET = {"none": 0, "channel": 1, "annulus": 2, "field": 3, "ellipse": 4, "shapes": 5}
 INIT = {"sod_rows": 0, "incline": 1, "vortex": 2, "sod_x": 3, "uniform": 4, "shu": 5}
 
 OK, E_ARG, E_HIP, E_NOMEM, E_IO, E_STATE, E_NONFINITE = range(7)

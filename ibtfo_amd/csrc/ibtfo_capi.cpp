@@ -241,6 +241,23 @@ extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int npr
             g.p[0] = prm[0]; g.p[1] = prm[1]; g.p[2] = prm[2]; g.p[3] = prm[3]; g.p[4] = cos(ang); g.p[5] = sin(ang);
             break;
         }
+        case IBTFO_LS_SHAPES: {
+            if (!prm || nprm < 1 || nprm != 1 + (int)prm[0] * SHAPE_STRIDE) {
+                set_error("IBTFO_LS_SHAPES needs 1 + n*%d parameters", SHAPE_STRIDE);
+                return IBTFO_E_ARG;
+            }
+            for (int k = 0; k < (int)prm[0]; k++) {
+                const double* sh = prm + 1 + k * SHAPE_STRIDE;
+                if (sh[0] != 0.0 && (sh[4] < 3 || sh[4] > 8)) { set_error("polygon needs 3..8 vertices"); return IBTFO_E_ARG; }
+            }
+            if (nprm > 1) {
+                HIPCHK(hipMalloc(&dphi, (nprm - 1) * sizeof(double)));
+                HIPCHK(hipMemcpy(dphi, prm + 1, (nprm - 1) * sizeof(double), hipMemcpyHostToDevice));
+            }
+            g.shapes = dphi;
+            g.nshapes = (int)prm[0];
+            break;
+        }
         case IBTFO_LS_FIELD: {
             if (!phi_host) { set_error("IBTFO_LS_FIELD needs phi"); return IBTFO_E_ARG; }
             HIPCHK(hipMalloc(&dphi, (size_t)L.M * L.N * sizeof(double)));
@@ -455,7 +472,7 @@ static void build_args(ibtfo_ctx* x) {
     f.bc = c.boundary; f.gam = c.gamma; f.gm1 = c.gamma - 1;
     f.dxg = (x->Li + 1 * x->dxc) - (x->Li + 0 * x->dxc);
     f.dyg = (x->Hi + 2 * x->dxc) - (x->Hi + 1 * x->dxc);
-    f.use_cur = 1; f.require_ran = 1; f.ring_prev = 1;
+    f.use_cur = 1; f.require_ran = 1; f.ring_prev = 1; f.quirks = c.quirks;
 
     EdgeArgs& e = x->ea;
     memset(&e, 0, sizeof e);

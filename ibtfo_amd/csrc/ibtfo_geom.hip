@@ -38,6 +38,34 @@ __device__ double hook_phi(const GeomArgs& g, double x, double y, long long q) {
             double gn = ib_sqrt(gx * gx + gy * gy);
             return (s > 0.0 && gn > 0.0) ? (s * s - 1.0) / (2.0 * gn) : -((a < b) ? a : b);
         }
+        case 5: {  // union of circles and convex polygons; exact signed distance (< 0 inside a body)
+            double best = 1e300;
+            for (int k = 0; k < g.nshapes; k++) {
+                const double* sh = g.shapes + k * SHAPE_STRIDE;
+                double d;
+                if (sh[0] == 0.0) {
+                    d = ib_sqrt((x - sh[1]) * (x - sh[1]) + (y - sh[2]) * (y - sh[2])) - sh[3];
+                } else {
+                    int nv = (int)sh[4];
+                    double dmin = 1e300;
+                    bool inside = true;
+                    for (int e = 0; e < nv; e++) {
+                        double ax = sh[5 + 2 * e], ay = sh[6 + 2 * e];
+                        double bx = sh[5 + 2 * ((e + 1) % nv)], by = sh[6 + 2 * ((e + 1) % nv)];
+                        double ex = bx - ax, ey = by - ay, px = x - ax, py = y - ay;
+                        double t = (px * ex + py * ey) / (ex * ex + ey * ey);
+                        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+                        double qx = px - t * ex, qy = py - t * ey;
+                        double dd = qx * qx + qy * qy;
+                        dmin = dd < dmin ? dd : dmin;
+                        if (ex * py - ey * px < 0.0) inside = false;  // vertices counter-clockwise
+                    }
+                    d = inside ? -ib_sqrt(dmin) : ib_sqrt(dmin);
+                }
+                best = d < best ? d : best;
+            }
+            return best;
+        }
         default:
             return 1.0;
     }

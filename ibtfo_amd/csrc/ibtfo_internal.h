@@ -87,6 +87,7 @@ struct FillArgs {
     int fixed;
     int require_ran;   // skip unless the preceding step executed
     int ring_prev;     // read ring donors from the previous state (the ring before Global_Boundary)
+    int quirks;
 };
 
 struct EdgeArgs {

@@ -101,6 +101,27 @@ template <bool Y, typename T> IB_HD void flux_of(T gm1, const T* U, T* F) {
     phys_flux<Y>(rho, u, v, p, U[3], F);
 }
 
+// Corrected mode only (no reference counterpart): positivity of a conserved
+// state, the per-face first-order fallback and the density/pressure floors.
+template <typename T> IB_HD bool physical(T gm1, const T* q) {
+    T rho = q[0], u = q[1] / rho, v = q[2] / rho;
+    T p = gm1 * (q[3] - T(0.5) * rho * (u * u + v * v));
+    return rho > T(0) && p > T(0);
+}
+template <typename T> IB_HD void face_fallback(T gm1, const T* a, const T* b, T* lo, T* hi) {
+    if (physical(gm1, lo) && physical(gm1, hi)) return;
+#pragma unroll
+    for (int k = 0; k < 4; k++) { lo[k] = a[k]; hi[k] = b[k]; }
+}
+constexpr double RHO_FLOOR = 1e-8, P_FLOOR = 1e-8;
+IB_HD void apply_floor(double gm1, double* q) {
+    if (!(q[0] > RHO_FLOOR)) q[0] = RHO_FLOOR;
+    double u = q[1] / q[0], v = q[2] / q[0];
+    double ke = 0.5 * q[0] * (u * u + v * v);
+    double p = gm1 * (q[3] - ke);
+    if (!(p > P_FLOOR)) q[3] = P_FLOOR / gm1 + ke;
+}
+
 struct Gas {
     double gam, gm1, gp1, c1;  // gamma, gamma-1, gamma+1, (gamma-1)/2/gamma
 };
@@ -109,7 +130,7 @@ template <typename T> struct GasT { T gam, gm1, gp1, c1; };
 // HLL/Riemann_solver.cpp:131-168 (x), :202-235 (y). Returns true when the
 // first branch (S_lo >= 0) is taken; F is then F(U_lo) (the y solver's caller
 // decides what that branch stores).
-template <bool Y, typename T> IB_HD bool hll_face(const GasT<T>& g, const T* Ulo, const T* Uhi, T* F) {
+template <bool Y, typename T> IB_HD bool hll_face(const GasT<T>& g, const T* Ulo, const T* Uhi, T* F, bool quirks) {
     T rl, ul, vl, pl, rh, uh, vh, ph;
     primitives(g.gm1, Ulo, rl, ul, vl, pl);
     primitives(g.gm1, Uhi, rh, uh, vh, ph);
@@ -118,8 +139,19 @@ template <bool Y, typename T> IB_HD bool hll_face(const GasT<T>& g, const T* Ulo
     T rho_bar = T(0.5) * (rl + rh);
     T a_bar = T(0.5) * (al + ah);
     T p_star = T(0.5) * (pl + ph) + T(0.5) * (unl - unh) * (rho_bar * a_bar);
-    T Sl = unl - al * ib_sqrt(g.gp1 * p_star / T(2) / g.gam / pl + g.c1);
-    T Sh = unh + ah * ib_sqrt(g.gp1 * p_star / T(2) / g.gam / ph + g.c1);
+    T Sl, Sh;
+    if (quirks) {
+        Sl = unl - al * ib_sqrt(g.gp1 * p_star / T(2) / g.gam / pl + g.c1);
+        Sh = unh + ah * ib_sqrt(g.gp1 * p_star / T(2) / g.gam / ph + g.c1);
+    } else {
+        // corrected estimate (Toro, Riemann Solvers, 3rd ed., eq. 10.59-10.60): p* >= 0 and
+        // the shock factor only where p* exceeds the side's pressure
+        p_star = p_star > T(0) ? p_star : T(0);
+        T ql = p_star <= pl ? T(1) : ib_sqrt(g.gp1 * p_star / T(2) / g.gam / pl + g.c1);
+        T qh = p_star <= ph ? T(1) : ib_sqrt(g.gp1 * p_star / T(2) / g.gam / ph + g.c1);
+        Sl = unl - al * ql;
+        Sh = unh + ah * qh;
+    }
     T Fl[4], Fh[4];
     phys_flux<Y>(rl, ul, vl, pl, Ulo[3], Fl);
     if (Sl >= T(0)) {

@@ -139,9 +139,10 @@ __device__ __forceinline__ void xface(const GasT<double>& g, double dt, double d
             UR[k] = UR[k] + inc;
         }
     }
+    if (SCH != SCH_GODUNOV && !quirks) face_fallback(g.gm1, u1, u2, UL, UR);
     if (RS == RS_SW) sw_face<false>(g, UL, UR, F);
     else if (RS == RS_PVRS) { bool s; pvrs_face<false>(g, UL, UR, F, s); }
-    else hll_face<false>(g, UL, UR, F);
+    else hll_face<false>(g, UL, UR, F, quirks != 0);
 }
 
 // y face at j+1/2 of column c; Fx is this cell's x-face flux in the same row,
@@ -169,6 +170,7 @@ __device__ __forceinline__ void yface(const GasT<double>& g, double dt, double d
             UU[k] = UU[k] + inc;
         }
     }
+    if (SCH != SCH_GODUNOV && !quirks) face_fallback(g.gm1, u1, u2, UD, UU);
     if (RS == RS_SW) {
         sw_face<true>(g, UD, UU, G);
     } else if (RS == RS_PVRS) {
@@ -176,7 +178,7 @@ __device__ __forceinline__ void yface(const GasT<double>& g, double dt, double d
         pvrs_face<true>(g, UD, UU, G, stray);
         if (stray && quirks) G[3] = Fx[3];
     } else {
-        bool up = hll_face<true>(g, UD, UU, G);
+        bool up = hll_face<true>(g, UD, UU, G, quirks != 0);
         if (up && quirks) {
 #pragma unroll
             for (int k = 0; k < 4; k++) G[k] = Fx[k];
@@ -296,6 +298,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
                         double v = w[1][k] - dt * (F[k] - Fm) / dx;
                         un[k] = v - dt * (G[k] - Gp[k]) / dy;
                     }
+                    if (!a.quirks) apply_floor(a.gm1, un);
                 } else {
 #pragma unroll
                     for (int k = 0; k < 4; k++) un[k] = w[1][k];
@@ -452,6 +455,12 @@ __global__ void k_fill(FillArgs a) {
     const double n_x = gh.nx, n_y = gh.ny;
     double rw, uw, vw, pw;
     if (a.bc == 1) {  // IDWBaseGFM
+        if (!a.quirks && gh.a[0] + gh.a[1] + gh.a[2] + gh.a[3] == 0.0) {
+            // corrected mode: no fluid donor, the ghost keeps its value
+            double* o = a.out + 4 * (long long)gi;
+            for (int k = 0; k < 4; k++) o[k] = U[gh.cell + k * PL];
+            return;
+        }
         idw(gh.d1, gh.ring & 15, gh.a, rw, uw, vw, pw);
         double u_n = uw * n_x + vw * n_y;
         double u_t = uw * n_y - vw * n_x;

@@ -1,0 +1,68 @@
+"""Synthetic problem definitions used by bench.py, the tests and the serving
+demo (seeded, so every run sees the same geometry)."""
+import math
+
+import numpy as np
+
+SHAPE_STRIDE = 21
+
+
+def c4_shapes(seed=20261202, H=1.0):
+    """8-20 bodies (circles or convex polygons with 3-8 vertices), radius
+    U(0.02, 0.1)*H, centres in [0.2, 1.9] x [0.1, 0.9]; returns the flat
+    IBTFO_LS_SHAPES parameter list."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(8, 21))
+    prm = [float(n)]
+    for _ in range(n):
+        rec = [0.0] * SHAPE_STRIDE
+        r = float(rng.uniform(0.02, 0.1) * H)
+        cx, cy = float(rng.uniform(0.2, 1.9)), float(rng.uniform(0.1, 0.9))
+        if rng.random() < 0.5:
+            rec[0], rec[1], rec[2], rec[3] = 0.0, cx, cy, r
+        else:
+            nv = int(rng.integers(3, 9))
+            ang = np.sort(rng.uniform(0, 2 * math.pi, nv))
+            rec[0], rec[1], rec[2], rec[3], rec[4] = 1.0, cx, cy, r, float(nv)
+            for e in range(nv):
+                rec[5 + 2 * e] = cx + r * math.cos(ang[e])
+                rec[6 + 2 * e] = cy + r * math.sin(ang[e])
+        prm += rec
+    return prm
+
+
+def shapes_phi(prm, x, y):
+    """numpy evaluation of the IBTFO_LS_SHAPES signed distance (same formulas)."""
+    n = int(prm[0])
+    best = np.full(np.broadcast(x, y).shape, 1e300)
+    for k in range(n):
+        sh = prm[1 + k * SHAPE_STRIDE: 1 + (k + 1) * SHAPE_STRIDE]
+        if sh[0] == 0.0:
+            d = np.sqrt((x - sh[1]) * (x - sh[1]) + (y - sh[2]) * (y - sh[2])) - sh[3]
+        else:
+            nv = int(sh[4])
+            dmin = np.full_like(best, 1e300)
+            inside = np.ones_like(best, dtype=bool)
+            for e in range(nv):
+                ax, ay = sh[5 + 2 * e], sh[6 + 2 * e]
+                bx, by = sh[5 + 2 * ((e + 1) % nv)], sh[6 + 2 * ((e + 1) % nv)]
+                ex, ey = bx - ax, by - ay
+                px, py = x - ax, y - ay
+                t = np.clip((px * ex + py * ey) / (ex * ex + ey * ey), 0.0, 1.0)
+                qx, qy = px - t * ex, py - t * ey
+                dmin = np.minimum(dmin, qx * qx + qy * qy)
+                inside &= ~(ex * py - ey * px < 0.0)
+            d = np.where(inside, -np.sqrt(dmin), np.sqrt(dmin))
+        best = np.minimum(best, d)
+    return best
+
+
+# Mach 3 inflow (rho, u, v, p) of the 4096x2048 immersed-body workload
+C4_INFLOW = (1.4, 3.0, 0.0, 1.0)
+
+
+def c4_config(ib, N_x=4096, N_y=2048, Riemann="HLL", Limiter="VanLeer", Boundary="IDWBaseGFM", quirks=0,
+              max_iter=10 ** 9, CFL_number=0.5):
+    return ib.config(N_x=N_x, N_y=N_y, lo_x=0.0, lo_y=0.0, hi_x=2.0, hi_y=1.0, Psy_time=1e30, max_iter=max_iter,
+                     CFL_number=CFL_number, gamma=1.4, Scheme="MUSCL", Riemann=Riemann, Limiter=Limiter,
+                     Boundary=Boundary, Edge="inflow", quirks=quirks, edge_state=C4_INFLOW)

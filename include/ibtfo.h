@@ -60,7 +60,10 @@ enum ibtfo_levelset_kind {
     IBTFO_LS_CHANNEL = 1,  /* inclined channel, prm {angle_deg, AxisD}: Test/InclineSod/Level_Set_function.cpp:8-56 */
     IBTFO_LS_ANNULUS = 2,  /* annulus about (0.5,0.5), prm {f0, f1}: Test/Vortex/Level_Set_function.cpp:15-40 */
     IBTFO_LS_FIELD = 3,    /* phi given per cell (host array, reference AoS cell order) */
-    IBTFO_LS_ELLIPSE = 4   /* solid ellipse, prm {xc, yc, a, b, angle_deg}; phi = signed distance estimate */
+    IBTFO_LS_ELLIPSE = 4,  /* solid ellipse, prm {xc, yc, a, b, angle_deg}; phi = signed distance estimate */
+    IBTFO_LS_SHAPES = 5    /* union of bodies, exact signed distance: prm {n, n x [type (0 circle, 1 convex
+                              polygon), cx, cy, r, nverts, x0, y0, ..., x7, y7]} (21 doubles per body,
+                              polygon vertices counter-clockwise) */
 };
 
 /* initialisers (Initialize user hook) */

@@ -135,13 +135,23 @@ inline Prim prim_y(double gam, const double* U) {
 // ------------------------------------------------------------------ HLL
 // HLL/Riemann_solver.cpp:126-172 (x) and :196-238 (y). `lo` is the left/lower state.
 // Returns 1 when the first branch (S_lo >= 0) is taken.
-inline int hll_speeds(double gam, const Prim& a, const Prim& b, double& S_lo, double& S_hi) {
+// quirks = 0: the corrected estimate (Toro, Riemann Solvers, 3rd ed., eq. 10.59-10.60):
+// p* clamped at 0 and the shock factor applied only where p* > p_K. No reference output.
+inline int hll_speeds(double gam, const Prim& a, const Prim& b, double& S_lo, double& S_hi, int quirks = 1) {
     double a_lo = root(gam * a.p / a.rho), a_hi = root(gam * b.p / b.rho);
     double rho_bar = 0.5 * (a.rho + b.rho);
     double a_bar = 0.5 * (a_lo + a_hi);
     double p_star = 0.5 * (a.p + b.p) + 0.5 * (a.un - b.un) * (rho_bar * a_bar);
-    S_lo = a.un - a_lo * std::sqrt((gam + 1) * p_star / 2 / gam / a.p + (gam - 1) / 2 / gam);
-    S_hi = b.un + a_hi * std::sqrt((gam + 1) * p_star / 2 / gam / b.p + (gam - 1) / 2 / gam);
+    if (quirks) {
+        S_lo = a.un - a_lo * std::sqrt((gam + 1) * p_star / 2 / gam / a.p + (gam - 1) / 2 / gam);
+        S_hi = b.un + a_hi * std::sqrt((gam + 1) * p_star / 2 / gam / b.p + (gam - 1) / 2 / gam);
+    } else {
+        p_star = p_star > 0.0 ? p_star : 0.0;
+        double q_lo = p_star <= a.p ? 1.0 : std::sqrt((gam + 1) * p_star / 2 / gam / a.p + (gam - 1) / 2 / gam);
+        double q_hi = p_star <= b.p ? 1.0 : std::sqrt((gam + 1) * p_star / 2 / gam / b.p + (gam - 1) / 2 / gam);
+        S_lo = a.un - a_lo * q_lo;
+        S_hi = b.un + a_hi * q_hi;
+    }
     return S_lo >= 0;
 }
 void riemann_hll_x(Ctx& c) {
@@ -154,7 +164,7 @@ void riemann_hll_x(Ctx& c) {
             const double *FL = c.slot(i, j, S_FL), *FR = c.slot(i, j, S_FR);
             double* F = c.slot(i, j, S_FOLD);
             double SL, SR;
-            hll_speeds(c.gam, prim_x(c.gam, UL), prim_x(c.gam, UR), SL, SR);
+            hll_speeds(c.gam, prim_x(c.gam, UL), prim_x(c.gam, UR), SL, SR, c.quirks);
             for (int k = 0; k < 4; k++) {
                 if (SL >= 0) F[k] = FL[k];
                 else if (SR <= 0) F[k] = FR[k];
@@ -172,7 +182,7 @@ void riemann_hll_y(Ctx& c) {
             const double *GD = c.slot(i, j, S_GD), *GU = c.slot(i, j, S_GU);
             double* G = c.slot(i, j, S_GOLD);
             double SD, SU;
-            hll_speeds(c.gam, prim_y(c.gam, UD), prim_y(c.gam, UU), SD, SU);
+            hll_speeds(c.gam, prim_y(c.gam, UD), prim_y(c.gam, UU), SD, SU, c.quirks);
             for (int k = 0; k < 4; k++) {
                 if (SD >= 0) {
                     // reference: slot left as is, i.e. the x-flux of this cell (aliased
@@ -465,18 +475,61 @@ void advance_lax(Ctx& c, double dt, const double* U, double* Un, const double* X
                         0.5 * dt * (c.slot(i, j + 1, 3)[k] - c.slot(i, j - 1, 3)[k]) / c.dy;
 }
 
+// Corrected mode only (no reference counterpart): a face whose reconstructed state
+// has rho <= 0 or p <= 0 on either side falls back to the two cell averages.
+inline bool physical(double gam, const double* q) {
+    double rho = q[0], u = q[1] / rho, v = q[2] / rho;
+    double p = (gam - 1) * (q[3] - 0.5 * rho * (u * u + v * v));
+    return rho > 0 && p > 0;
+}
+void positivity_fallback(Ctx& c, const double* U, int ydir) {
+    const Grid& g = c.g;
+    int s_lo = ydir ? S_UD : S_UL, s_hi = ydir ? S_UU : S_UR;
+#pragma omp parallel for
+    for (int j = 0; j < g.N - (ydir ? 1 : 0); j++)
+        for (int i = 0; i < g.M - (ydir ? 0 : 1); i++) {
+            double* lo = c.slot(i, j, s_lo);
+            double* hi = c.slot(i, j, s_hi);
+            if (physical(c.gam, lo) && physical(c.gam, hi)) continue;
+            const double* a = U + g.c(i, j) * 4;
+            const double* b = U + (ydir ? g.c(i, j + 1) : g.c(i + 1, j)) * 4;
+            for (int k = 0; k < 4; k++) { lo[k] = a[k]; hi[k] = b[k]; }
+        }
+}
+
+// Corrected mode only (no reference counterpart): density and pressure floors on
+// updated fluid cells, so near-vacuum regions behind bodies stay computable.
+const double RHO_FLOOR = 1e-8, P_FLOOR = 1e-8;
+inline void apply_floor(double gam, double* q) {
+    if (!(q[0] > RHO_FLOOR)) q[0] = RHO_FLOOR;
+    double u = q[1] / q[0], v = q[2] / q[0];
+    double ke = 0.5 * q[0] * (u * u + v * v);
+    double p = (gam - 1) * (q[3] - ke);
+    if (!(p > P_FLOOR)) q[3] = P_FLOOR / (gam - 1) + ke;
+}
+void floors(Ctx& c, double* Un, const double* XY) {
+    const Grid& g = c.g;
+#pragma omp parallel for
+    for (int j = g.ng; j < g.N - g.ng; j++)
+        for (int i = g.ng; i < g.nx + g.ng; i++)
+            if (XY[g.c(i, j) * 6 + 5] == 0) apply_floor(c.gam, Un + g.c(i, j) * 4);
+}
+
 void advance(Ctx& c, int scheme, int rs, int lim, double dt, const double* U, double* Un, const double* XY) {
     if (scheme == SCH_LAX) { advance_lax(c, dt, U, Un, XY); return; }
     if (scheme == SCH_GODUNOV) copy_x(c, U);
     else recon_x(c, U, lim, scheme == SCH_MHM ? c.g.ng - 1 : 1);
     if (scheme == SCH_MHM) predictor(c, dt, 0);
+    if (!c.quirks && scheme != SCH_GODUNOV) positivity_fallback(c, U, 0);
     riemann_x(c, rs);
     update_x(c, dt, U, Un, XY);
     if (scheme == SCH_GODUNOV) copy_y(c, U);
     else recon_y(c, U, lim);
     if (scheme == SCH_MHM) predictor(c, dt, 1);
+    if (!c.quirks && scheme != SCH_GODUNOV) positivity_fallback(c, U, 1);
     riemann_y(c, rs);
     update_y(c, dt, Un, XY);
+    if (!c.quirks) floors(c, Un, XY);
 }
 
 inline void cons_to_prim(double gam, const double* U, double& rho, double& u, double& v, double& p) {
@@ -760,7 +813,7 @@ void or_scheme_index(int bc, int nx, int ny, int ng, const double* XY, double* S
 // Boundary: RegularBound.cpp:22-34; IDWBaseGFM/GFM_Bound.cpp:23-106;
 // LinearBaseGFM/GFM_Bound.cpp:58-575; BCGFM/BCGFM_Bound.cpp:81-175; HBCGFM_Bound.cpp:163-178.
 void or_boundary(int bc, int nx, int ny, int ng, double gam, double* U_OLD, double* U_NEW, const double* XY,
-                 const double* S) {
+                 const double* S, int quirks) {
     Grid g(nx, ny, ng);
     int W = or_sch_width(bc);
     if (bc == BC_REGULAR) {
@@ -803,6 +856,7 @@ void or_boundary(int bc, int nx, int ny, int ng, double gam, double* U_OLD, doub
                     P = (a1 * p[0] + a2 * p[1] + a3 * p[2] + a4 * p[3]) / (a1 + a2 + a3 + a4);
                 };
                 if (bc == BC_IDW) {
+                    if (!quirks && s[4] + s[5] + s[6] + s[7] == 0) continue;  // no fluid donor: keep the ghost
                     idw(0, 4, rw, uw, vw, pw);
                     double u_n = uw * n_x + vw * n_y;
                     double u_t = uw * n_y - vw * n_x;
@@ -998,13 +1052,13 @@ int or_run(int scheme, int rs, int lim, int bc, int edge, const double* edge_prm
            int nx, int ny, int ng, double gam, double cfl, double t_end, int max_iter, int init_boundary,
            double* U_OLD, double* U_NEW, const double* XY, const double* S, double* T, double* dt_log,
            double* t_log, double* mass_log) {
-    if (init_boundary) or_boundary(bc, nx, ny, ng, gam, U_OLD, U_NEW, XY, S);
+    if (init_boundary) or_boundary(bc, nx, ny, ng, gam, U_OLD, U_NEW, XY, S, quirks);
     double dt = or_compute_dt(L, H, nx, ny, ng, gam, cfl, U_OLD, XY);
     double now_t = 0;
     int iter = 0;
     while (now_t < t_end && iter < max_iter) {
         or_advance(scheme, rs, lim, quirks, L, H, nx, ny, ng, gam, dt, U_OLD, T, U_NEW, XY);
-        or_boundary(bc, nx, ny, ng, gam, U_OLD, U_NEW, XY, S);
+        or_boundary(bc, nx, ny, ng, gam, U_OLD, U_NEW, XY, S, quirks);
         or_global_boundary(edge, edge_prm, nx, ny, ng, gam, U_OLD, U_NEW);
         dt = or_compute_dt(L, H, nx, ny, ng, gam, cfl, U_OLD, XY);
         double m = or_mass(L, H, nx, ny, ng, U_OLD, XY);
