@@ -457,6 +457,7 @@ static void build_args(ibtfo_ctx* x) {
     a.gam = c.gamma; a.gm1 = c.gamma - 1; a.gp1 = c.gamma + 1; a.c1 = (c.gamma - 1) / 2 / c.gamma;
     a.dx = (c.hi_x - c.lo_x) / c.N_x;   // MUSCL_Scheme.cpp:64-65
     a.dy = (c.hi_y - c.lo_y) / c.N_y;
+    a.r_gam = 1.0 / a.gam; a.r_gm1 = 1.0 / a.gm1; a.r_dx = 1.0 / a.dx; a.r_dy = 1.0 / a.dy;
     a.cfl = c.CFL_number;
     a.t_end = c.Psy_time;
     a.max_iter = c.max_iter;

@@ -51,6 +51,7 @@ struct StepArgs {
     double* mass_part;  // one partial sum per block
     const double* corner;  // [2 sides][2ng rows][4]: U_NEW ring at columns ng / N_x+ng-1
     double gam, gm1, gp1, c1, dx, dy, cfl, t_end;
+    double r_gam, r_gm1, r_dx, r_dy;  // correctly rounded reciprocals of the constant divisors
     int max_iter;
     int quirks;
     int edge_fused;         // write the reference Global_Boundary ring inside the step

@@ -34,6 +34,47 @@ IB_HD float ib_sqrt(float x) { return __builtin_sqrtf(x); }
 IB_HD double ib_fabs(double x) { return __builtin_fabs(x); }
 IB_HD float ib_fabs(float x) { return __builtin_fabsf(x); }
 
+// --------------------------------------------------------------- division
+// Correctly rounded quotients at a fraction of the cost of the generic IEEE
+// sequence (v_div_scale x2 / v_rcp / 4 fma / mul / fma / v_div_fmas / v_div_fixup):
+//  * rcp_of(b): y ~ 1/b from v_rcp_f64 refined by two Newton steps (the
+//    refinement the IEEE sequence itself performs);
+//  * dq(a, r): q0 = a*y, residual rr = a - b*q0 (exact, one fma), q = q0 + rr*y
+//    (Markstein's final correction, also the IEEE sequence's), which is RN(a/b);
+//    the sign is taken from q0 so that a zero quotient keeps IEEE's sign;
+//  * divisions by run constants (dx, dy, gamma, gamma-1) use y = RN(1/d)
+//    computed on the host, with the same correction;
+//  * one reciprocal serves all quotients with the same denominator.
+// The IEEE sequence's operand scaling is omitted, so the result equals IEEE
+// division for operands and quotients of magnitude in [2^-1000, 2^1000]; a zero
+// or non-finite denominator gives NaN (IEEE: +-inf for a finite non-zero
+// numerator), which only arises once a state is already non-physical.
+template <typename T> struct Rcp { T b, y; };
+
+IB_HD Rcp<double> rcp_of(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    return Rcp<double>{b, y};
+#else
+    return Rcp<double>{b, 1.0 / b};
+#endif
+}
+IB_HD Rcp<float> rcp_of(float b) { return Rcp<float>{b, 1.0f / b}; }
+IB_HD double ib_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+IB_HD float ib_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+IB_HD double ib_copysign(double a, double b) { return __builtin_copysign(a, b); }
+IB_HD float ib_copysign(float a, float b) { return __builtin_copysignf(a, b); }
+template <typename T> IB_HD T dq(T a, const Rcp<T>& r) {
+    T q0 = a * r.y;
+    T rr = ib_fma(-q0, r.b, a);
+    return ib_copysign(ib_fma(rr, r.y, q0), q0);
+}
+template <typename T> IB_HD T ddiv(T a, T b) { return dq(a, rcp_of(b)); }
+
 // minmod/Slope_limiter.H:4: (a*b > 0) ? ((a/b > 1) ? b : a) : 0.
 // For a*b > 0 the rounded quotient exceeds 1 exactly when |a| > |b|(1 + 2^-53)
 // (the tie at 1 + 2^-53 rounds to 1.0), which is tested without a division:
@@ -51,19 +92,19 @@ template <typename T> IB_HD T lim_minmod(T a, T b) {
 // is the unparenthesised difference P - Q: `(r*r + r) * b / (r*r + 1)` becomes
 // (r*r + r)*P - Q/(r*r + 1).
 template <typename T> IB_HD T lim_vanleer_ref(T a, T P, T Q) {
-    T r = (a + T(1e-7)) / ((P - Q) + T(1e-7));
-    return (r > T(0)) ? ((r * r + r) * P - Q / (r * r + T(1))) : T(0);
+    T r = ddiv(a + T(1e-7), (P - Q) + T(1e-7));
+    return (r > T(0)) ? ((r * r + r) * P - ddiv(Q, r * r + T(1))) : T(0);
 }
 // the same limiter with its argument parenthesised (corrected mode)
 template <typename T> IB_HD T lim_vanleer_fixed(T a, T b) {
-    T r = (a + T(1e-7)) / (b + T(1e-7));
-    return (r > T(0)) ? ((r * r + r) * b / (r * r + T(1))) : T(0);
+    T r = ddiv(a + T(1e-7), b + T(1e-7));
+    return (r > T(0)) ? ddiv((r * r + r) * b, r * r + T(1)) : T(0);
 }
 // VanAlbada/Slope_limiter.H:4 does not compile (undeclared r); its evident
 // intent, with r defined as in the sibling limiter.
 template <typename T> IB_HD T lim_vanalbada(T a, T b) {
-    T r = (a + T(1e-7)) / (b + T(1e-7));
-    return (r > T(0)) ? ((r + ib_fabs(r)) * b / (T(1) + ib_fabs(r))) : T(0);
+    T r = ddiv(a + T(1e-7), b + T(1e-7));
+    return (r > T(0)) ? ddiv((r + ib_fabs(r)) * b, T(1) + ib_fabs(r)) : T(0);
 }
 template <int LIM, typename T> IB_HD T slope(T a, T P, T Q, bool quirks) {
     if (LIM == LIM_MINMOD) return lim_minmod(a, P - Q);
@@ -83,90 +124,107 @@ template <int LIM, typename T> IB_HD void recon(const T* u0, const T* u1, const 
     }
 }
 
-struct Prim4 {};
+// Gas and mesh constants, with the reciprocals of the constant divisors.
+template <typename T> struct GasT {
+    T gam, gm1, gp1, c1;  // gamma, gamma-1, gamma+1, (gamma-1)/2/gamma
+    Rcp<T> r_gam, r_gm1, r_dx, r_dy;
+};
 
-// Conserve2Flux.cpp:66-74 (x) / :101-109 (y): physical flux of a conserved state.
+// primitive view of a conserved state (rho, u, v, p), with 1/rho kept for reuse
+template <typename T> struct PState {
+    T rho, u, v, p, E;
+    Rcp<T> ir;
+};
+// Conserve2Flux.cpp:66-69: u = rho u / rho, v = rho v / rho, p = (gamma-1)(E - rho(u^2+v^2)/2)
+template <typename T> IB_HD PState<T> pstate(T gm1, const T* U) {
+    PState<T> s;
+    s.rho = U[0];
+    s.ir = rcp_of(s.rho);
+    s.u = dq(U[1], s.ir);
+    s.v = dq(U[2], s.ir);
+    s.p = gm1 * (U[3] - T(0.5) * s.rho * (s.u * s.u + s.v * s.v));
+    s.E = U[3];
+    return s;
+}
+// sqrt(gamma p / rho), where the reference calls pow(gamma p / rho, 0.5)
+template <typename T> IB_HD T sound(const GasT<T>& g, const PState<T>& s) { return ib_sqrt(dq(g.gam * s.p, s.ir)); }
+
+// Conserve2Flux.cpp:71-74 (x) / :106-109 (y): physical flux of a conserved state.
 // The tangential momentum term is rho*u*v evaluated left to right in both directions.
 template <bool Y, typename T> IB_HD void phys_flux(T rho, T u, T v, T p, T E, T* F) {
     if (!Y) { F[0] = rho * u; F[1] = rho * u * u + p; F[2] = rho * u * v; F[3] = (E + p) * u; }
     else { F[0] = rho * v; F[1] = rho * u * v; F[2] = rho * v * v + p; F[3] = (E + p) * v; }
 }
+template <bool Y, typename T> IB_HD void phys_flux(const PState<T>& s, T* F) { phys_flux<Y>(s.rho, s.u, s.v, s.p, s.E, F); }
 template <typename T> IB_HD void primitives(T gm1, const T* U, T& rho, T& u, T& v, T& p) {
     rho = U[0]; u = U[1] / rho; v = U[2] / rho;
     p = gm1 * (U[3] - T(0.5) * rho * (u * u + v * v));
 }
 template <bool Y, typename T> IB_HD void flux_of(T gm1, const T* U, T* F) {
-    T rho, u, v, p;
-    primitives(gm1, U, rho, u, v, p);
-    phys_flux<Y>(rho, u, v, p, U[3], F);
+    PState<T> s = pstate(gm1, U);
+    phys_flux<Y>(s, F);
 }
 
-// Corrected mode only (no reference counterpart): positivity of a conserved
-// state, the per-face first-order fallback and the density/pressure floors.
-template <typename T> IB_HD bool physical(T gm1, const T* q) {
-    T rho = q[0], u = q[1] / rho, v = q[2] / rho;
-    T p = gm1 * (q[3] - T(0.5) * rho * (u * u + v * v));
-    return rho > T(0) && p > T(0);
-}
-template <typename T> IB_HD void face_fallback(T gm1, const T* a, const T* b, T* lo, T* hi) {
-    if (physical(gm1, lo) && physical(gm1, hi)) return;
-#pragma unroll
-    for (int k = 0; k < 4; k++) { lo[k] = a[k]; hi[k] = b[k]; }
-}
+// Corrected mode only (no reference counterpart): positivity of a state, the
+// per-face first-order fallback and the density/pressure floors.
+template <typename T> IB_HD bool physical(const PState<T>& s) { return s.rho > T(0) && s.p > T(0); }
 constexpr double RHO_FLOOR = 1e-8, P_FLOOR = 1e-8;
-IB_HD void apply_floor(double gm1, double* q) {
+IB_HD void apply_floor(double gm1, const Rcp<double>& r_gm1, double* q) {
     if (!(q[0] > RHO_FLOOR)) q[0] = RHO_FLOOR;
-    double u = q[1] / q[0], v = q[2] / q[0];
+    Rcp<double> ir = rcp_of(q[0]);
+    double u = dq(q[1], ir), v = dq(q[2], ir);
     double ke = 0.5 * q[0] * (u * u + v * v);
     double p = gm1 * (q[3] - ke);
-    if (!(p > P_FLOOR)) q[3] = P_FLOOR / gm1 + ke;
+    if (!(p > P_FLOOR)) q[3] = dq(P_FLOOR, r_gm1) + ke;
 }
-
-struct Gas {
-    double gam, gm1, gp1, c1;  // gamma, gamma-1, gamma+1, (gamma-1)/2/gamma
-};
-template <typename T> struct GasT { T gam, gm1, gp1, c1; };
 
 // HLL/Riemann_solver.cpp:131-168 (x), :202-235 (y). Returns true when the
 // first branch (S_lo >= 0) is taken; F is then F(U_lo) (the y solver's caller
 // decides what that branch stores).
-template <bool Y, typename T> IB_HD bool hll_face(const GasT<T>& g, const T* Ulo, const T* Uhi, T* F, bool quirks) {
-    T rl, ul, vl, pl, rh, uh, vh, ph;
-    primitives(g.gm1, Ulo, rl, ul, vl, pl);
-    primitives(g.gm1, Uhi, rh, uh, vh, ph);
-    T unl = Y ? vl : ul, unh = Y ? vh : uh;
-    T al = ib_sqrt(g.gam * pl / rl), ah = ib_sqrt(g.gam * ph / rh);
-    T rho_bar = T(0.5) * (rl + rh);
+// (gamma+1) p* / 2 / gamma / p_K + (gamma-1)/2/gamma, in the reference's order
+template <typename T> IB_HD T shock_factor_arg(const GasT<T>& g, T p_star, T pK) {
+    T t = g.gp1 * p_star * T(0.5);
+    t = dq(t, g.r_gam);
+    t = ddiv(t, pK);
+    return t + g.c1;
+}
+template <bool Y, typename T>
+IB_HD bool hll_face(const GasT<T>& g, const PState<T>& L, const PState<T>& R, const T* Ulo, const T* Uhi, T* F,
+                    bool quirks) {
+    T unl = Y ? L.v : L.u, unh = Y ? R.v : R.u;
+    T al = sound(g, L), ah = sound(g, R);
+    T rho_bar = T(0.5) * (L.rho + R.rho);
     T a_bar = T(0.5) * (al + ah);
-    T p_star = T(0.5) * (pl + ph) + T(0.5) * (unl - unh) * (rho_bar * a_bar);
+    T p_star = T(0.5) * (L.p + R.p) + T(0.5) * (unl - unh) * (rho_bar * a_bar);
     T Sl, Sh;
     if (quirks) {
-        Sl = unl - al * ib_sqrt(g.gp1 * p_star / T(2) / g.gam / pl + g.c1);
-        Sh = unh + ah * ib_sqrt(g.gp1 * p_star / T(2) / g.gam / ph + g.c1);
+        Sl = unl - al * ib_sqrt(shock_factor_arg(g, p_star, L.p));
+        Sh = unh + ah * ib_sqrt(shock_factor_arg(g, p_star, R.p));
     } else {
         // corrected estimate (Toro, Riemann Solvers, 3rd ed., eq. 10.59-10.60): p* >= 0 and
         // the shock factor only where p* exceeds the side's pressure
         p_star = p_star > T(0) ? p_star : T(0);
-        T ql = p_star <= pl ? T(1) : ib_sqrt(g.gp1 * p_star / T(2) / g.gam / pl + g.c1);
-        T qh = p_star <= ph ? T(1) : ib_sqrt(g.gp1 * p_star / T(2) / g.gam / ph + g.c1);
+        T ql = p_star <= L.p ? T(1) : ib_sqrt(shock_factor_arg(g, p_star, L.p));
+        T qh = p_star <= R.p ? T(1) : ib_sqrt(shock_factor_arg(g, p_star, R.p));
         Sl = unl - al * ql;
         Sh = unh + ah * qh;
     }
     T Fl[4], Fh[4];
-    phys_flux<Y>(rl, ul, vl, pl, Ulo[3], Fl);
+    phys_flux<Y>(L, Fl);
     if (Sl >= T(0)) {
 #pragma unroll
         for (int k = 0; k < 4; k++) F[k] = Fl[k];
         return true;
     }
-    phys_flux<Y>(rh, uh, vh, ph, Uhi[3], Fh);
+    phys_flux<Y>(R, Fh);
     if (Sh <= T(0)) {
 #pragma unroll
         for (int k = 0; k < 4; k++) F[k] = Fh[k];
         return false;
     }
+    Rcp<T> iw = rcp_of(Sh - Sl);
 #pragma unroll
-    for (int k = 0; k < 4; k++) F[k] = (Sh * Fl[k] - Sl * Fh[k] + Sh * Sl * (Uhi[k] - Ulo[k])) / (Sh - Sl);
+    for (int k = 0; k < 4; k++) F[k] = dq(Sh * Fl[k] - Sl * Fh[k] + Sh * Sl * (Uhi[k] - Ulo[k]), iw);
     return false;
 }
 
@@ -182,22 +240,21 @@ template <bool Y, typename T> IB_HD void sw_half(const GasT<T>& g, T rho, T un, 
     T l4 = plus ? T(0.5) * (q + s4) : T(0.5) * (q - s4);
     T u2 = Y ? (ut * ut + un * un) : (un * un + ut * ut);
     T twogm1 = T(2) * g.gm1;
-    out[0] = rho * (twogm1 * l1 + l3 + l4) / T(2) / g.gam;
-    out[1] = rho * (twogm1 * un * l1 + (un - a) * l3 + (un + a) * l4) / T(2) / g.gam;
-    out[2] = rho * (twogm1 * ut * l1 + l3 * ut + l4 * ut) / T(2) / g.gam;
-    out[3] = rho * (g.gm1 * u2 * l1 + (h - a * un) * l3 + (h + a * un) * l4) / T(2) / g.gam;
+    // rho * (...) / 2 / gamma
+    out[0] = dq(rho * (twogm1 * l1 + l3 + l4) * T(0.5), g.r_gam);
+    out[1] = dq(rho * (twogm1 * un * l1 + (un - a) * l3 + (un + a) * l4) * T(0.5), g.r_gam);
+    out[2] = dq(rho * (twogm1 * ut * l1 + l3 * ut + l4 * ut) * T(0.5), g.r_gam);
+    out[3] = dq(rho * (g.gm1 * u2 * l1 + (h - a * un) * l3 + (h + a * un) * l4) * T(0.5), g.r_gam);
 }
-template <bool Y, typename T> IB_HD void sw_face(const GasT<T>& g, const T* Ulo, const T* Uhi, T* F) {
-    T rl, ul, vl, pl, rh, uh, vh, ph;
-    primitives(g.gm1, Ulo, rl, ul, vl, pl);
-    primitives(g.gm1, Uhi, rh, uh, vh, ph);
-    T al = ib_sqrt(g.gam * pl / rl), ah = ib_sqrt(g.gam * ph / rh);
-    T el = pl / g.gm1 / rl, eh = ph / g.gm1 / rh;
-    T hl = el + ul * ul / T(2) + vl * vl / T(2) + pl / rl;
-    T hh = eh + uh * uh / T(2) + vh * vh / T(2) + ph / rh;
+template <bool Y, typename T> IB_HD void sw_face(const GasT<T>& g, const PState<T>& L, const PState<T>& R, T* F) {
+    T al = sound(g, L), ah = sound(g, R);
+    // e = p / (gamma-1) / rho ; h = e + u*u/2 + v*v/2 + p/rho
+    T el = dq(dq(L.p, g.r_gm1), L.ir), eh = dq(dq(R.p, g.r_gm1), R.ir);
+    T hl = el + L.u * L.u * T(0.5) + L.v * L.v * T(0.5) + dq(L.p, L.ir);
+    T hh = eh + R.u * R.u * T(0.5) + R.v * R.v * T(0.5) + dq(R.p, R.ir);
     T fp[4], fn[4];
-    sw_half<Y>(g, rl, Y ? vl : ul, Y ? ul : vl, al, hl, true, fp);
-    sw_half<Y>(g, rh, Y ? vh : uh, Y ? uh : vh, ah, hh, false, fn);
+    sw_half<Y>(g, L.rho, Y ? L.v : L.u, Y ? L.u : L.v, al, hl, true, fp);
+    sw_half<Y>(g, R.rho, Y ? R.v : R.u, Y ? R.u : R.v, ah, hh, false, fn);
     if (!Y) {
         F[0] = fp[0] + fn[0]; F[1] = fp[1] + fn[1]; F[2] = fp[2] + fn[2]; F[3] = fp[3] + fn[3];
     } else {

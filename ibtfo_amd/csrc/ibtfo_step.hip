@@ -43,9 +43,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 // maximum when the candidate is NaN).
 __device__ __forceinline__ void dt_terms(double gam, double gm1, const double* U, double& amax, double& umax,
                                          double& vmax, int& bad) {
-    double rho = U[0], u = U[1] / U[0], v = U[2] / U[0];
-    double p = gm1 * (U[3] - 0.5 * rho * (u * u + v * v));
-    double a = ib_sqrt(gam * p / rho);
+    PState<double> s = pstate(gm1, U);
+    double rho = s.rho, u = s.u, v = s.v, p = s.p;
+    double a = ib_sqrt(dq(gam * p, s.ir));
     amax = (amax < a) ? a : amax;
     double au = ib_fabs(u), av = ib_fabs(v);
     umax = (umax < au) ? au : umax;
@@ -118,67 +118,71 @@ __device__ void finish_block(Ctl* ctl, double* mass_part, int nblocks, int bid, 
     }
 }
 
-template <int SCH, int RS, int LIM>
-__device__ __forceinline__ void xface(const GasT<double>& g, double dt, double dx, int quirks, const double* u0,
-                                      const double* u1, const double* u2, const double* u3, double* F) {
-    double UL[4], UR[4];
+// Reconstructed (and, for MUSCL-Hancock, predicted) face states of one face from
+// the 4 cells around it, with the corrected mode's positivity fallback.
+template <int SCH, int LIM, bool Y>
+__device__ __forceinline__ void face_states(const GasT<double>& g, double dt, int quirks, const double* u0,
+                                            const double* u1, const double* u2, const double* u3, double* Ulo,
+                                            double* Uhi, PState<double>& L, PState<double>& R) {
     if (SCH == SCH_GODUNOV) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) { UL[k] = u1[k]; UR[k] = u2[k]; }
+        for (int k = 0; k < 4; k++) { Ulo[k] = u1[k]; Uhi[k] = u2[k]; }
     } else {
-        recon<LIM>(u0, u1, u2, u3, UL, UR, quirks != 0);
+        recon<LIM>(u0, u1, u2, u3, Ulo, Uhi, quirks != 0);
     }
-    if (SCH == SCH_MHM) {  // MHM_Scheme.cpp:61-96 (same increment on both sides of the face)
-        double FL[4], FR[4];
-        flux_of<false>(g.gm1, UL, FL);
-        flux_of<false>(g.gm1, UR, FR);
+    L = pstate(g.gm1, Ulo);
+    R = pstate(g.gm1, Uhi);
+    if (SCH == SCH_MHM) {  // MHM_Scheme.cpp:61-96 / :148-183 (same increment on both sides of the face)
+        double Fl[4], Fh[4];
+        phys_flux<Y>(L, Fl);
+        phys_flux<Y>(R, Fh);
+        const Rcp<double>& rh = Y ? g.r_dy : g.r_dx;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            double inc = 0.5 * (FL[k] - FR[k]) * dt / dx;
-            UL[k] = UL[k] + inc;
-            UR[k] = UR[k] + inc;
+            double inc = dq(0.5 * (Fl[k] - Fh[k]) * dt, rh);
+            Ulo[k] = Ulo[k] + inc;
+            Uhi[k] = Uhi[k] + inc;
         }
+        L = pstate(g.gm1, Ulo);
+        R = pstate(g.gm1, Uhi);
     }
-    if (SCH != SCH_GODUNOV && !quirks) face_fallback(g.gm1, u1, u2, UL, UR);
-    if (RS == RS_SW) sw_face<false>(g, UL, UR, F);
+    if (SCH != SCH_GODUNOV && !quirks && !(physical(L) && physical(R))) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) { Ulo[k] = u1[k]; Uhi[k] = u2[k]; }
+        L = pstate(g.gm1, Ulo);
+        R = pstate(g.gm1, Uhi);
+    }
+}
+
+template <int SCH, int RS, int LIM>
+__device__ __forceinline__ void xface(const GasT<double>& g, double dt, int quirks, const double* u0,
+                                      const double* u1, const double* u2, const double* u3, double* F) {
+    double UL[4], UR[4];
+    PState<double> L, R;
+    face_states<SCH, LIM, false>(g, dt, quirks, u0, u1, u2, u3, UL, UR, L, R);
+    if (RS == RS_SW) sw_face<false>(g, L, R, F);
     else if (RS == RS_PVRS) { bool s; pvrs_face<false>(g, UL, UR, F, s); }
-    else hll_face<false>(g, UL, UR, F, quirks != 0);
+    else hll_face<false>(g, L, R, UL, UR, F, quirks != 0);
 }
 
 // y face at j+1/2 of column c; Fx is this cell's x-face flux in the same row,
 // which is what the reference leaves in the aliased G_OLD slot when its y solver
 // stores nothing (HLL/Riemann_solver.cpp:224-226, PVRS/Riemann_solver.cpp:550).
 template <int SCH, int RS, int LIM>
-__device__ __forceinline__ void yface(const GasT<double>& g, double dt, double dy, int quirks, const double* u0,
+__device__ __forceinline__ void yface(const GasT<double>& g, double dt, int quirks, const double* u0,
                                       const double* u1, const double* u2, const double* u3, const double* Fx,
                                       double* G) {
     double UD[4], UU[4];
-    if (SCH == SCH_GODUNOV) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) { UD[k] = u1[k]; UU[k] = u2[k]; }
-    } else {
-        recon<LIM>(u0, u1, u2, u3, UD, UU, quirks != 0);
-    }
-    if (SCH == SCH_MHM) {  // MHM_Scheme.cpp:148-183
-        double GD[4], GU[4];
-        flux_of<true>(g.gm1, UD, GD);
-        flux_of<true>(g.gm1, UU, GU);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            double inc = 0.5 * (GD[k] - GU[k]) * dt / dy;
-            UD[k] = UD[k] + inc;
-            UU[k] = UU[k] + inc;
-        }
-    }
-    if (SCH != SCH_GODUNOV && !quirks) face_fallback(g.gm1, u1, u2, UD, UU);
+    PState<double> D, Uu;
+    face_states<SCH, LIM, true>(g, dt, quirks, u0, u1, u2, u3, UD, UU, D, Uu);
     if (RS == RS_SW) {
-        sw_face<true>(g, UD, UU, G);
+        sw_face<true>(g, D, Uu, G);
     } else if (RS == RS_PVRS) {
         bool stray;
         pvrs_face<true>(g, UD, UU, G, stray);
         if (stray && quirks) G[3] = Fx[3];
     } else {
-        bool up = hll_face<true>(g, UD, UU, G, quirks != 0);
+        bool up = hll_face<true>(g, D, Uu, UD, UU, G, quirks != 0);
         if (up && quirks) {
 #pragma unroll
             for (int k = 0; k < 4; k++) G[k] = Fx[k];
@@ -200,7 +204,12 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
     const double* __restrict__ Ui = a.buf[cur];
     double* __restrict__ Uo = a.buf[cur ^ 1];
     const long long PL = L.plane;
-    GasT<double> g{a.gam, a.gm1, a.gp1, a.c1};
+    GasT<double> g;
+    g.gam = a.gam; g.gm1 = a.gm1; g.gp1 = a.gp1; g.c1 = a.c1;
+    g.r_gam = Rcp<double>{a.gam, a.r_gam};
+    g.r_gm1 = Rcp<double>{a.gm1, a.r_gm1};
+    g.r_dx = Rcp<double>{a.dx, a.r_dx};
+    g.r_dy = Rcp<double>{a.dy, a.r_dy};
     const double dx = a.dx, dy = a.dy;
     const int ng = L.ng, nx = L.nx, ny = L.ny;
 
@@ -280,8 +289,8 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
             xp2[k] = rb[b][k][lane + 3];
         }
         double F[4], G[4];
-        xface<SCH, RS, LIM>(g, dt, dx, a.quirks, xm, w[1], xp, xp2, F);
-        yface<SCH, RS, LIM>(g, dt, dy, a.quirks, w[0], w[1], w[2], w[3], F, G);
+        xface<SCH, RS, LIM>(g, dt, a.quirks, xm, w[1], xp, xp2, F);
+        yface<SCH, RS, LIM>(g, dt, a.quirks, w[0], w[1], w[2], w[3], F, G);
 
         if (j >= j0) {
 #pragma unroll
@@ -295,10 +304,10 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
                         double Fm = fb[k][lane - 1];
-                        double v = w[1][k] - dt * (F[k] - Fm) / dx;
-                        un[k] = v - dt * (G[k] - Gp[k]) / dy;
+                        double v = w[1][k] - dq(dt * (F[k] - Fm), g.r_dx);
+                        un[k] = v - dq(dt * (G[k] - Gp[k]), g.r_dy);
                     }
-                    if (!a.quirks) apply_floor(a.gm1, un);
+                    if (!a.quirks) apply_floor(a.gm1, g.r_gm1, un);
                 } else {
 #pragma unroll
                     for (int k = 0; k < 4; k++) un[k] = w[1][k];
