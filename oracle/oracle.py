@@ -265,9 +265,9 @@ def parse_ref(path, strings=False):
     n = L.ref_parse(path.encode(), int(strings), buf, len(buf))
     assert n >= 0
     out = {}
-    for line in buf.value.decode("utf-8", "surrogateescape").split("\n"):
-        if not line:
+    for rec in buf.value.decode("utf-8", "surrogateescape").split("\x1e"):
+        if not rec:
             continue
-        k, v = line.split("\t", 1)
+        k, v = rec.split("\x1f", 1)
         out[k] = v if strings else float(v)
     return out

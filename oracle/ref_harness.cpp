@@ -111,20 +111,20 @@ void ref_advance(double L, double H, int nx, int ny, int ng, double gamma, doubl
     TimeAdvance(L, H, nx, ny, ng, gamma, dt, U_OLD, U_TMP, U_NEW, XY, nullptr, 1, 0);
 }
 
-// Parsed .inp numbers()/strings() maps, written as "key\tvalue\n" lines into buf.
+// Parsed .inp numbers()/strings() maps, written as key 0x1f value 0x1e records into buf.
 int ref_parse(const char* inp, int strings, char* buf, int cap) {
     ParamReader rd;
     std::string out;
     rd.open(inp);
     if (strings) {
         std::map<std::string, std::string> m = rd.strings();
-        for (auto& kv : m) out += kv.first + "\t" + kv.second + "\n";
+        for (auto& kv : m) out += kv.first + "\x1f" + kv.second + "\x1e";
     } else {
         std::map<std::string, double> m = rd.numbers();
         char tmp[64];
         for (auto& kv : m) {
             snprintf(tmp, sizeof tmp, "%.17g", kv.second);
-            out += kv.first + "\t" + tmp + "\n";
+            out += kv.first + "\x1f" + tmp + "\x1e";
         }
     }
     if ((int)out.size() + 1 > cap) return -(int)out.size() - 1;

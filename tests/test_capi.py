@@ -1,0 +1,52 @@
+"""The C-ABI library loads without a GPU and exports every function that
+include/ibtfo.h declares; the CPU-only parts of the ABI behave."""
+import ctypes as C
+import os
+import re
+
+import ibtfo_amd as ib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, "include", "ibtfo.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ibtfo_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared()
+    assert len(names) >= 25
+    L = ib.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_config_struct_matches_header():
+    # field order and size of ibtfo_config as the header declares it
+    assert C.sizeof(ib.Config) == 4 * 4 + 7 * 8 + 8 * 4 + 4 * 8
+
+
+def test_defaults_are_reference_driver_defaults():
+    c = ib.config()
+    assert (c.N_x, c.N_y, c.max_iter, c.num_ghost_cell) == (500, 200, 100000, 2)
+    assert (c.CFL_number, c.Psy_time, c.gamma, c.hi_x, c.hi_y) == (0.3, 1.0, 1.4, 1.0, 1.0)
+    assert c.quirks == 1
+
+
+def test_bad_config_rejected_before_any_device_call():
+    for kw in [dict(num_ghost_cell=1), dict(N_x=2), dict(scheme=9), dict(gamma=1.0), dict(Time="RK2")]:
+        cfg = ib.config(**kw)
+        h = C.c_void_p()
+        rc = ib.lib().ibtfo_create(C.byref(cfg), C.byref(h))
+        assert rc == ib.E_ARG, kw
+        assert ib.lib().ibtfo_last_error()
+
+
+def test_footprint_scales_with_grid():
+    a = ib.lib().ibtfo_footprint(C.byref(ib.config(N_x=1024, N_y=512)))
+    b = ib.lib().ibtfo_footprint(C.byref(ib.config(N_x=2048, N_y=1024)))
+    assert 3.5 < b / a < 4.5
+    big = ib.lib().ibtfo_footprint(C.byref(ib.config(N_x=4096, N_y=2048)))
+    assert 0.5e9 < big < 2e9

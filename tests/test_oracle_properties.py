@@ -1,0 +1,190 @@
+"""Closed-form and invariant checks of the CPU oracle: what the mathematics fixes
+independently of the reference's code (each chosen so that a dropped term, a
+wrong sign or index, or a transposed operand fails it)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from exact_riemann import sample, star_state
+
+
+def test_exact_riemann_reference_values():
+    p, u = star_state((1.0, 0.0, 1.0), (0.125, 0.0, 0.1))
+    assert abs(p - 0.30313) < 1e-5 and abs(u - 0.92745) < 1e-5
+
+
+def test_minmod_table(oracle_mod):
+    O = oracle_mod
+    vals = [-3.0, -2.0, -1.0, -0.5, 0.0, 0.5, 1.0, 2.0, 3.0]
+    for a in vals:
+        for b in vals:
+            want = 0.0 if a * b <= 0 else (a if abs(a) <= abs(b) else b)
+            assert O.lib().or_limiter(0, 1, a, b, 0.0) == want, (a, b)
+
+
+def test_vanleer_macro_defect_and_fixed_form(oracle_mod):
+    O = oracle_mod
+    L = O.lib()
+    # uniform data: a = b = 0 (P = Q = u). The parenthesised formula gives 0 slope;
+    # the reference's expanded macro gives (r^2+r)P - Q/(r^2+1) with r = 1: 1.5 u.
+    for u in [0.3, 1.0, 2.5]:
+        assert L.or_limiter(1, 0, 0.0, u, u) == 0.0
+        assert L.or_limiter(1, 1, 0.0, u, u) == 2 * u - u / 2
+    # smooth data: fixed form = (r^2 + r) b / (r^2 + 1) with r = (a+1e-7)/(b+1e-7)
+    rng = np.random.default_rng(3)
+    for a, b in rng.normal(size=(200, 2)):
+        r = (a + 1e-7) / (b + 1e-7)
+        want = (r * r + r) * b / (r * r + 1) if r > 0 else 0.0
+        assert L.or_limiter(1, 0, a, b, 0.0) == want
+
+
+def _cons(r, u, v, p, g=1.4):
+    return np.array([r, r * u, r * v, p / (g - 1) + 0.5 * r * (u * u + v * v)])
+
+
+def test_physical_flux_textbook(oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        r, p = rng.uniform(0.1, 3, 2)
+        u, v = rng.normal(size=2)
+        U = _cons(r, u, v, p)
+        F, G = np.zeros(4), np.zeros(4)
+        O.lib().or_flux(0, C.c_double(1.4), O.ptr(U), O.ptr(F))
+        O.lib().or_flux(1, C.c_double(1.4), O.ptr(U), O.ptr(G))
+        E = U[3]
+        assert np.allclose(F, [r * u, r * u * u + p, r * u * v, (E + p) * u], rtol=1e-13, atol=1e-14)
+        assert np.allclose(G, [r * v, r * u * v, r * v * v + p, (E + p) * v], rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("rs", [0, 3])  # HLL, Steger-Warming
+@pytest.mark.parametrize("ydir", [0, 1])
+def test_consistency_F_of_equal_states(oracle_mod, rs, ydir):
+    """A two-state flux with equal states is the physical flux (Steger-Warming up
+    to its eigenvalue smoothing, delta = 1e-6)."""
+    O = oracle_mod
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        U = _cons(rng.uniform(0.2, 2), rng.normal(), rng.normal(), rng.uniform(0.2, 2))
+        F, Fp = np.zeros(4), np.zeros(4)
+        O.lib().or_face_flux(rs, ydir, C.c_double(1.4), O.ptr(U), O.ptr(U), O.ptr(F))
+        O.lib().or_flux(ydir, C.c_double(1.4), O.ptr(U), O.ptr(Fp))
+        tol = 1e-13 if rs == 0 else 1e-9
+        assert np.allclose(F, Fp, rtol=tol, atol=tol), (F, Fp)
+
+
+def test_hll_supersonic_upwinding(oracle_mod):
+    O = oracle_mod
+    UL, UR = _cons(1.0, 5.0, 0.0, 1.0), _cons(0.5, 4.0, 0.1, 0.4)
+    F, FL = np.zeros(4), np.zeros(4)
+    O.lib().or_face_flux(0, 0, C.c_double(1.4), O.ptr(UL), O.ptr(UR), O.ptr(F))
+    O.lib().or_flux(0, C.c_double(1.4), O.ptr(UL), O.ptr(FL))
+    assert np.array_equal(F, FL)
+
+
+@pytest.mark.parametrize("scheme", ["MUSCL", "MHM", "Godunov", "Lax"])
+@pytest.mark.parametrize("rs", ["HLL", "StegerWarming", "PVRS"])
+def test_free_stream_preserved_bitwise(oracle_mod, scheme, rs):
+    O = oracle_mod
+    c = O.Case(N_x=24, N_y=16, hi_y=0.5, Scheme=scheme, Riemann=rs, case="uniform", edge="periodic",
+               Psy_time=10.0, extra=dict(rho=1.3, u=0.7, v=-0.4, p=0.9))
+    st = O.setup(c)
+    U0 = st["U_OLD"].copy()
+    r = O.run_oracle(c, 15, state=st)
+    assert r["steps"] == 15
+    assert np.array_equal(r["U_OLD"], U0)
+
+
+def test_x_sod_is_y_invariant_and_conservative(oracle_mod):
+    O = oracle_mod
+    c = O.Case(N_x=200, N_y=8, hi_y=0.04, Scheme="MHM", case="sodx", CFL_number=0.5, Psy_time=0.15)
+    r = O.run_oracle(c, 100000)
+    U = r["U_OLD"].reshape(c.N, c.M, 4)
+    inner = U[2:-2, 2:-2]
+    assert all(np.array_equal(inner[j], inner[0]) for j in range(inner.shape[0]))
+    # mass conserved until waves reach the walls (shock reaches x = 1 at t ~ 0.28)
+    m0 = 0.5 * 1.0 + 0.5 * 0.125
+    mass = np.sum(inner[..., 0]) * (1.0 / 200) * (0.04 / 8) / 0.04
+    assert abs(mass - m0) < 1e-13
+
+
+def test_compute_dt_closed_form(oracle_mod):
+    O = oracle_mod
+    c = O.Case(N_x=20, N_y=10, hi_y=0.5, case="uniform", extra=dict(rho=1.2, u=0.3, v=-0.8, p=2.0))
+    st = O.setup(c)
+    a = (1.4 * 2.0 / 1.2) ** 0.5
+    want = c.CFL_number * (1.0 / 20) / (a + 0.8)
+    assert abs(O.compute_dt(c, st["U_OLD"], st["XY"]) - want) <= 1e-15 * want
+
+
+@pytest.mark.parametrize("nx", [100, 200, 400])
+def test_sod_converges_to_exact_solution(oracle_mod, nx):
+    """C1's scheme (MHM + minmod + HLL) against the exact Riemann solution:
+    L1(rho) falls with resolution (first-order-or-better at the discontinuities)."""
+    O = oracle_mod
+    c = O.Case(N_x=nx, N_y=4, hi_y=4.0 / nx, Scheme="MHM", case="sodx", CFL_number=0.5, Psy_time=0.2)
+    r = O.run_oracle(c, 100000)
+    U = r["U_OLD"].reshape(c.N, c.M, 4)[2, 2:-2]
+    dx = 1.0 / nx
+    x = (np.arange(nx) + 0.5) * dx
+    # the reference's clock: each step advances by the dt computed after it, so the
+    # state after n steps is at sum of the dt actually used (dt0 .. dt_{n-1})
+    dt0 = O.compute_dt(c, O.setup(c)["U_OLD"], O.setup(c)["XY"])
+    t_state = dt0 + float(np.sum(r["dt"][:-1]))
+    ex = sample((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), (x - 0.5) / t_state)
+    l1 = np.mean(np.abs(U[:, 0] - ex[:, 0]))
+    bound = {100: 0.02, 200: 0.011, 400: 0.006}[nx]
+    assert l1 < bound, l1
+
+
+def test_classification_brute_force(oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(2)
+    c = O.Case(N_x=30, N_y=20, hi_y=20 / 30, case="uniform")
+    phi = rng.normal(size=c.M * c.N) + 0.3
+    c.phi = phi
+    st = O.setup(c)
+    t = st["XY"][5::6].reshape(c.N, c.M)
+    P = phi.reshape(c.N, c.M)
+    ng = 2
+    for j in range(c.N):
+        for i in range(c.M):
+            if not (ng <= i < c.N_x + ng and ng <= j < c.N_y + ng):
+                want = 2
+            elif P[j, i] >= 0:
+                want = 0
+            else:
+                want = 1
+                for k in range(1, ng + 1):
+                    for (ii, jj) in [(i - k, j), (i + k, j), (i, j - k), (i, j + k)]:
+                        if ng <= ii < c.N_x + ng and ng <= jj < c.N_y + ng and P[jj, ii] >= 0:
+                            want = 2
+            assert t[j, i] == want, (i, j)
+
+
+def test_idw_fill_reproduces_wall_parallel_flow(oracle_mod):
+    """Uniform flow along an inclined channel: the mirror-point IDW average of a
+    uniform field is that field, and its normal velocity is zero, so ghosts take
+    the fluid state."""
+    O = oracle_mod
+    ang = np.deg2rad(30.0)
+    u, v = 0.8 * np.cos(ang), 0.8 * np.sin(ang)
+    c = O.Case(N_x=60, N_y=60, hi_y=1.0, Boundary="IDWBaseGFM", case="incline", extra=dict(angle=30.0))
+    st = O.setup(c)
+    U = st["U_OLD"].reshape(-1, 4)
+    Uf = _cons(1.1, u, v, 0.7)
+    U[:] = Uf
+    st["U_NEW"][:] = st["U_OLD"]
+    O.lib().or_boundary(1, c.N_x, c.N_y, 2, C.c_double(1.4), O.ptr(st["U_OLD"]), O.ptr(st["U_NEW"]),
+                        O.ptr(st["XY"]), O.ptr(st["S"]), 1)
+    t = st["XY"][5::6].reshape(c.N, c.M)
+    S = st["S"].reshape(c.N, c.M, 8)
+    UU = st["U_OLD"].reshape(c.N, c.M, 4)
+    n = 0
+    for j in range(2, c.N - 2):
+        for i in range(2, c.M - 2):
+            if t[j, i] == 2 and S[j, i, 4:8].sum() > 0:
+                assert np.allclose(UU[j, i], Uf, rtol=1e-9, atol=1e-9), (i, j, UU[j, i])
+                n += 1
+    assert n > 50
