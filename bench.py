@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: FP64 cell-updates/s of the fused Euler step on one MI355X.
 
-Workload (BASELINE config "C4"): 4096 x 2048 cells on [0,2] x [0,1], Mach 3
+Workload: 4096 x 2048 cells on [0,2] x [0,1], Mach 3
 inflow (rho, u, v, p) = (1.4, 3, 0, 1) past 9 seeded bodies (seed 20261202:
 circles and convex polygons, exact signed-distance level set), MUSCL + van Leer
 limiter + HLL flux, IDW ghost-fluid boundary, supersonic inflow / zero-gradient
@@ -34,11 +34,11 @@ HBM_PEAK_GBS = 8000.0
 def gpu_leg(args):
     import ibtfo_amd as ib
     from ibtfo_amd import workloads as W
-    cfg = W.c4_config(ib, N_x=args.nx, N_y=args.ny, Riemann=args.riemann, Limiter=args.limiter,
+    cfg = W.mach3_bodies_config(ib, N_x=args.nx, N_y=args.ny, Riemann=args.riemann, Limiter=args.limiter,
                       Boundary=args.boundary)
     s = ib.Solver(cfg)
-    s.geometry("shapes", W.c4_shapes())
-    s.initialize("uniform", W.C4_INFLOW)
+    s.geometry("shapes", W.bodies_shapes())
+    s.initialize("uniform", W.MACH3_INFLOW)
     s.prepare(True)
     nghost = s.num_ghosts()
     if args.warmup:
@@ -51,7 +51,7 @@ def gpu_leg(args):
     value = cells * args.steps / (total_ms / 1e3)
     achieved = BYTES_PER_CELL * cells / (kern_ms / 1e3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_c4.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_step_mach3_bodies.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -80,7 +80,7 @@ def cpu_leg(args):
     Li = -dx * 2 + 0.0 + dx / 2
     x = (Li + np.arange(M) * dx)[None, :].repeat(N, 0)
     y = (Li + np.arange(N) * dx)[:, None].repeat(M, 1)
-    case.phi = W.shapes_phi(W.c4_shapes(), x, y).reshape(-1)
+    case.phi = W.shapes_phi(W.bodies_shapes(), x, y).reshape(-1)
     st = O.setup(case)
     t0 = time.perf_counter()
     r = O.run_oracle(case, steps, state=st)
@@ -88,6 +88,88 @@ def cpu_leg(args):
     return dict(value=nx * ny * r["steps"] / el, cores=threads, steps=r["steps"], seconds=el,
                 sample="%dx%d cells, %d steps of the same workload (oracle, OpenMP %d threads)" % (
                     nx, ny, r["steps"], threads))
+
+
+COPY_BYTES = 1 << 28  # calibration copy: 256 MiB read + 256 MiB written
+
+
+def pmc_probe(args):
+    """Workload for the two hardware-counter passes (FETCH_SIZE, then WRITE_SIZE):
+    a few steps of the timed workload, then a plain device copy of known size
+    (torch elementwise multiply by 1.0, COPY_BYTES read and written) whose
+    counter readings calibrate the counters' units."""
+    import torch
+    import ibtfo_amd as ib
+    from ibtfo_amd import workloads as W
+    cfg = W.mach3_bodies_config(ib, N_x=args.nx, N_y=args.ny, Riemann=args.riemann, Limiter=args.limiter,
+                                Boundary=args.boundary)
+    s = ib.Solver(cfg)
+    s.geometry("shapes", W.bodies_shapes())
+    s.initialize("uniform", W.MACH3_INFLOW)
+    s.prepare(True)
+    s.step(args.steps)
+    s.status()
+    s.close()
+    src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print("pmc probe: %d steps, 3 copies of %d bytes" % (args.steps, COPY_BYTES), flush=True)
+    return 0
+
+
+def _pmc_means(root, counter):
+    """Mean counter value per dispatch, per kernel name, from rocprofv3 CSV output."""
+    import csv
+    import glob
+    acc = {}
+    files = glob.glob(os.path.join(root, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no counter_collection.csv under %s" % root)
+    for fn in files:
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                k = row["Kernel_Name"]
+                v = acc.setdefault(k, {})
+                d = row.get("Dispatch_Id", len(v))
+                v[d] = v.get(d, 0.0) + float(row["Counter_Value"])
+    return {k: sum(v.values()) / len(v) for k, v in acc.items()}
+
+
+def pmc_summary(args):
+    """Turns the FETCH_SIZE and WRITE_SIZE passes into HBM bytes per step-kernel
+    launch: each counter is scaled by (known copy bytes) / (counter read for the
+    calibration copy)."""
+    fetch = _pmc_means(args.pmc_fetch, "FETCH_SIZE")
+    write = _pmc_means(args.pmc_write, "WRITE_SIZE")
+
+    def pick(d, key):
+        ks = [k for k in d if key in k]
+        if not ks:
+            raise SystemExit("no kernel matching %r in %s" % (key, sorted(d)))
+        return ks[0], d[ks[0]]
+
+    kname, f_step = pick(fetch, "k_step")
+    _, w_step = pick(write, "k_step")
+    cname, f_copy = pick(fetch, "MulFunctor")
+    _, w_copy = pick(write, "MulFunctor")
+    sf = COPY_BYTES / (f_copy * 1024.0)
+    sw = COPY_BYTES / (w_copy * 1024.0)
+    cells = args.nx * args.ny
+    out = dict(kernel=kname, calibration_kernel=cname, copy_bytes=COPY_BYTES,
+               fetch_kib_per_launch=f_step, write_kib_per_launch=w_step,
+               copy_fetch_kib=f_copy, copy_write_kib=w_copy, fetch_scale=sf, write_scale=sw,
+               bytes_per_launch=f_step * 1024.0 * sf + w_step * 1024.0 * sw,
+               algorithmic_bytes_per_launch=BYTES_PER_CELL * cells, cells=cells)
+    out["bytes_per_cell"] = out["bytes_per_launch"] / cells
+    print(json.dumps(out, indent=1))
+    if args.pmc_out:
+        with open(args.pmc_out, "w") as f:
+            json.dump(out, f, indent=1)
+    return 0
 
 
 def main():
@@ -103,11 +185,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true", help="run the hardware-counter workload and exit")
+    ap.add_argument("--pmc-fetch", help="rocprofv3 output folder of the FETCH_SIZE pass (summary mode)")
+    ap.add_argument("--pmc-write", help="rocprofv3 output folder of the WRITE_SIZE pass (summary mode)")
+    ap.add_argument("--pmc-out", help="where the summary mode writes its JSON")
     args = ap.parse_args()
+    if args.pmc_fetch and args.pmc_write:
+        return pmc_summary(args)
     if args.gpus != 1:
         print("bench.py: this build runs the whole domain on one GPU; --gpus %d is not supported "
               "(multi-GPU slab decomposition is a design note only)" % args.gpus, file=sys.stderr)
         return 2
+    if args.pmc_probe:
+        return pmc_probe(args)
     g = gpu_leg(args)
     cpu = None
     if not args.no_cpu:

@@ -116,12 +116,12 @@ extern "C" long long ibtfo_footprint(const ibtfo_config* c) {
 
 static void free_ctx(ibtfo_ctx* x) {
     if (!x) return;
-    if (x->graph) hipGraphExecDestroy(x->graph);
-    hipFree(x->state[0]); hipFree(x->state[1]); hipFree(x->type);
-    hipFree(x->phi); hipFree(x->gnx); hipFree(x->gny);
-    hipFree(x->ghosts); hipFree(x->gvals); hipFree(x->dcount);
-    hipFree(x->ctl); hipFree(x->mass_part); hipFree(x->corner);
-    if (x->own_stream) hipStreamDestroy(x->own_stream);
+    if (x->graph) (void)hipGraphExecDestroy(x->graph);
+    (void)hipFree(x->state[0]); (void)hipFree(x->state[1]); (void)hipFree(x->type);
+    (void)hipFree(x->phi); (void)hipFree(x->gnx); (void)hipFree(x->gny);
+    (void)hipFree(x->ghosts); (void)hipFree(x->gvals); (void)hipFree(x->dcount);
+    (void)hipFree(x->ctl); (void)hipFree(x->mass_part); (void)hipFree(x->corner);
+    if (x->own_stream) (void)hipStreamDestroy(x->own_stream);
     delete x;
 }
 
@@ -175,7 +175,7 @@ extern "C" int ibtfo_create(const ibtfo_config* cfg, ibtfo_ctx** out) {
 
 extern "C" int ibtfo_destroy(ibtfo_ctx* x) {
     if (!x) return IBTFO_E_ARG;
-    hipStreamSynchronize(x->stream);
+    (void)hipStreamSynchronize(x->stream);
     free_ctx(x);
     return IBTFO_OK;
 }
@@ -183,7 +183,7 @@ extern "C" int ibtfo_destroy(ibtfo_ctx* x) {
 extern "C" int ibtfo_set_stream(ibtfo_ctx* x, void* s) {
     if (!x) return IBTFO_E_ARG;
     x->stream = s ? (hipStream_t)s : x->own_stream;
-    if (x->graph) { hipGraphExecDestroy(x->graph); x->graph = nullptr; x->graph_chunk = 0; }
+    if (x->graph) { (void)hipGraphExecDestroy(x->graph); x->graph = nullptr; x->graph_chunk = 0; }
     return IBTFO_OK;
 }
 
@@ -269,8 +269,8 @@ extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int npr
     }
     HIPCHK(geom_run(g, x->stream));
     // ghost list (upper bound: all interior cells of type 2; count first)
-    hipFree(x->ghosts); x->ghosts = nullptr;
-    hipFree(x->gvals); x->gvals = nullptr;
+    (void)hipFree(x->ghosts); x->ghosts = nullptr;
+    (void)hipFree(x->gvals); x->gvals = nullptr;
     x->nghost = 0;
     if (x->cfg.boundary != IBTFO_REGULAR) {
         unsigned int zero = 0, cnt = 0;
@@ -289,7 +289,7 @@ extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int npr
         x->nghost = (int)cnt;
     }
     HIPCHK(hipStreamSynchronize(x->stream));
-    if (dphi) hipFree(dphi);
+    if (dphi) (void)hipFree(dphi);
     x->have_geometry = 1;
     x->prepared = 0;
     return IBTFO_OK;
@@ -552,16 +552,16 @@ extern "C" int ibtfo_step_graph(ibtfo_ctx* x, int nsteps, int chunk) {
     if (!x || nsteps < 0 || chunk <= 0) return IBTFO_E_ARG;
     if (!x->prepared) { set_error("ibtfo_step_graph before ibtfo_prepare"); return IBTFO_E_STATE; }
     if (!x->graph || x->graph_chunk != chunk) {
-        if (x->graph) { hipGraphExecDestroy(x->graph); x->graph = nullptr; }
-        hipGraph_t g;
+        if (x->graph) { (void)hipGraphExecDestroy(x->graph); x->graph = nullptr; }
+        hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
         int rc = IBTFO_OK;
         for (int s = 0; s < chunk && rc == IBTFO_OK; s++) rc = enqueue_step(x);
         hipError_t e = hipStreamEndCapture(x->stream, &g);
+        if (e == hipSuccess && rc == IBTFO_OK && g) e = hipGraphInstantiate(&x->graph, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
         if (rc) return rc;
         HIPCHK(e);
-        HIPCHK(hipGraphInstantiate(&x->graph, g, nullptr, nullptr, 0));
-        hipGraphDestroy(g);
         x->graph_chunk = chunk;
     }
     int full = nsteps / chunk, rest = nsteps % chunk;
@@ -659,10 +659,10 @@ extern "C" int ibtfo_time_steps(ibtfo_ctx* x, int nsteps, double* total_ms, doub
             HIPCHK(hipEventElapsedTime(&m, ev[2 * s], ev[2 * s + 1]));
             sum += m;
         }
-        for (auto& e : ev) hipEventDestroy(e);
+        for (auto& e : ev) (void)hipEventDestroy(e);
         *step_kernel_ms = sum / nsteps;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return IBTFO_OK;
 }

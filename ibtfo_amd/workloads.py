@@ -7,7 +7,7 @@ import numpy as np
 SHAPE_STRIDE = 21
 
 
-def c4_shapes(seed=20261202, H=1.0):
+def bodies_shapes(seed=20261202, H=1.0):
     """8-20 bodies (circles or convex polygons with 3-8 vertices), radius
     U(0.02, 0.1)*H, centres in [0.2, 1.9] x [0.1, 0.9]; returns the flat
     IBTFO_LS_SHAPES parameter list."""
@@ -58,11 +58,11 @@ def shapes_phi(prm, x, y):
 
 
 # Mach 3 inflow (rho, u, v, p) of the 4096x2048 immersed-body workload
-C4_INFLOW = (1.4, 3.0, 0.0, 1.0)
+MACH3_INFLOW = (1.4, 3.0, 0.0, 1.0)
 
 
-def c4_config(ib, N_x=4096, N_y=2048, Riemann="HLL", Limiter="VanLeer", Boundary="IDWBaseGFM", quirks=0,
+def mach3_bodies_config(ib, N_x=4096, N_y=2048, Riemann="HLL", Limiter="VanLeer", Boundary="IDWBaseGFM", quirks=0,
               max_iter=10 ** 9, CFL_number=0.5):
     return ib.config(N_x=N_x, N_y=N_y, lo_x=0.0, lo_y=0.0, hi_x=2.0, hi_y=1.0, Psy_time=1e30, max_iter=max_iter,
                      CFL_number=CFL_number, gamma=1.4, Scheme="MUSCL", Riemann=Riemann, Limiter=Limiter,
-                     Boundary=Boundary, Edge="inflow", quirks=quirks, edge_state=C4_INFLOW)
+                     Boundary=Boundary, Edge="inflow", quirks=quirks, edge_state=MACH3_INFLOW)

@@ -123,7 +123,7 @@ def test_graph_replay_matches_eager(oracle_mod):
     assert a["steps"] == b["steps"] == 40
 
 
-def _c4_case(O, nx=256, ny=128, **kw):
+def _bodies_case(O, nx=256, ny=128, **kw):
     from ibtfo_amd import workloads as W
     c = O.Case(N_x=nx, N_y=ny, lo_x=0.0, lo_y=0.0, hi_x=2.0, hi_y=1.0, Psy_time=1e30, max_iter=10 ** 9,
                CFL_number=0.5, Scheme=kw.get("Scheme", "MUSCL"), Riemann=kw.get("Riemann", "HLL"),
@@ -133,16 +133,16 @@ def _c4_case(O, nx=256, ny=128, **kw):
     Li = -dx * 2 + dx / 2
     x = (Li + np.arange(c.M) * dx)[None, :].repeat(c.N, 0)
     y = (Li + np.arange(c.N) * dx)[:, None].repeat(c.M, 1)
-    c.phi = W.shapes_phi(W.c4_shapes(), x, y).reshape(-1)
+    c.phi = W.shapes_phi(W.bodies_shapes(), x, y).reshape(-1)
     return c
 
 
 @pytest.mark.parametrize("riemann", ["HLL", "StegerWarming"])
-def test_c4_corrected_mode_bitexact(oracle_mod, riemann):
+def test_mach3_bodies_corrected_mode_bitexact(oracle_mod, riemann):
     """Mach 3 inflow past the seeded bodies, corrected arithmetic (HLL wave-speed
     fix, positivity fallback, floors, parenthesised van Leer): GPU == oracle."""
     O = oracle_mod
-    c = _c4_case(O, Riemann=riemann)
+    c = _bodies_case(O, Riemann=riemann)
     g = gpu_run(c, 60)
     o = O.run_oracle(c, 60, root_mode=1)
     assert g["steps"] == o["steps"] == 60
@@ -157,9 +157,9 @@ def test_shapes_levelset_matches_numpy(oracle_mod):
     import ibtfo_amd as ib
     from ibtfo_amd import workloads as W
     O = oracle_mod
-    c = _c4_case(O, nx=512, ny=256)
-    s = ib.Solver(W.c4_config(ib, N_x=512, N_y=256))
-    s.geometry("shapes", W.c4_shapes())
+    c = _bodies_case(O, nx=512, ny=256)
+    s = ib.Solver(W.mach3_bodies_config(ib, N_x=512, N_y=256))
+    s.geometry("shapes", W.bodies_shapes())
     XY = s.xycoord()
     phi = XY[2::6]
     assert np.max(np.abs(phi - c.phi)) <= 1e-15
