@@ -29,7 +29,28 @@ template <> struct FpConst<float> {
     static constexpr float huge = 0x1.fffffep+127f;
 };
 
-IB_HD double ib_sqrt(double x) { return __builtin_sqrt(x); }
+// Square root. On the device, for x in [2^-767, 2^1023] (every sound speed and
+// shock factor of a physical state), the rsq seed + Goldschmidt refinement +
+// two residual corrections that the compiler's IEEE sqrt runs after its range
+// scaling, without the scaling and special-value fixups: the same arithmetic on
+// the same operands, so the same correctly rounded result. Other x (zero,
+// tiny, infinite, negative, NaN) take the full IEEE sequence.
+IB_HD double ib_sqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (!(x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023)) return __builtin_sqrt(x);
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+#else
+    return __builtin_sqrt(x);
+#endif
+}
 IB_HD float ib_sqrt(float x) { return __builtin_sqrtf(x); }
 IB_HD double ib_fabs(double x) { return __builtin_fabs(x); }
 IB_HD float ib_fabs(float x) { return __builtin_fabsf(x); }
@@ -95,10 +116,23 @@ template <typename T> IB_HD T lim_vanleer_ref(T a, T P, T Q) {
     T r = ddiv(a + T(1e-7), (P - Q) + T(1e-7));
     return (r > T(0)) ? ((r * r + r) * P - ddiv(Q, r * r + T(1))) : T(0);
 }
-// the same limiter with its argument parenthesised (corrected mode)
+// The same limiter with its argument parenthesised (corrected mode), written
+// with one division: with A = a + 1e-7, B = b + 1e-7 and r = A/B,
+// (r*r + r) b / (r*r + 1) = b A (A + B) / (A*A + B*B), and r > 0 iff A*B > 0.
+// lim(a, b) and lim(b, a) share the denominator, so a cell's two slopes in one
+// direction cost one reciprocal (vanleer_pair).
 template <typename T> IB_HD T lim_vanleer_fixed(T a, T b) {
-    T r = ddiv(a + T(1e-7), b + T(1e-7));
-    return (r > T(0)) ? ddiv((r * r + r) * b, r * r + T(1)) : T(0);
+    T A = a + T(1e-7), B = b + T(1e-7);
+    return (A * B > T(0)) ? ddiv(b * (A * (A + B)), A * A + B * B) : T(0);
+}
+// {lim(a, b), lim(b, a)} of lim_vanleer_fixed
+template <typename T> IB_HD void vanleer_pair(T a, T b, T& sab, T& sba) {
+    T A = a + T(1e-7), B = b + T(1e-7);
+    T n = A + B;
+    Rcp<T> rd = rcp_of(A * A + B * B);
+    bool pos = A * B > T(0);
+    sab = pos ? dq(b * (A * n), rd) : T(0);
+    sba = pos ? dq(a * (B * n), rd) : T(0);
 }
 // VanAlbada/Slope_limiter.H:4 does not compile (undeclared r); its evident
 // intent, with r defined as in the sibling limiter.
@@ -110,6 +144,26 @@ template <int LIM, typename T> IB_HD T slope(T a, T P, T Q, bool quirks) {
     if (LIM == LIM_MINMOD) return lim_minmod(a, P - Q);
     if (LIM == LIM_VANLEER) return quirks ? lim_vanleer_ref(a, P, Q) : lim_vanleer_fixed(a, P - Q);
     return lim_vanalbada(a, P - Q);
+}
+
+// Both slopes of one cell from its neighbours um, u0, up (per component):
+// lo = u0 + 0.5 lim(u0 - um, up - u0) is the cell's state on its upper face
+// (U_L of face +1/2), hi = u0 - 0.5 lim(up - u0, u0 - um) its state on its lower
+// face (U_R of face -1/2); recon() below evaluates the same expressions face by face.
+template <int LIM, typename T> IB_HD void slope_pair(const T* um, const T* u0, const T* up, T* lo, T* hi,
+                                                     bool quirks) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        T sl, sh;
+        if (LIM == LIM_VANLEER && !quirks) {
+            vanleer_pair(u0[k] - um[k], up[k] - u0[k], sl, sh);
+        } else {
+            sl = slope<LIM>(u0[k] - um[k], up[k], u0[k], quirks);
+            sh = slope<LIM>(up[k] - u0[k], u0[k], um[k], quirks);
+        }
+        lo[k] = u0[k] + T(0.5) * sl;
+        hi[k] = u0[k] - T(0.5) * sh;
+    }
 }
 
 // Face states from the 4 cells around face i+1/2 (MUSCL_Scheme.cpp:83-85, :99-101):

@@ -6,14 +6,15 @@
 // (now_t += dt, iter++, stop test), for MUSCL / MUSCL-Hancock / Godunov with
 // HLL(=HLLC) / PVRS / Steger-Warming and minmod / VanLeer / VanAlbada.
 //
-// Work decomposition: a 64-lane block owns a strip of 63 interior columns and
+// Work decomposition: a 64-lane block owns a strip of 62 interior columns and
 // `rows` interior rows and marches upward through it. Lane t holds column
-// c = cb - 1 + t: it keeps a 4-row window of its column in registers (for the
-// y reconstruction), stages its row value in LDS for its x neighbours, computes
-// the x-face flux at c+1/2 and the y-face flux at j+1/2 once each, takes the
-// x-face at c-1/2 from lane t-1 through LDS and the y-face at j-1/2 from its own
-// previous iteration. Each state value is read from HBM once per step (halo
-// re-reads hit L2) and written once.
+// c = cb - 1 + t (lanes 0 and 63 are the strip's halo columns). Per row, each
+// lane reconstructs both face states of its own cell in x (neighbours through
+// LDS) and in y (a 3-row register window), so each limiter pair is evaluated
+// once; it takes the right state of face c+1/2 from lane t+1 and the x flux of
+// face c-1/2 from lane t-1 through LDS, and carries the y flux of face j-1/2
+// and its cell's upper y state from the previous row. Each state value is read
+// from HBM once per step (halo re-reads hit L2) and written once.
 #include "ibtfo_internal.h"
 
 namespace ibtfo {
@@ -40,13 +41,15 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // ComputeDt.cpp:55-63 contribution of one fluid cell (std::max keeps the old
-// maximum when the candidate is NaN).
+// maximum when the candidate is NaN); amax accumulates the squared sound speed.
 __device__ __forceinline__ void dt_terms(double gam, double gm1, const double* U, double& amax, double& umax,
                                          double& vmax, int& bad) {
     PState<double> s = pstate(gm1, U);
     double rho = s.rho, u = s.u, v = s.v, p = s.p;
-    double a = ib_sqrt(dq(gam * p, s.ir));
-    amax = (amax < a) ? a : amax;
+    // the maximum sound speed is taken as sqrt of the maximum a^2: RN(sqrt) is
+    // monotone, so this equals the maximum of the rounded sound speeds
+    double a2 = dq(gam * p, s.ir);
+    amax = (amax < a2) ? a2 : amax;
     double au = ib_fabs(u), av = ib_fabs(v);
     umax = (umax < au) ? au : umax;
     vmax = (vmax < av) ? av : vmax;
@@ -83,7 +86,7 @@ __device__ void finish_block(Ctl* ctl, double* mass_part, int nblocks, int bid, 
     for (int q = lane; q < nblocks; q += 64) m += ld_agent(mass_part + q);
     m = wave_sum(m);
     if (lane == 0) {
-        double A = __longlong_as_double((long long)ld_agent(&ctl->amax));
+        double A = ib_sqrt(__longlong_as_double((long long)ld_agent(&ctl->amax)));
         double Um = __longlong_as_double((long long)ld_agent(&ctl->umax));
         double Vm = __longlong_as_double((long long)ld_agent(&ctl->vmax));
         double dt = cfl * dx / (A + ((Um < Vm) ? Vm : Um));
@@ -118,18 +121,13 @@ __device__ void finish_block(Ctl* ctl, double* mass_part, int nblocks, int bid, 
     }
 }
 
-// Reconstructed (and, for MUSCL-Hancock, predicted) face states of one face from
-// the 4 cells around it, with the corrected mode's positivity fallback.
-template <int SCH, int LIM, bool Y>
-__device__ __forceinline__ void face_states(const GasT<double>& g, double dt, int quirks, const double* u0,
-                                            const double* u1, const double* u2, const double* u3, double* Ulo,
-                                            double* Uhi, PState<double>& L, PState<double>& R) {
-    if (SCH == SCH_GODUNOV) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) { Ulo[k] = u1[k]; Uhi[k] = u2[k]; }
-    } else {
-        recon<LIM>(u0, u1, u2, u3, Ulo, Uhi, quirks != 0);
-    }
+// Face states of one face from the reconstructed states Ulo/Uhi (in/out; the
+// cell averages for Godunov) and the two cell averages beside it: the
+// MUSCL-Hancock predictor and the corrected mode's positivity fallback.
+template <int SCH, bool Y>
+__device__ __forceinline__ void face_states(const GasT<double>& g, double dt, int quirks, const double* u1,
+                                            const double* u2, double* Ulo, double* Uhi, PState<double>& L,
+                                            PState<double>& R) {
     L = pstate(g.gm1, Ulo);
     R = pstate(g.gm1, Uhi);
     if (SCH == SCH_MHM) {  // MHM_Scheme.cpp:61-96 / :148-183 (same increment on both sides of the face)
@@ -154,27 +152,26 @@ __device__ __forceinline__ void face_states(const GasT<double>& g, double dt, in
     }
 }
 
-template <int SCH, int RS, int LIM>
-__device__ __forceinline__ void xface(const GasT<double>& g, double dt, int quirks, const double* u0,
-                                      const double* u1, const double* u2, const double* u3, double* F) {
-    double UL[4], UR[4];
+// x face between cells u1 (left) and u2 (right), reconstructed states UL, UR
+template <int SCH, int RS>
+__device__ __forceinline__ void xface(const GasT<double>& g, double dt, int quirks, const double* u1,
+                                      const double* u2, double* UL, double* UR, double* F) {
     PState<double> L, R;
-    face_states<SCH, LIM, false>(g, dt, quirks, u0, u1, u2, u3, UL, UR, L, R);
+    face_states<SCH, false>(g, dt, quirks, u1, u2, UL, UR, L, R);
     if (RS == RS_SW) sw_face<false>(g, L, R, F);
     else if (RS == RS_PVRS) { bool s; pvrs_face<false>(g, UL, UR, F, s); }
     else hll_face<false>(g, L, R, UL, UR, F, quirks != 0);
 }
 
-// y face at j+1/2 of column c; Fx is this cell's x-face flux in the same row,
-// which is what the reference leaves in the aliased G_OLD slot when its y solver
-// stores nothing (HLL/Riemann_solver.cpp:224-226, PVRS/Riemann_solver.cpp:550).
-template <int SCH, int RS, int LIM>
-__device__ __forceinline__ void yface(const GasT<double>& g, double dt, int quirks, const double* u0,
-                                      const double* u1, const double* u2, const double* u3, const double* Fx,
-                                      double* G) {
-    double UD[4], UU[4];
+// y face between cells u1 (below) and u2 (above); Fx is the lower cell's x-face
+// flux in the same row, which is what the reference leaves in the aliased G_OLD
+// slot when its y solver stores nothing (HLL/Riemann_solver.cpp:224-226,
+// PVRS/Riemann_solver.cpp:550).
+template <int SCH, int RS>
+__device__ __forceinline__ void yface(const GasT<double>& g, double dt, int quirks, const double* u1,
+                                      const double* u2, double* UD, double* UU, const double* Fx, double* G) {
     PState<double> D, Uu;
-    face_states<SCH, LIM, true>(g, dt, quirks, u0, u1, u2, u3, UD, UU, D, Uu);
+    face_states<SCH, true>(g, dt, quirks, u1, u2, UD, UU, D, Uu);
     if (RS == RS_SW) {
         sw_face<true>(g, D, Uu, G);
     } else if (RS == RS_PVRS) {
@@ -187,6 +184,18 @@ __device__ __forceinline__ void yface(const GasT<double>& g, double dt, int quir
 #pragma unroll
             for (int k = 0; k < 4; k++) G[k] = Fx[k];
         }
+    }
+}
+
+// Both reconstructed states of a cell in one direction (cell averages for Godunov)
+template <int SCH, int LIM>
+__device__ __forceinline__ void cell_states(const double* um, const double* u0, const double* up, double* lo,
+                                            double* hi, int quirks) {
+    if (SCH == SCH_GODUNOV) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) { lo[k] = u0[k]; hi[k] = u0[k]; }
+    } else {
+        slope_pair<LIM>(um, u0, up, lo, hi, quirks != 0);
     }
 }
 
@@ -218,12 +227,13 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
     const int cl = min(max(c, 0), L.M - 1);
     const int j0 = ng + a.rows * blockIdx.y;
     const int j1 = min(j0 + a.rows, ny + ng);
-    // lanes 0..2 also stage the strip's outer halo columns cb-2, cb+63, cb+64
-    const int ecol = min(max(lane == 0 ? cb - 2 : cb + 62 + lane, 0), L.M - 1);
-    const int eidx = lane == 0 ? 0 : 64 + lane;
+    // lanes 0 and 1 also stage the outer halo columns cb-2 and cb+63
+    const int ecol = min(max(lane == 0 ? cb - 2 : cb + 63, 0), L.M - 1);
+    const int eidx = lane == 0 ? 0 : 65;
 
-    __shared__ double rb[2][4][68];
-    __shared__ double fb[4][64];
+    __shared__ double rb[2][4][66];  // row values of columns cb-2 .. cb+63
+    __shared__ double hb[4][64];     // lower-face states of the x pairs
+    __shared__ double fb[4][64];     // x-face fluxes
 
     // reference Global_Boundary corners (Test/Sod/Global_Boudary.cpp:62-75 applied to
     // ring rows): U_OLD(i<ng or i>=N_x+ng, j ring) = U_NEW(ng or N_x+ng-1, j)
@@ -241,15 +251,24 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
         }
     }
 
-    double w[4][4];
-    double ex[4] = {0, 0, 0, 0};
+    // column window: w[0..2] = rows j, j+1, j+2 of column c at iteration j;
+    // ylo = the y state of row j on its upper face (from the previous pair)
+    double w[3][4], ylo[4], ex[4] = {0, 0, 0, 0};
+    {
+        double wm[4];
+        long long o = L.at(cl, j0 - 2);
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        long long o = L.at(cl, j0 - 2 + r);
+        for (int k = 0; k < 4; k++) wm[k] = Ui[o + k * PL];
 #pragma unroll
-        for (int k = 0; k < 4; k++) w[r][k] = Ui[o + k * PL];
+        for (int r = 0; r < 3; r++) {
+            o = L.at(cl, j0 - 1 + r);
+#pragma unroll
+            for (int k = 0; k < 4; k++) w[r][k] = Ui[o + k * PL];
+        }
+        double hi_unused[4];
+        cell_states<SCH, LIM>(wm, w[0], w[1], ylo, hi_unused, a.quirks);
     }
-    if (lane < 3) {
+    if (lane < 2) {
         long long o = L.at(ecol, j0 - 1);
 #pragma unroll
         for (int k = 0; k < 4; k++) ex[k] = Ui[o + k * PL];
@@ -258,39 +277,53 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
     double Gp[4] = {0, 0, 0, 0};
     double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
     int bad = 0;
-    const bool upd_col = lane >= 1 && c < nx + ng;
+    const bool upd_col = lane >= 1 && lane <= STRIP_W && c < nx + ng;
+    const int ln = min(lane + 1, 63);
 
     for (int j = j0 - 1; j < j1; j++) {
         const int b = j & 1;
-#pragma unroll
-        for (int k = 0; k < 4; k++) rb[b][k][lane + 1] = w[1][k];
-        if (lane < 3) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) rb[b][k][eidx] = ex[k];
-        }
-        __syncthreads();
-        // prefetch the next window row and next extra column
+        // prefetch the next window row and next halo value
         double nw[4], nex[4] = {0, 0, 0, 0};
         {
             long long o = L.at(cl, min(j + 3, L.N - 1));
 #pragma unroll
             for (int k = 0; k < 4; k++) nw[k] = Ui[o + k * PL];
-            if (lane < 3) {
+            if (lane < 2) {
                 long long oe = L.at(ecol, min(j + 1, L.N - 1));
 #pragma unroll
                 for (int k = 0; k < 4; k++) nex[k] = Ui[oe + k * PL];
             }
         }
-        double xm[4], xp[4], xp2[4];
+        double F[4] = {0, 0, 0, 0};
+        // x faces of row j; the first iteration (j = j0-1) only needs the y face
+        // below row j0, unless that face may take the aliased x flux
+        if (j >= j0 || (a.quirks && RS != RS_SW)) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            xm[k] = rb[b][k][lane];
-            xp[k] = rb[b][k][lane + 2];
-            xp2[k] = rb[b][k][lane + 3];
+            for (int k = 0; k < 4; k++) rb[b][k][lane + 1] = w[0][k];
+            if (lane < 2) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) rb[b][k][eidx] = ex[k];
+            }
+            __syncthreads();
+            double xm[4], xp[4], xlo[4], xhi[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                xm[k] = rb[b][k][lane];
+                xp[k] = rb[b][k][lane + 2];
+            }
+            cell_states<SCH, LIM>(xm, w[0], xp, xlo, xhi, a.quirks);
+#pragma unroll
+            for (int k = 0; k < 4; k++) hb[k][lane] = xhi[k];
+            __syncthreads();
+            double xr[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) xr[k] = hb[k][ln];
+            xface<SCH, RS>(g, dt, a.quirks, w[0], xp, xlo, xr, F);
         }
-        double F[4], G[4];
-        xface<SCH, RS, LIM>(g, dt, a.quirks, xm, w[1], xp, xp2, F);
-        yface<SCH, RS, LIM>(g, dt, a.quirks, w[0], w[1], w[2], w[3], F, G);
+        // y face j+1/2: this row's upper state and the lower state of row j+1
+        double ynlo[4], ynhi[4], G[4];
+        cell_states<SCH, LIM>(w[0], w[1], w[2], ynlo, ynhi, a.quirks);
+        yface<SCH, RS>(g, dt, a.quirks, w[0], w[1], ylo, ynhi, F, G);
 
         if (j >= j0) {
 #pragma unroll
@@ -304,13 +337,13 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
                         double Fm = fb[k][lane - 1];
-                        double v = w[1][k] - dq(dt * (F[k] - Fm), g.r_dx);
+                        double v = w[0][k] - dq(dt * (F[k] - Fm), g.r_dx);
                         un[k] = v - dq(dt * (G[k] - Gp[k]), g.r_dy);
                     }
                     if (!a.quirks) apply_floor(a.gm1, g.r_gm1, un);
                 } else {
 #pragma unroll
-                    for (int k = 0; k < 4; k++) un[k] = w[1][k];
+                    for (int k = 0; k < 4; k++) un[k] = w[0][k];
                 }
 #pragma unroll
                 for (int k = 0; k < 4; k++) Uo[o + k * PL] = un[k];
@@ -348,10 +381,10 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             Gp[k] = G[k];
+            ylo[k] = ynlo[k];
             w[0][k] = w[1][k];
             w[1][k] = w[2][k];
-            w[2][k] = w[3][k];
-            w[3][k] = nw[k];
+            w[2][k] = nw[k];
             ex[k] = nex[k];
         }
     }
@@ -381,7 +414,7 @@ __global__ __launch_bounds__(64) void k_step_lax(StepArgs a) {
     const int j1 = min(j0 + a.rows, ny + ng);
     double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
     int bad = 0;
-    if (lane >= 1 && c < nx + ng) {
+    if (lane >= 1 && lane <= STRIP_W && c < nx + ng) {
         for (int j = j0; j < j1; j++) {
             const long long o = L.at(c, j);
             double un[4];

@@ -64,10 +64,14 @@ inline double lim_vanleer_ref(double a, double P, double Q) {
     double r = (a + 1e-7) / ((P - Q) + 1e-7);
     return (r > 0) ? ((r * r + r) * P - Q / (r * r + 1)) : 0;
 }
-// The formula VanLeer/Slope_limiter.H:4 writes, with its argument parenthesised.
+// The formula VanLeer/Slope_limiter.H:4 writes, with its argument parenthesised
+// (corrected mode), in one-division form: with A = a + 1e-7, B = b + 1e-7 and
+// r = A/B, (r*r + r) b / (r*r + 1) = b A (A + B) / (A*A + B*B) and r > 0 iff
+// A*B > 0. The two forms agree to a few ulp (tests pin the identity); this one
+// is the corrected mode's definition.
 inline double lim_vanleer_fixed(double a, double b) {
-    double r = (a + 1e-7) / (b + 1e-7);
-    return (r > 0) ? ((r * r + r) * b / (r * r + 1)) : 0;
+    double A = a + 1e-7, B = b + 1e-7;
+    return (A * B > 0) ? (b * (A * (A + B)) / (A * A + B * B)) : 0;
 }
 // VanAlbada/Slope_limiter.H:4 does not compile (undeclared r); this is its evident
 // intent with r defined as in the sibling limiter. No reference output exists.

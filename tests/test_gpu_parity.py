@@ -43,14 +43,14 @@ def test_sodx_bitexact(oracle_mod, scheme, riemann, limiter):
         assert max(rel_linf(g["U_OLD"], p["U_OLD"])) <= REL_TOL
 
 
-def test_sod_c1_full(oracle_mod):
-    """Config 1: 400x100 Sod, MHM + minmod + HLLC(=HLL), CFL 0.5, t_end 0.2."""
+def test_sod_400x100_full(oracle_mod):
+    """400x100 Sod, MHM + minmod + HLLC(=HLL), CFL 0.5, t_end 0.2."""
     O = oracle_mod
     c = _case(O, N_x=400, N_y=100, hi_y=0.25, Scheme="MHM", Riemann="HLLC", Limiter="minmod", case="sodx",
               CFL_number=0.5, Psy_time=0.2)
     g = gpu_run(c, 100000, graph_chunk=32)
     o = O.run_oracle(c, 100000, root_mode=1)
-    print("C1 steps", g["steps"], o["steps"], "t", g["t"], flush=True)
+    print("sod 400x100 steps", g["steps"], o["steps"], "t", g["t"], flush=True)
     assert g["steps"] == o["steps"] and g["steps"] > 100
     assert nan_equal_bits(g["U_OLD"], o["U_OLD"]) == 0
     p = O.run_oracle(c, 100000, root_mode=0)

@@ -31,12 +31,23 @@ def test_vanleer_macro_defect_and_fixed_form(oracle_mod):
     for u in [0.3, 1.0, 2.5]:
         assert L.or_limiter(1, 0, 0.0, u, u) == 0.0
         assert L.or_limiter(1, 1, 0.0, u, u) == 2 * u - u / 2
-    # smooth data: fixed form = (r^2 + r) b / (r^2 + 1) with r = (a+1e-7)/(b+1e-7)
+    # corrected form: (r^2 + r) b / (r^2 + 1) with r = (a+1e-7)/(b+1e-7), the
+    # formula the header writes; the oracle evaluates it with one division, which
+    # agrees with the two-division evaluation to a few ulp and is exact in rationals
+    from fractions import Fraction as Fr
     rng = np.random.default_rng(3)
-    for a, b in rng.normal(size=(200, 2)):
+    vals = list(rng.normal(size=(400, 2))) + [(1e-9, -2e-9), (0.0, 0.0), (3.0, -1e-7), (-1e-7, 2.0)]
+    for a, b in vals:
+        got = L.or_limiter(1, 0, a, b, 0.0)
+        if b + 1e-7 == 0.0:  # r = A/0: the two-division form is NaN, the corrected one 0
+            assert got == 0.0
+            continue
         r = (a + 1e-7) / (b + 1e-7)
-        want = (r * r + r) * b / (r * r + 1) if r > 0 else 0.0
-        assert L.or_limiter(1, 0, a, b, 0.0) == want
+        two_div = (r * r + r) * b / (r * r + 1) if r > 0 else 0.0
+        assert abs(got - two_div) <= 8 * np.spacing(abs(two_div)) + 1e-300, (a, b, got, two_div)
+        A, B = Fr(a) + Fr(1e-7), Fr(b) + Fr(1e-7)
+        exact = Fr(b) * A * (A + B) / (A * A + B * B) if A * B > 0 else Fr(0)
+        assert abs(Fr(got) - exact) <= Fr(4 * np.spacing(abs(float(exact)) + 1e-300)), (a, b)
 
 
 def _cons(r, u, v, p, g=1.4):
@@ -120,7 +131,7 @@ def test_compute_dt_closed_form(oracle_mod):
 
 @pytest.mark.parametrize("nx", [100, 200, 400])
 def test_sod_converges_to_exact_solution(oracle_mod, nx):
-    """C1's scheme (MHM + minmod + HLL) against the exact Riemann solution:
+    """The 400x100 Sod workload's scheme (MHM + minmod + HLL) against the exact Riemann solution:
     L1(rho) falls with resolution (first-order-or-better at the discontinuities)."""
     O = oracle_mod
     c = O.Case(N_x=nx, N_y=4, hi_y=4.0 / nx, Scheme="MHM", case="sodx", CFL_number=0.5, Psy_time=0.2)
