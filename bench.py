@@ -1,18 +1,20 @@
 #!/usr/bin/env python3
-"""Benchmark: FP64 cell-updates/s of the fused Euler step on one MI355X.
+"""Benchmark: FP64 cell-updates/s of the Euler step on one MI355X.
 
-Workload: 4096 x 2048 cells on [0,2] x [0,1], Mach 3
-inflow (rho, u, v, p) = (1.4, 3, 0, 1) past 9 seeded bodies (seed 20261202:
-circles and convex polygons, exact signed-distance level set), MUSCL + van Leer
-limiter + HLL flux, IDW ghost-fluid boundary, supersonic inflow / zero-gradient
-edges, FP64. A step = one fused step kernel (reconstruction, fluxes, update,
-edge ring, dt and mass reductions) + ghost fill + edge kernel. The state is
-initialised on the device before the timed region; dt never leaves the device.
+Workload: 4096 x 2048 cells on [0,2] x [0,1], Mach 3 inflow (rho, u, v, p) =
+(1.4, 3, 0, 1) past 9 seeded bodies (seed 20261202: circles and convex polygons,
+exact signed-distance level set), MUSCL + van Leer limiter + HLL flux, IDW
+ghost-fluid boundary, supersonic inflow / zero-gradient edges, FP64. A step =
+the step kernel (reconstruction, fluxes, update, per-block dt/mass partials),
+the one-block dt/clock kernel, the two ghost-fill kernels and the edge kernel.
+The state is initialised on the device before the timed region and dt never
+leaves the device. The step kernel's tile height is picked by the library's
+autotuner; the timed loop replays a captured HIP graph of the step sequence.
 
 Prints one JSON line. `value` = N_x*N_y*steps / t_loop (HIP events around K
-queued steps on the solve's stream). `roofline` is for the step kernel alone:
-65 algorithmic bytes per cell update (read U^n 4xFP64, write U^{n+1} 4xFP64,
-one uint8 cell flag) divided by its mean launch duration (HIP events around each
+steps on the solve's stream). `roofline` is for the step kernel alone: 65
+algorithmic bytes per cell update (read U^n 4xFP64, write U^{n+1} 4xFP64, one
+uint8 cell flag) divided by its mean launch duration (HIP events around each
 launch in a second timed pass). `cpu_baseline` times the C++ oracle (the
 reference's own loop structure, OpenMP) on a bounded sample of the same workload.
 """
@@ -41,8 +43,17 @@ def gpu_leg(args):
     s.initialize("uniform", W.MACH3_INFLOW)
     s.prepare(True)
     nghost = s.num_ghosts()
+    tuned = None
+    if args.rows > 0:
+        s.set_rows(args.rows)
+    elif not args.no_tune:
+        tuned = s.autotune(args.tune_steps)
+    rows, nbx, nby = s.tile()
     if args.warmup:
         s.step(args.warmup)
+        s.status()
+    if args.graph_chunk > 0:  # capture the step sequence once; the timed loop replays it
+        s.step(args.graph_chunk, graph_chunk=args.graph_chunk)
         s.status()
     total_ms, kern_ms = s.time_steps(args.steps, kernel=True)
     it, t, dt, rc = s.status()
@@ -56,7 +67,8 @@ def gpu_leg(args):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
     out = dict(value=value, ms_per_step=ms_step, kernel_ms=kern_ms, achieved=achieved, traffic=traffic,
-               iter=it, t=t, dt=dt, status=rc, nghost=nghost)
+               iter=it, t=t, dt=dt, status=rc, nghost=nghost, rows=rows, grid=[nbx, nby],
+               tuned=tuned is not None)
     s.close()
     return out
 
@@ -107,6 +119,8 @@ def pmc_probe(args):
     s.geometry("shapes", W.bodies_shapes())
     s.initialize("uniform", W.MACH3_INFLOW)
     s.prepare(True)
+    if args.rows > 0:
+        s.set_rows(args.rows)
     s.step(args.steps)
     s.status()
     s.close()
@@ -182,6 +196,10 @@ def main():
     ap.add_argument("--riemann", default="HLL")
     ap.add_argument("--limiter", default="VanLeer")
     ap.add_argument("--boundary", default="IDWBaseGFM")
+    ap.add_argument("--rows", type=int, default=0, help="fixed rows per strip (skips the autotuner)")
+    ap.add_argument("--no-tune", action="store_true", help="default tile shape, no autotuning")
+    ap.add_argument("--tune-steps", type=int, default=20)
+    ap.add_argument("--graph-chunk", type=int, default=100, help="steps per captured HIP graph (0: plain launches)")
     ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -220,6 +238,8 @@ def main():
                                "MUSCL + %s + %s, %s, inflow/zero-gradient edges, FP64"
                                % (args.nx, args.ny, args.limiter, args.riemann, args.boundary),
                    "N_x": args.nx, "N_y": args.ny, "ghost_cells": g["nghost"], "steps_taken": g["iter"],
+                   "tile": {"columns": 62, "rows": g["rows"], "grid": g["grid"], "autotuned": g["tuned"]},
+                   "launch": "hipGraph of %d steps" % args.graph_chunk if args.graph_chunk > 0 else "stream",
                    "parallelism": "single GPU, whole domain resident"},
         "roofline": {"bound": "hbm", "achieved": g["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": g["achieved"] / HBM_PEAK_GBS, "traffic": g["traffic"],

@@ -60,7 +60,8 @@ def lib():
                      "ibtfo_compute_dt", "ibtfo_mass", "ibtfo_boundary", "ibtfo_prepare", "ibtfo_geometry",
                      "ibtfo_initialize", "ibtfo_upload_state", "ibtfo_download_state", "ibtfo_download_geometry",
                      "ibtfo_download_scheme_index", "ibtfo_sch_width", "ibtfo_num_ghosts", "ibtfo_dims",
-                     "ibtfo_time_steps", "ibtfo_device_state", "ibtfo_set_stream"]:
+                     "ibtfo_time_steps", "ibtfo_device_state", "ibtfo_set_stream", "ibtfo_set_rows",
+                     "ibtfo_tile", "ibtfo_autotune"]:
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -195,6 +196,20 @@ class Solver:
 
     def boundary(self):
         _check(lib().ibtfo_boundary(self.h), "ibtfo_boundary")
+
+    def set_rows(self, rows):
+        _check(lib().ibtfo_set_rows(self.h, int(rows)), "ibtfo_set_rows")
+
+    def tile(self):
+        r, a, b = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().ibtfo_tile(self.h, C.byref(r), C.byref(a), C.byref(b)), "ibtfo_tile")
+        return r.value, a.value, b.value
+
+    def autotune(self, probe_steps=20):
+        """Picks the step kernel's rows per strip by timing; returns (rows, ms/step)."""
+        r, ms = C.c_int(), C.c_double()
+        _check(lib().ibtfo_autotune(self.h, int(probe_steps), C.byref(r), C.byref(ms)), "ibtfo_autotune")
+        return r.value, ms.value
 
     def time_steps(self, n, kernel=True):
         tot, ker = C.c_double(), C.c_double()

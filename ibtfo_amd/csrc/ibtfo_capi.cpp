@@ -40,8 +40,9 @@ struct ibtfo_ctx {
     double* gvals;
     unsigned int* dcount;
     Ctl* ctl;
-    double* mass_part;
-    int mass_cap;
+    double* mass_part;  // PART_STRIDE doubles per step-kernel block
+    int mass_cap;       // blocks
+    int rows_override;  // step-kernel rows per strip (0 = default shape)
     double* corner;
     double Li, Hi, dxc;  // cell-centre origin and spacing (Psy_coord)
     int have_geometry, have_state, prepared;
@@ -49,6 +50,7 @@ struct ibtfo_ctx {
     FillArgs fa;
     EdgeArgs ea;
     ReduceArgs ra;
+    FinishArgs fin;
     hipGraphExec_t graph;
     int graph_chunk;
     std::vector<double> corner_host;
@@ -75,11 +77,14 @@ static Layout make_layout(const ibtfo_config* c) {
     return L;
 }
 
-static int step_grid(const Layout& L, int* nbx, int* nby, int* rows) {
+// Step-kernel grid: strips of STRIP_W columns x `rows` rows. rows_override = 0
+// picks the default shape (about 8192 blocks, 32 waves per CU to cover the
+// latency of a 3-wave occupancy, and at least 4 rows per strip);
+// ibtfo_set_rows / ibtfo_autotune choose another.
+static int step_grid(const Layout& L, int rows_override, int* nbx, int* nby, int* rows) {
     *nbx = (L.nx + STRIP_W - 1) / STRIP_W;
-    // aim at >= 2048 blocks (8 per CU); at least 8 rows per strip
-    int want_y = std::max(1, 2048 / *nbx);
-    int r = std::max(8, (L.ny + want_y - 1) / want_y);
+    int want_y = std::max(1, 8192 / *nbx);
+    int r = rows_override > 0 ? rows_override : std::max(4, (L.ny + want_y - 1) / want_y);
     r = std::min(r, std::max(L.ny, 1));
     *rows = r;
     *nby = (L.ny + r - 1) / r;
@@ -149,9 +154,9 @@ extern "C" int ibtfo_create(const ibtfo_config* cfg, ibtfo_ctx** out) {
     if (hipMalloc(&x->ctl, sizeof(Ctl)) != hipSuccess) return fail("control");
     if (hipMalloc(&x->dcount, sizeof(unsigned int)) != hipSuccess) return fail("counter");
     int nbx, nby, rows;
-    step_grid(L, &nbx, &nby, &rows);
+    step_grid(L, 0, &nbx, &nby, &rows);
     x->mass_cap = std::max(nbx * nby, 4096);
-    if (hipMalloc(&x->mass_part, x->mass_cap * sizeof(double)) != hipSuccess) return fail("partials");
+    if (hipMalloc(&x->mass_part, PART_STRIDE * (size_t)x->mass_cap * sizeof(double)) != hipSuccess) return fail("partials");
     if (hipMalloc(&x->corner, 2 * 2 * L.ng * 4 * sizeof(double)) != hipSuccess) return fail("corner");
     if (hipStreamCreate(&x->own_stream) != hipSuccess) return fail("stream");
     x->stream = x->own_stream;
@@ -464,7 +469,7 @@ static void build_args(ibtfo_ctx* x) {
     a.quirks = c.quirks;
     a.edge_fused = (c.edge == IBTFO_EDGE_REFERENCE && c.scheme != IBTFO_LAX) ? 1 : 0;
     a.corner_from_input = c.boundary != IBTFO_REGULAR;
-    step_grid(L, &a.nbx, &a.nby, &a.rows);
+    step_grid(L, x->rows_override, &a.nbx, &a.nby, &a.rows);
 
     FillArgs& f = x->fa;
     memset(&f, 0, sizeof f);
@@ -483,6 +488,12 @@ static void build_args(ibtfo_ctx* x) {
     e.inflow[0] = r; e.inflow[1] = r * u; e.inflow[2] = r * v;
     e.inflow[3] = p / (c.gamma - 1) + 0.5 * r * (u * u + v * v);
 
+    FinishArgs& fn = x->fin;
+    memset(&fn, 0, sizeof fn);
+    fn.ctl = x->ctl; fn.part = x->mass_part; fn.nblocks = a.nbx * a.nby;
+    fn.cfl = c.CFL_number; fn.dx = a.dx; fn.t_end = c.Psy_time; fn.max_iter = c.max_iter;
+    fn.advance = 1; fn.reset = 0;
+
     ReduceArgs& q = x->ra;
     memset(&q, 0, sizeof q);
     q.L = L; q.type = x->type; q.ctl = x->ctl; q.mass_part = x->mass_part;
@@ -494,6 +505,7 @@ static void build_args(ibtfo_ctx* x) {
 static int enqueue_step(ibtfo_ctx* x) {
     const ibtfo_config& c = x->cfg;
     HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, x->sa, x->stream));
+    HIPCHK(launch_finish(x->fin, x->stream));
     if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_fill(x->fa, x->stream));
     if (c.edge == IBTFO_EDGE_REFERENCE && c.scheme == IBTFO_LAX)
         HIPCHK(launch_edge_ref(x->ea, x->corner, x->sa.corner_from_input, x->stream));
@@ -631,7 +643,7 @@ extern "C" int ibtfo_time_steps(ibtfo_ctx* x, int nsteps, double* total_ms, doub
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipStreamSynchronize(x->stream));
     HIPCHK(hipEventRecord(e0, x->stream));
-    int rc = ibtfo_step(x, nsteps);
+    int rc = x->graph ? ibtfo_step_graph(x, nsteps, x->graph_chunk) : ibtfo_step(x, nsteps);
     if (rc) return rc;
     HIPCHK(hipEventRecord(e1, x->stream));
     HIPCHK(hipEventSynchronize(e1));
@@ -646,6 +658,7 @@ extern "C" int ibtfo_time_steps(ibtfo_ctx* x, int nsteps, double* total_ms, doub
             HIPCHK(hipEventRecord(ev[2 * s], x->stream));
             HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, x->sa, x->stream));
             HIPCHK(hipEventRecord(ev[2 * s + 1], x->stream));
+            HIPCHK(launch_finish(x->fin, x->stream));
             if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_fill(x->fa, x->stream));
             if (c.edge == IBTFO_EDGE_REFERENCE && c.scheme == IBTFO_LAX)
                 HIPCHK(launch_edge_ref(x->ea, x->corner, x->sa.corner_from_input, x->stream));
@@ -664,5 +677,108 @@ extern "C" int ibtfo_time_steps(ibtfo_ctx* x, int nsteps, double* total_ms, doub
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_set_rows(ibtfo_ctx* x, int rows) {
+    if (!x || rows < 0) { set_error("ibtfo_set_rows: bad argument"); return IBTFO_E_ARG; }
+    int nbx, nby, r;
+    step_grid(x->L, rows, &nbx, &nby, &r);
+    if (nbx * nby > x->mass_cap) {
+        double* p = nullptr;
+        if (hipMalloc(&p, PART_STRIDE * (size_t)nbx * nby * sizeof(double)) != hipSuccess) {
+            set_error("ibtfo_set_rows: allocation of %d block partials failed", nbx * nby);
+            return IBTFO_E_NOMEM;
+        }
+        HIPCHK(hipStreamSynchronize(x->stream));
+        (void)hipFree(x->mass_part);
+        x->mass_part = p;
+        x->mass_cap = nbx * nby;
+    }
+    x->rows_override = rows;
+    if (x->graph) { (void)hipGraphExecDestroy(x->graph); x->graph = nullptr; x->graph_chunk = 0; }
+    build_args(x);
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_tile(const ibtfo_ctx* x, int* rows, int* nbx, int* nby) {
+    if (!x) return IBTFO_E_ARG;
+    int a, b, r;
+    step_grid(x->L, x->rows_override, &a, &b, &r);
+    if (rows) *rows = r;
+    if (nbx) *nbx = a;
+    if (nby) *nby = b;
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_autotune(ibtfo_ctx* x, int probe_steps, int* best_rows, double* best_ms) {
+    if (!x || probe_steps <= 0) { set_error("ibtfo_autotune: bad argument"); return IBTFO_E_ARG; }
+    if (!x->prepared) { set_error("ibtfo_autotune before ibtfo_prepare"); return IBTFO_E_STATE; }
+    Ctl head;
+    int rc = read_ctl_head(x, head);
+    if (rc) return rc;
+    if (head.done) { set_error("ibtfo_autotune: the run has already finished"); return IBTFO_E_STATE; }
+    const Layout& L = x->L;
+    const size_t sb = 4 * (size_t)L.plane * sizeof(double);
+    char* snap = nullptr;
+    if (hipMalloc(&snap, 2 * sb + sizeof(Ctl)) != hipSuccess) {
+        set_error("ibtfo_autotune: allocation of the state snapshot failed");
+        return IBTFO_E_NOMEM;
+    }
+    auto copy_all = [&](bool save) -> int {
+        char* st0 = snap;
+        char* st1 = snap + sb;
+        char* ct = snap + 2 * sb;
+        HIPCHK(hipMemcpyAsync(save ? st0 : (char*)x->state[0], save ? (char*)x->state[0] : st0, sb,
+                              hipMemcpyDeviceToDevice, x->stream));
+        HIPCHK(hipMemcpyAsync(save ? st1 : (char*)x->state[1], save ? (char*)x->state[1] : st1, sb,
+                              hipMemcpyDeviceToDevice, x->stream));
+        HIPCHK(hipMemcpyAsync(save ? ct : (char*)x->ctl, save ? (char*)x->ctl : ct, sizeof(Ctl),
+                              hipMemcpyDeviceToDevice, x->stream));
+        HIPCHK(hipStreamSynchronize(x->stream));
+        return IBTFO_OK;
+    };
+    const int keep = x->rows_override;
+    static const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
+    int best = 0;
+    double bms = 1e300;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rc = copy_all(true);
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
+        set_error("ibtfo_autotune: event creation failed");
+        rc = IBTFO_E_HIP;
+    }
+    for (int r : cand) {
+        if (rc) break;
+        if (r > L.ny) break;
+        if ((rc = ibtfo_set_rows(x, r))) break;
+        if ((rc = ibtfo_step(x, 2))) break;
+        if (hipEventRecord(e0, x->stream) != hipSuccess) { rc = IBTFO_E_HIP; break; }
+        if ((rc = ibtfo_step(x, probe_steps))) break;
+        float ms = 0;
+        if (hipEventRecord(e1, x->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            set_error("ibtfo_autotune: timing failed");
+            rc = IBTFO_E_HIP;
+            break;
+        }
+        Ctl h;
+        if ((rc = read_ctl_head(x, h))) break;
+        if (h.done) { set_error("ibtfo_autotune: the run ends within the probe"); rc = IBTFO_E_STATE; break; }
+        if (ms / probe_steps < bms) { bms = ms / probe_steps; best = r; }
+        if ((rc = copy_all(false))) break;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) {
+        (void)copy_all(false);
+        (void)hipFree(snap);
+        (void)ibtfo_set_rows(x, keep);
+        return rc;
+    }
+    (void)hipFree(snap);
+    if ((rc = ibtfo_set_rows(x, best))) return rc;
+    if (best_rows) *best_rows = best;
+    if (best_ms) *best_ms = bms;
     return IBTFO_OK;
 }

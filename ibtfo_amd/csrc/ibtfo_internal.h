@@ -36,9 +36,6 @@ struct Ctl {
     int ran;        // 1 if the last launched step executed
     int status;     // 0 ok; 1 non-finite value met in the dt reduction
     int pad;
-    unsigned long long amax, umax, vmax;  // running maxima (bit patterns of non-negative doubles)
-    unsigned int arrive;                  // blocks finished in the current step
-    unsigned int pad2;
     double mass_now;                      // CaculateMass of the state seen by the last reduction
     double log_dt[LOGCAP], log_t[LOGCAP], log_mass[LOGCAP];
 };
@@ -48,7 +45,7 @@ struct StepArgs {
     Layout L;
     const uint8_t* type;
     Ctl* ctl;
-    double* mass_part;  // one partial sum per block
+    double* mass_part;  // per-block partials, PART_STRIDE doubles each
     const double* corner;  // [2 sides][2ng rows][4]: U_NEW ring at columns ng / N_x+ng-1
     double gam, gm1, gp1, c1, dx, dy, cfl, t_end;
     double r_gam, r_gm1, r_dx, r_dy;  // correctly rounded reciprocals of the constant divisors
@@ -113,12 +110,26 @@ struct ReduceArgs {
     int reset;  // 1: set iter = 0, now_t = 0 (start of a run)
 };
 
+// the dt / mass reduction over the step kernel's per-block partials
+constexpr int PART_STRIDE = 8;  // doubles per block: max a^2, max |u|, max |v|, mass, flag, pad
+constexpr int FINISH_THREADS = 1024;
+struct FinishArgs {
+    Ctl* ctl;
+    const double* part;
+    int nblocks;
+    double cfl, dx, t_end;
+    int max_iter;
+    int advance;  // 1: a step was taken (advance the clock); 0: stand-alone ComputeDt
+    int reset;    // with advance = 0: start the run clock
+};
+
 // kernel launchers (ibtfo_step.hip)
 hipError_t launch_step(int scheme, int rs, int lim, int prec, const StepArgs& a, hipStream_t s);
 hipError_t launch_fill(const FillArgs& a, hipStream_t s);
 hipError_t launch_scatter(const FillArgs& a, hipStream_t s);
 hipError_t launch_edge(const EdgeArgs& a, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
+hipError_t launch_finish(const FinishArgs& f, hipStream_t s);
 hipError_t launch_step_lax(const StepArgs& a, hipStream_t s);
 hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_from_input, hipStream_t s);
 

@@ -56,68 +56,87 @@ __device__ __forceinline__ void dt_terms(double gam, double gm1, const double* U
     bad |= !(rho > 0.0 && p > 0.0);
 }
 
-// Per-block reduction and the last block's finish: dt for the next step,
-// time / iteration / stop flag, mass, log entry. `advance` = 0 for the
-// stand-alone ComputeDt (no time step taken).
-__device__ void finish_block(Ctl* ctl, double* mass_part, int nblocks, int bid, double amax, double umax,
-                             double vmax, double msum, int bad, double cfl, double dx, double t_end, int max_iter,
-                             int advance, int reset) {
-    __shared__ int s_last;
+// Per-block partial of the dt / mass reductions: {max a^2, max |u|, max |v|,
+// mass, non-physical flag} at part[PART_STRIDE*bid ..], written with plain
+// stores. The one-block k_finish launched after the step reduces them in block
+// order (deterministic), so no grid-wide fence or atomic is needed: the kernel
+// boundary publishes the partials.
+__device__ __forceinline__ void block_partials(double* part, int bid, double amax, double umax, double vmax,
+                                               double msum, int bad) {
     amax = wave_max(amax);
     umax = wave_max(umax);
     vmax = wave_max(vmax);
     msum = wave_sum(msum);
     bad = __any(bad);
-    const int lane = threadIdx.x;
-    if (lane == 0) {
-        if (amax > 0.0) atomicMax(&ctl->amax, (unsigned long long)__double_as_longlong(amax));
-        if (umax > 0.0) atomicMax(&ctl->umax, (unsigned long long)__double_as_longlong(umax));
-        if (vmax > 0.0) atomicMax(&ctl->vmax, (unsigned long long)__double_as_longlong(vmax));
-        if (bad) atomicOr(&ctl->status, 1);
-        mass_part[bid] = msum;
-        __threadfence();
-        unsigned int prev = atomicAdd(&ctl->arrive, 1u);
-        s_last = (prev == (unsigned int)nblocks - 1);
+    if (threadIdx.x == 0) {
+        double* q = part + PART_STRIDE * (long long)bid;
+        q[0] = amax; q[1] = umax; q[2] = vmax; q[3] = msum; q[4] = bad ? 1.0 : 0.0;
     }
+}
+
+// ComputeDt (ComputeDt.cpp:46-67) and CaculateMass (CaculateMass.cpp:27-38) from
+// the block partials, then the host loop's bookkeeping of Test/Sod/Sod.cpp:154-177:
+// dt for the next step, now_t += dt, iter++, stop test, log entry. `advance` = 0
+// for the stand-alone ComputeDt (no step taken; `reset` starts the run clock).
+__global__ __launch_bounds__(FINISH_THREADS) void k_finish(FinishArgs f) {
+    Ctl* ctl = f.ctl;
+    if (f.advance && ctl->done) return;  // the step kernel did not run
+    __shared__ double sA[FINISH_THREADS], sU[FINISH_THREADS], sV[FINISH_THREADS], sM[FINISH_THREADS];
+    __shared__ int sB[FINISH_THREADS];
+    const int t = threadIdx.x;
+    double A = 0.0, Um = 0.0, Vm = 0.0, m = 0.0;
+    int bad = 0;
+    for (int q = t; q < f.nblocks; q += FINISH_THREADS) {
+        const double* r = f.part + PART_STRIDE * (long long)q;
+        double a2 = r[0], u = r[1], v = r[2];
+        A = (A < a2) ? a2 : A;
+        Um = (Um < u) ? u : Um;
+        Vm = (Vm < v) ? v : Vm;
+        m += r[3];
+        bad |= r[4] != 0.0;
+    }
+    sA[t] = A; sU[t] = Um; sV[t] = Vm; sM[t] = m; sB[t] = bad;
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    double m = 0.0;
-    for (int q = lane; q < nblocks; q += 64) m += ld_agent(mass_part + q);
-    m = wave_sum(m);
-    if (lane == 0) {
-        double A = ib_sqrt(__longlong_as_double((long long)ld_agent(&ctl->amax)));
-        double Um = __longlong_as_double((long long)ld_agent(&ctl->umax));
-        double Vm = __longlong_as_double((long long)ld_agent(&ctl->vmax));
-        double dt = cfl * dx / (A + ((Um < Vm) ? Vm : Um));
+    for (int w = FINISH_THREADS / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            sA[t] = (sA[t] < sA[t + w]) ? sA[t + w] : sA[t];
+            sU[t] = (sU[t] < sU[t + w]) ? sU[t + w] : sU[t];
+            sV[t] = (sV[t] < sV[t + w]) ? sV[t + w] : sV[t];
+            sM[t] = sM[t] + sM[t + w];
+            sB[t] |= sB[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        A = ib_sqrt(sA[0]);
+        Um = sU[0];
+        Vm = sV[0];
+        m = sM[0];
+        if (sB[0]) ctl->status |= 1;
+        double dt = f.cfl * f.dx / (A + ((Um < Vm) ? Vm : Um));
         ctl->dt = dt;
-        if (advance) {
-            double t = ctl->now_t + dt;
+        if (f.advance) {
+            double tn = ctl->now_t + dt;
             int it = ctl->iter + 1;
             int slot = ctl->iter % LOGCAP;
             ctl->log_dt[slot] = dt;
-            ctl->log_t[slot] = t;
+            ctl->log_t[slot] = tn;
             ctl->log_mass[slot] = m;
             ctl->mass_now = m;
-            ctl->now_t = t;
+            ctl->now_t = tn;
             ctl->iter = it;
-            ctl->done = !(t < t_end && it < max_iter);
+            ctl->done = !(tn < f.t_end && it < f.max_iter);
             ctl->cur ^= 1;
             ctl->ran = 1;
         } else {
             ctl->mass_now = m;
-            if (reset) {
+            if (f.reset) {
                 ctl->now_t = 0.0;
                 ctl->iter = 0;
-                ctl->done = !(0.0 < t_end && 0 < max_iter);
+                ctl->done = !(0.0 < f.t_end && 0 < f.max_iter);
                 ctl->ran = 0;
             }
         }
-        ctl->amax = 0ull;
-        ctl->umax = 0ull;
-        ctl->vmax = 0ull;
-        __threadfence();
-        ctl->arrive = 0u;
     }
 }
 
@@ -388,8 +407,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
             ex[k] = nex[k];
         }
     }
-    finish_block(ctl, a.mass_part, a.nbx * a.nby, blockIdx.y * a.nbx + blockIdx.x, amax, umax, vmax, msum, bad,
-                 a.cfl, dx, a.t_end, a.max_iter, 1, 0);
+    block_partials(a.mass_part, blockIdx.y * a.nbx + blockIdx.x, amax, umax, vmax, msum, bad);
 }
 
 // Lax-Friedrichs, Lax_Scheme.cpp:27-62: U_new = average of the 4 neighbours minus
@@ -439,8 +457,7 @@ __global__ __launch_bounds__(64) void k_step_lax(StepArgs a) {
         }
     }
     // the reference edge ring for Lax runs as the separate edge kernel
-    finish_block(ctl, a.mass_part, a.nbx * a.nby, blockIdx.y * a.nbx + blockIdx.x, amax, umax, vmax, msum, bad,
-                 a.cfl, a.dx, a.t_end, a.max_iter, 1, 0);
+    block_partials(a.mass_part, blockIdx.y * a.nbx + blockIdx.x, amax, umax, vmax, msum, bad);
 }
 
 // ComputeDt + CaculateMass of the current state (ComputeDt.cpp:46-67,
@@ -459,8 +476,7 @@ __global__ __launch_bounds__(64) void k_reduce(ReduceArgs a) {
             msum += U[0] * a.dx * a.dy;
         }
     }
-    finish_block(a.ctl, a.mass_part, a.nblocks, blockIdx.x, amax, umax, vmax, msum, bad, a.cfl, a.dx, a.t_end,
-                 a.max_iter, 0, a.reset);
+    block_partials(a.mass_part, blockIdx.x, amax, umax, vmax, msum, bad);
 }
 
 // Ghost-cell fill, phase 1: values from the donors of U_NEW (the state right
@@ -695,6 +711,16 @@ hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_f
 }
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce, dim3(a.nblocks), dim3(64), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    FinishArgs f;
+    f.ctl = a.ctl; f.part = a.mass_part; f.nblocks = a.nblocks;
+    f.cfl = a.cfl; f.dx = a.dx; f.t_end = a.t_end; f.max_iter = a.max_iter;
+    f.advance = 0; f.reset = a.reset;
+    return launch_finish(f, s);
+}
+hipError_t launch_finish(const FinishArgs& f, hipStream_t s) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, f);
     return hipGetLastError();
 }
 

@@ -154,10 +154,22 @@ int ibtfo_mass(ibtfo_ctx* ctx, double* mass);
 int ibtfo_boundary(ibtfo_ctx* ctx);
 
 /* Timing: runs nsteps steps between two HIP events recorded on the context
- * stream and returns the elapsed milliseconds; *step_kernel_ms gets the mean
+ * stream (replaying the captured graph when ibtfo_step_graph has captured one)
+ * and returns the elapsed milliseconds; *step_kernel_ms gets the mean
  * duration of the fused step kernel alone measured with events around each
  * launch in a second pass of `nsteps` steps (may be NULL to skip). */
 int ibtfo_time_steps(ibtfo_ctx* ctx, int nsteps, double* total_ms, double* step_kernel_ms);
+
+/* Step-kernel tile shape: a block updates a strip of 62 columns x `rows` rows.
+ * rows = 0 restores the default shape. May allocate (block partials). */
+int ibtfo_set_rows(ibtfo_ctx* ctx, int rows);
+/* current rows per strip and the step grid (blocks in x and y) */
+int ibtfo_tile(const ibtfo_ctx* ctx, int* rows, int* nbx, int* nby);
+/* Tile-shape autotuner: times `probe_steps` steps of the current solve for each
+ * candidate rows value, restoring the state and clock after each probe, and
+ * keeps the fastest. Allocates a temporary snapshot of both state buffers.
+ * Fails with IBTFO_E_STATE when the run would end within a probe. */
+int ibtfo_autotune(ibtfo_ctx* ctx, int probe_steps, int* best_rows, double* best_ms_per_step);
 
 /* Device pointers of the current state planes (rho, rho u, rho v, E), their
  * row pitch in doubles and the offset of cell (0,0) in each plane. */

@@ -148,7 +148,43 @@ def test_mach3_bodies_corrected_mode_bitexact(oracle_mod, riemann):
     assert g["steps"] == o["steps"] == 60
     assert nan_equal_bits(g["XY"][5::6], o["XY"][5::6]) == 0
     assert nan_equal_bits(g["U_OLD"], o["U_OLD"]) == 0
-    assert np.isfinite(g["U_OLD"]).all() or True
+    assert np.isfinite(g["U_OLD"]).all()
+
+
+def test_tile_shapes_and_autotune_bitexact(oracle_mod):
+    """Every strip height (ragged last strips included) and the autotuner give
+    the oracle's state; the autotuner leaves the solve's state and clock as it
+    found them."""
+    import ibtfo_amd as ib
+    from ibtfo_amd import workloads as W
+    O = oracle_mod
+    c = _bodies_case(O, nx=200, ny=100)
+    o = O.run_oracle(c, 12, root_mode=1)
+
+    def solver():
+        s = ib.Solver(W.mach3_bodies_config(ib, N_x=200, N_y=100))
+        s.geometry("field", (), phi=c.phi)
+        s.initialize("uniform", W.MACH3_INFLOW)
+        s.prepare(True)
+        return s
+
+    for rows in [1, 3, 7, 33, 100, 500]:
+        s = solver()
+        s.set_rows(rows)
+        r, nbx, nby = s.tile()
+        assert r == min(rows, 100) and nby == -(-100 // r) and nbx == -(-200 // 62)
+        s.step(12)
+        assert s.status()[0] == 12
+        assert nan_equal_bits(s.state(), o["U_OLD"]) == 0, rows
+        s.close()
+    s = solver()
+    s.step(4)
+    before = s.status()
+    best, ms = s.autotune(3)
+    assert best > 0 and ms > 0 and s.tile()[0] == best
+    assert s.status()[:3] == before[:3]
+    s.step(8)
+    assert nan_equal_bits(s.state(), o["U_OLD"]) == 0
 
 
 def test_shapes_levelset_matches_numpy(oracle_mod):
