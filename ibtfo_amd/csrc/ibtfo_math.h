@@ -220,17 +220,11 @@ template <bool Y, typename T> IB_HD void flux_of(T gm1, const T* U, T* F) {
 }
 
 // Corrected mode only (no reference counterpart): positivity of a state, the
-// per-face first-order fallback and the density/pressure floors.
+// per-face first-order fallback and the density/pressure floors (applied to
+// updated fluid cells: rho <- max(rho, RHO_FLOOR); if p <= P_FLOOR,
+// E <- P_FLOOR/(gamma-1) + kinetic energy).
 template <typename T> IB_HD bool physical(const PState<T>& s) { return s.rho > T(0) && s.p > T(0); }
 constexpr double RHO_FLOOR = 1e-8, P_FLOOR = 1e-8;
-IB_HD void apply_floor(double gm1, const Rcp<double>& r_gm1, double* q) {
-    if (!(q[0] > RHO_FLOOR)) q[0] = RHO_FLOOR;
-    Rcp<double> ir = rcp_of(q[0]);
-    double u = dq(q[1], ir), v = dq(q[2], ir);
-    double ke = 0.5 * q[0] * (u * u + v * v);
-    double p = gm1 * (q[3] - ke);
-    if (!(p > P_FLOOR)) q[3] = dq(P_FLOOR, r_gm1) + ke;
-}
 
 // HLL/Riemann_solver.cpp:131-168 (x), :202-235 (y). Returns true when the
 // first branch (S_lo >= 0) is taken; F is then F(U_lo) (the y solver's caller

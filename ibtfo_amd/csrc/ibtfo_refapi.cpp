@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/ibtfo.h"
+#include "../../include/ibtfo_ref.h"
 
 namespace {
 struct Bound {

@@ -56,6 +56,29 @@ __device__ __forceinline__ void dt_terms(double gam, double gm1, const double* U
     bad |= !(rho > 0.0 && p > 0.0);
 }
 
+// The corrected mode's density / pressure floors on an updated fluid cell fused
+// with its dt_terms: both need 1/rho, u, v and p, which are evaluated once, in
+// pstate's operation order.
+__device__ __forceinline__ void floor_and_dt(double gam, double gm1, const Rcp<double>& r_gm1, double* q,
+                                             int quirks, double& amax, double& umax, double& vmax, int& bad) {
+    if (!quirks && !(q[0] > RHO_FLOOR)) q[0] = RHO_FLOOR;
+    const double rho = q[0];
+    Rcp<double> ir = rcp_of(rho);
+    double u = dq(q[1], ir), v = dq(q[2], ir);
+    double ke = 0.5 * rho * (u * u + v * v);
+    double p = gm1 * (q[3] - ke);
+    if (!quirks && !(p > P_FLOOR)) {
+        q[3] = dq(P_FLOOR, r_gm1) + ke;
+        p = gm1 * (q[3] - ke);
+    }
+    double a2 = dq(gam * p, ir);
+    amax = (amax < a2) ? a2 : amax;
+    double au = ib_fabs(u), av = ib_fabs(v);
+    umax = (umax < au) ? au : umax;
+    vmax = (vmax < av) ? av : vmax;
+    bad |= !(rho > 0.0 && p > 0.0);
+}
+
 // Per-block partial of the dt / mass reductions: {max a^2, max |u|, max |v|,
 // mass, non-physical flag} at part[PART_STRIDE*bid ..], written with plain
 // stores. The one-block k_finish launched after the step reduces them in block
@@ -359,7 +382,8 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
                         double v = w[0][k] - dq(dt * (F[k] - Fm), g.r_dx);
                         un[k] = v - dq(dt * (G[k] - Gp[k]), g.r_dy);
                     }
-                    if (!a.quirks) apply_floor(a.gm1, g.r_gm1, un);
+                    floor_and_dt(a.gam, a.gm1, g.r_gm1, un, a.quirks, amax, umax, vmax, bad);
+                    msum += un[0] * dx * dy;
                 } else {
 #pragma unroll
                     for (int k = 0; k < 4; k++) un[k] = w[0][k];
@@ -390,10 +414,6 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
                             long long q = L.at(nx + ng + r, j);
                             for (int k = 0; k < 4; k++) Uo[q + k * PL] = un[k];
                         }
-                }
-                if (fluid) {
-                    dt_terms(a.gam, a.gm1, un, amax, umax, vmax, bad);
-                    msum += un[0] * dx * dy;
                 }
             }
         }

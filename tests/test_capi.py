@@ -10,9 +10,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared():
-    text = open(os.path.join(ROOT, "include", "ibtfo.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(ibtfo_[a-z_0-9]+)\s*\(", text)))
+    names = set()
+    for h in ("ibtfo.h", "ibtfo_ref.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(ibtfo_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_every_declared_symbol_is_exported():
@@ -21,6 +24,27 @@ def test_every_declared_symbol_is_exported():
     L = ib.lib()
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
+
+
+def test_reference_signatures_are_exported():
+    """The C++ shim exports the reference's free functions with its signatures."""
+    import shutil
+    import subprocess
+    nm = shutil.which("nm")
+    if nm is None:
+        import pytest
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-DC", "--defined-only", ib.LIB_PATH], capture_output=True, text=True).stdout
+    for sig in ["Psy_coord(double, double, double, double, int, int, int, double*, int, int)",
+                "Level_Set(char*, int, int, int, double*, int, int)",
+                "Scheme_Index(int, int, int, double*, double*, int, int)",
+                "Boundary(int, int, int, double, double*, double*, double*, double*, int, int)",
+                "Global_Boundary(int, int, int, double, double*, double*, int, int)",
+                "TimeAdvance(double, double, int, int, int, double, double, double*, double*, double*, double*, "
+                "double*, int, int)",
+                "ComputeDt(double, double, int, int, int, double, double, double*, double*, double*, int, int)",
+                "CaculateMass(double, double, int, int, int, double*, double*, double*, int, int)"]:
+        assert sig in out, sig
 
 
 def test_config_struct_matches_header():

@@ -35,8 +35,8 @@ def test_incline_gfm(oracle_mod, bound):
     _cmp(O, c, 40)
 
 
-def test_c1_full_run(oracle_mod):
-    """Config 1 geometry with the reference's row-split Sod hook replaced by the
+def test_sod_400x100_full_run(oracle_mod):
+    """The 400x100 Sod tube (MHM + minmod + HLLC) with the reference's row-split Sod hook replaced by the
     x-split hook (the same hook compiled into both sides)."""
     O = oracle_mod
     c = O.Case(N_x=400, N_y=100, hi_y=0.25, Scheme="MHM", Riemann="HLLC", case="sodx", CFL_number=0.5,
