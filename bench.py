@@ -33,11 +33,12 @@ BYTES_PER_CELL = 65.0
 HBM_PEAK_GBS = 8000.0
 
 
-def gpu_leg(args):
+def gpu_leg(args, precision="FP64", keep_state=False):
     import ibtfo_amd as ib
     from ibtfo_amd import workloads as W
     cfg = W.mach3_bodies_config(ib, N_x=args.nx, N_y=args.ny, Riemann=args.riemann, Limiter=args.limiter,
-                      Boundary=args.boundary)
+                                Boundary=args.boundary)
+    cfg.precision = ib.PRECISION[precision]
     s = ib.Solver(cfg)
     s.geometry("shapes", W.bodies_shapes())
     s.initialize("uniform", W.MACH3_INFLOW)
@@ -69,6 +70,9 @@ def gpu_leg(args):
     out = dict(value=value, ms_per_step=ms_step, kernel_ms=kern_ms, achieved=achieved, traffic=traffic,
                iter=it, t=t, dt=dt, status=rc, nghost=nghost, rows=rows, grid=[nbx, nby],
                tuned=tuned is not None)
+    if keep_state:
+        out["rho"] = s.state()[0::4].copy()
+        out["types"] = s.xycoord()[5::6].copy()
     s.close()
     return out
 
@@ -203,6 +207,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--pmc-probe", action="store_true", help="run the hardware-counter workload and exit")
     ap.add_argument("--pmc-fetch", help="rocprofv3 output folder of the FETCH_SIZE pass (summary mode)")
     ap.add_argument("--pmc-write", help="rocprofv3 output folder of the WRITE_SIZE pass (summary mode)")
@@ -216,7 +221,19 @@ def main():
         return 2
     if args.pmc_probe:
         return pmc_probe(args)
-    g = gpu_leg(args)
+    g = gpu_leg(args, keep_state=not args.no_mixed)
+    mixed = None
+    if not args.no_mixed:
+        m = gpu_leg(args, precision="mixed", keep_state=True)
+        fluid = g["types"] == 0
+        l1 = float(np.mean(np.abs(m["rho"][fluid] - g["rho"][fluid])))
+        scale = float(np.mean(np.abs(g["rho"][fluid])))
+        mixed = {"form": "FP32 limiter slopes and Riemann fluxes; FP64 state, face-state primitives, update, "
+                         "ghost fill and reductions",
+                 "value": m["value"], "unit": "cell-updates/s", "ms_per_step": m["ms_per_step"],
+                 "kernel_ms": m["kernel_ms"], "speedup_vs_fp64": m["value"] / g["value"],
+                 "l1_rho_vs_fp64": l1, "l1_rho_vs_fp64_rel": l1 / scale, "steps_compared": m["iter"],
+                 "tile_rows": m["rows"]}
     cpu = None
     if not args.no_cpu:
         c = cpu_leg(args)
@@ -246,6 +263,7 @@ def main():
                      "kernel": "k_step<MUSCL,HLL,VanLeer>", "kernel_ms": g["kernel_ms"],
                      "bytes_per_cell": BYTES_PER_CELL},
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
     }
     print(json.dumps(line))
     return 0

@@ -100,8 +100,16 @@ static int validate(const ibtfo_config* c) {
         set_error("slot value out of range");
         return IBTFO_E_ARG;
     }
-    if (c->time != IBTFO_SINGLESTEP) { set_error("Time=RK2 is not available in this build"); return IBTFO_E_ARG; }
-    if (c->precision != IBTFO_FP64) { set_error("mixed precision is not available in this build"); return IBTFO_E_ARG; }
+    if (c->time != IBTFO_SINGLESTEP && c->time != IBTFO_RK2) { set_error("unknown Time slot"); return IBTFO_E_ARG; }
+    if (c->time == IBTFO_RK2 && c->scheme == IBTFO_LAX) {
+        set_error("Time=RK2 needs a reconstructing scheme (MUSCL, MHM, Godunov)");
+        return IBTFO_E_ARG;
+    }
+    if (c->precision != IBTFO_FP64 && c->precision != IBTFO_MIXED) { set_error("unknown precision"); return IBTFO_E_ARG; }
+    if (c->precision == IBTFO_MIXED && (c->quirks || c->scheme == IBTFO_LAX)) {
+        set_error("mixed precision needs quirks = 0 and a reconstructing scheme (MUSCL, MHM, Godunov)");
+        return IBTFO_E_ARG;
+    }
     if (!(c->gamma > 1.0) || !(c->hi_x > c->lo_x) || !(c->hi_y > c->lo_y)) { set_error("bad physical parameters"); return IBTFO_E_ARG; }
     return IBTFO_OK;
 }
@@ -502,16 +510,48 @@ static void build_args(ibtfo_ctx* x) {
     q.nblocks = std::min(x->mass_cap, std::max(1, (L.nx * L.ny + 255) / 256));
 }
 
-static int enqueue_step(ibtfo_ctx* x) {
+// Boundary and Global_Boundary after an update (the state just made current):
+// ghost fill phase 1, the edge ring, ghost fill phase 2.
+static int enqueue_boundaries(ibtfo_ctx* x) {
     const ibtfo_config& c = x->cfg;
-    HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, x->sa, x->stream));
-    HIPCHK(launch_finish(x->fin, x->stream));
     if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_fill(x->fa, x->stream));
     if (c.edge == IBTFO_EDGE_REFERENCE && c.scheme == IBTFO_LAX)
         HIPCHK(launch_edge_ref(x->ea, x->corner, x->sa.corner_from_input, x->stream));
     if (c.edge != IBTFO_EDGE_REFERENCE) HIPCHK(launch_edge(x->ea, x->stream));
     if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_scatter(x->fa, x->stream));
     return IBTFO_OK;
+}
+
+// One step of the reference loop. `ev` (optional) receives events around each
+// step-kernel launch: ev[0], ev[1] (single step or RK2 stage 1), ev[2], ev[3]
+// (RK2 stage 2).
+static int enqueue_step(ibtfo_ctx* x, hipEvent_t* ev = nullptr) {
+    const ibtfo_config& c = x->cfg;
+    int rc;
+    if (c.time == IBTFO_RK2) {
+        // Heun: U1 = U^n + dt L(U^n) into the other buffer, made current, with its
+        // boundaries; then (U^n + U1 + dt L(U1)) / 2 written over U^n
+        StepArgs s1 = x->sa;
+        s1.stage = 1;
+        if (ev) HIPCHK(hipEventRecord(ev[0], x->stream));
+        HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, s1, x->stream));
+        if (ev) HIPCHK(hipEventRecord(ev[1], x->stream));
+        FinishArgs flip = x->fin;
+        flip.advance = 2;
+        HIPCHK(launch_finish(flip, x->stream));
+        if ((rc = enqueue_boundaries(x))) return rc;
+        StepArgs s2 = x->sa;
+        s2.stage = 2;
+        if (ev) HIPCHK(hipEventRecord(ev[2], x->stream));
+        HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, s2, x->stream));
+        if (ev) HIPCHK(hipEventRecord(ev[3], x->stream));
+    } else {
+        if (ev) HIPCHK(hipEventRecord(ev[0], x->stream));
+        HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, x->sa, x->stream));
+        if (ev) HIPCHK(hipEventRecord(ev[1], x->stream));
+    }
+    HIPCHK(launch_finish(x->fin, x->stream));
+    return enqueue_boundaries(x);
 }
 
 static int run_reduce(ibtfo_ctx* x, int reset) {
@@ -651,28 +691,24 @@ extern "C" int ibtfo_time_steps(ibtfo_ctx* x, int nsteps, double* total_ms, doub
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     if (total_ms) *total_ms = ms;
     if (step_kernel_ms) {
-        std::vector<hipEvent_t> ev(2 * nsteps);
+        // per-launch events around the step kernel(s) of each step
+        const int per = x->cfg.time == IBTFO_RK2 ? 4 : 2;
+        std::vector<hipEvent_t> ev((size_t)per * nsteps, nullptr);
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-        const ibtfo_config& c = x->cfg;
-        for (int s = 0; s < nsteps; s++) {
-            HIPCHK(hipEventRecord(ev[2 * s], x->stream));
-            HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, x->sa, x->stream));
-            HIPCHK(hipEventRecord(ev[2 * s + 1], x->stream));
-            HIPCHK(launch_finish(x->fin, x->stream));
-            if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_fill(x->fa, x->stream));
-            if (c.edge == IBTFO_EDGE_REFERENCE && c.scheme == IBTFO_LAX)
-                HIPCHK(launch_edge_ref(x->ea, x->corner, x->sa.corner_from_input, x->stream));
-            if (c.edge != IBTFO_EDGE_REFERENCE) HIPCHK(launch_edge(x->ea, x->stream));
-            if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_scatter(x->fa, x->stream));
-        }
-        HIPCHK(hipEventSynchronize(ev[2 * nsteps - 1]));
+        for (int s = 0; s < nsteps && rc == IBTFO_OK; s++) rc = enqueue_step(x, &ev[(size_t)per * s]);
         double sum = 0;
-        for (int s = 0; s < nsteps; s++) {
-            float m = 0;
-            HIPCHK(hipEventElapsedTime(&m, ev[2 * s], ev[2 * s + 1]));
-            sum += m;
-        }
+        if (rc == IBTFO_OK && hipEventSynchronize(ev.back()) != hipSuccess) rc = IBTFO_E_HIP;
+        for (int s = 0; s < nsteps && rc == IBTFO_OK; s++)
+            for (int q = 0; q < per; q += 2) {
+                float m = 0;
+                if (hipEventElapsedTime(&m, ev[(size_t)per * s + q], ev[(size_t)per * s + q + 1]) != hipSuccess) {
+                    rc = IBTFO_E_HIP;
+                    break;
+                }
+                sum += m;
+            }
         for (auto& e : ev) (void)hipEventDestroy(e);
+        if (rc) { set_error("ibtfo_time_steps: per-launch timing failed"); return rc; }
         *step_kernel_ms = sum / nsteps;
     }
     (void)hipEventDestroy(e0);

@@ -55,6 +55,8 @@ struct StepArgs {
     int corner_from_input;  // corners from the input buffer's ring (GFM) or `corner` (Regular)
     int rows;               // rows per block strip
     int nbx, nby;           // grid
+    int stage;              // 0: single-step update; RK2 (Heun): 1 = U1 = U^n + dt L(U^n),
+                            // 2 = (U^n + U1 + dt L(U1)) / 2, written over U^n in place
 };
 
 // Ghost-cell records (compacted list of interior cells of type 2), built once.
@@ -119,7 +121,8 @@ struct FinishArgs {
     int nblocks;
     double cfl, dx, t_end;
     int max_iter;
-    int advance;  // 1: a step was taken (advance the clock); 0: stand-alone ComputeDt
+    int advance;  // 1: a step was taken (advance the clock); 0: stand-alone ComputeDt;
+                  // 2: RK2 stage boundary (make the stage-1 buffer current, no reduction)
     int reset;    // with advance = 0: start the run clock
 };
 

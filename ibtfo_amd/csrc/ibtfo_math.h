@@ -51,7 +51,14 @@ IB_HD double ib_sqrt(double x) {
     return __builtin_sqrt(x);
 #endif
 }
-IB_HD float ib_sqrt(float x) { return __builtin_sqrtf(x); }
+// FP32 (mixed-precision form): the hardware square root (1 ulp)
+IB_HD float ib_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return __builtin_sqrtf(x);
+#endif
+}
 IB_HD double ib_fabs(double x) { return __builtin_fabs(x); }
 IB_HD float ib_fabs(float x) { return __builtin_fabsf(x); }
 
@@ -84,7 +91,15 @@ IB_HD Rcp<double> rcp_of(double b) {
     return Rcp<double>{b, 1.0 / b};
 #endif
 }
-IB_HD Rcp<float> rcp_of(float b) { return Rcp<float>{b, 1.0f / b}; }
+// FP32 (mixed-precision form): the hardware reciprocal (1 ulp) and a plain
+// product, about 2 ulp per quotient; no correct rounding is attempted there.
+IB_HD Rcp<float> rcp_of(float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return Rcp<float>{b, __builtin_amdgcn_rcpf(b)};
+#else
+    return Rcp<float>{b, 1.0f / b};
+#endif
+}
 IB_HD double ib_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 IB_HD float ib_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 IB_HD double ib_copysign(double a, double b) { return __builtin_copysign(a, b); }
@@ -94,6 +109,7 @@ template <typename T> IB_HD T dq(T a, const Rcp<T>& r) {
     T rr = ib_fma(-q0, r.b, a);
     return ib_copysign(ib_fma(rr, r.y, q0), q0);
 }
+IB_HD float dq(float a, const Rcp<float>& r) { return a * r.y; }
 template <typename T> IB_HD T ddiv(T a, T b) { return dq(a, rcp_of(b)); }
 
 // minmod/Slope_limiter.H:4: (a*b > 0) ? ((a/b > 1) ? b : a) : 0.

@@ -104,6 +104,13 @@ __device__ __forceinline__ void block_partials(double* part, int bid, double ama
 __global__ __launch_bounds__(FINISH_THREADS) void k_finish(FinishArgs f) {
     Ctl* ctl = f.ctl;
     if (f.advance && ctl->done) return;  // the step kernel did not run
+    if (f.advance == 2) {
+        if (threadIdx.x == 0) {
+            ctl->cur ^= 1;
+            ctl->ran = 1;
+        }
+        return;
+    }
     __shared__ double sA[FINISH_THREADS], sU[FINISH_THREADS], sV[FINISH_THREADS], sM[FINISH_THREADS];
     __shared__ int sB[FINISH_THREADS];
     const int t = threadIdx.x;
@@ -163,66 +170,96 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_finish(FinishArgs f) {
     }
 }
 
-// Face states of one face from the reconstructed states Ulo/Uhi (in/out; the
-// cell averages for Godunov) and the two cell averages beside it: the
-// MUSCL-Hancock predictor and the corrected mode's positivity fallback.
-template <int SCH, bool Y>
-__device__ __forceinline__ void face_states(const GasT<double>& g, double dt, int quirks, const double* u1,
-                                            const double* u2, double* Ulo, double* Uhi, PState<double>& L,
-                                            PState<double>& R) {
-    L = pstate(g.gm1, Ulo);
-    R = pstate(g.gm1, Uhi);
+// Primitive view of an FP64 conserved state in the compute type T. FP64:
+// pstate itself (correctly rounded, the reference's order). Mixed precision:
+// 1/rho, u, v and p = (gamma-1)(E - rho |u|^2 / 2) are recovered in FP64 (no
+// correct rounding needed) and then rounded to FP32, because near vacuum the
+// FP32 difference E - rho |u|^2 / 2 would lose the pressure to cancellation.
+template <typename T> __device__ __forceinline__ PState<T> prim_of(double gm1, const double* U);
+template <> __device__ __forceinline__ PState<double> prim_of<double>(double gm1, const double* U) {
+    return pstate(gm1, U);
+}
+template <> __device__ __forceinline__ PState<float> prim_of<float>(double gm1, const double* U) {
+    const double rho = U[0];
+    double y = __builtin_amdgcn_rcp(rho);
+    y = __builtin_fma(y, __builtin_fma(-rho, y, 1.0), y);
+    const double u = U[1] * y, v = U[2] * y;
+    const double p = gm1 * (U[3] - 0.5 * rho * (u * u + v * v));
+    PState<float> s;
+    s.rho = (float)rho; s.u = (float)u; s.v = (float)v; s.p = (float)p; s.E = (float)U[3];
+    s.ir = Rcp<float>{s.rho, (float)y};
+    return s;
+}
+
+// Face states of one face from the reconstructed FP64 conserved states Ulo/Uhi
+// (in/out; the cell averages for Godunov) and the two cell averages u1, u2
+// beside it: the MUSCL-Hancock predictor and the corrected mode's positivity
+// fallback. T is the compute type of the primitive states and fluxes (double,
+// or float in the mixed-precision form).
+template <int SCH, bool Y, typename T>
+__device__ __forceinline__ void face_states(const GasT<T>& g, double gm1, T dt, int quirks, const double* u1,
+                                            const double* u2, double* Ulo, double* Uhi, PState<T>& L,
+                                            PState<T>& R) {
+    L = prim_of<T>(gm1, Ulo);
+    R = prim_of<T>(gm1, Uhi);
     if (SCH == SCH_MHM) {  // MHM_Scheme.cpp:61-96 / :148-183 (same increment on both sides of the face)
-        double Fl[4], Fh[4];
+        T Fl[4], Fh[4];
         phys_flux<Y>(L, Fl);
         phys_flux<Y>(R, Fh);
-        const Rcp<double>& rh = Y ? g.r_dy : g.r_dx;
+        const Rcp<T>& rh = Y ? g.r_dy : g.r_dx;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            double inc = dq(0.5 * (Fl[k] - Fh[k]) * dt, rh);
+            T inc = dq(T(0.5) * (Fl[k] - Fh[k]) * dt, rh);
             Ulo[k] = Ulo[k] + inc;
             Uhi[k] = Uhi[k] + inc;
         }
-        L = pstate(g.gm1, Ulo);
-        R = pstate(g.gm1, Uhi);
+        L = prim_of<T>(gm1, Ulo);
+        R = prim_of<T>(gm1, Uhi);
     }
     if (SCH != SCH_GODUNOV && !quirks && !(physical(L) && physical(R))) {
 #pragma unroll
         for (int k = 0; k < 4; k++) { Ulo[k] = u1[k]; Uhi[k] = u2[k]; }
-        L = pstate(g.gm1, Ulo);
-        R = pstate(g.gm1, Uhi);
+        L = prim_of<T>(gm1, Ulo);
+        R = prim_of<T>(gm1, Uhi);
     }
 }
 
+template <bool Y, int RS, typename T>
+__device__ __forceinline__ bool riemann(const GasT<T>& g, const PState<T>& L, const PState<T>& R,
+                                        const double* Ulo, const double* Uhi, T* F, bool quirks, bool& stray) {
+    T ul[4], uh[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) { ul[k] = (T)Ulo[k]; uh[k] = (T)Uhi[k]; }
+    stray = false;
+    if (RS == RS_SW) { sw_face<Y>(g, L, R, F); return false; }
+    if (RS == RS_PVRS) { pvrs_face<Y>(g, ul, uh, F, stray); return false; }
+    return hll_face<Y>(g, L, R, ul, uh, F, quirks);
+}
+
 // x face between cells u1 (left) and u2 (right), reconstructed states UL, UR
-template <int SCH, int RS>
-__device__ __forceinline__ void xface(const GasT<double>& g, double dt, int quirks, const double* u1,
-                                      const double* u2, double* UL, double* UR, double* F) {
-    PState<double> L, R;
-    face_states<SCH, false>(g, dt, quirks, u1, u2, UL, UR, L, R);
-    if (RS == RS_SW) sw_face<false>(g, L, R, F);
-    else if (RS == RS_PVRS) { bool s; pvrs_face<false>(g, UL, UR, F, s); }
-    else hll_face<false>(g, L, R, UL, UR, F, quirks != 0);
+template <int SCH, int RS, typename T>
+__device__ __forceinline__ void xface(const GasT<T>& g, double gm1, T dt, int quirks, const double* u1,
+                                      const double* u2, double* UL, double* UR, T* F) {
+    PState<T> L, R;
+    face_states<SCH, false>(g, gm1, dt, quirks, u1, u2, UL, UR, L, R);
+    bool stray;
+    riemann<false, RS>(g, L, R, UL, UR, F, quirks != 0, stray);
 }
 
 // y face between cells u1 (below) and u2 (above); Fx is the lower cell's x-face
 // flux in the same row, which is what the reference leaves in the aliased G_OLD
 // slot when its y solver stores nothing (HLL/Riemann_solver.cpp:224-226,
 // PVRS/Riemann_solver.cpp:550).
-template <int SCH, int RS>
-__device__ __forceinline__ void yface(const GasT<double>& g, double dt, int quirks, const double* u1,
-                                      const double* u2, double* UD, double* UU, const double* Fx, double* G) {
-    PState<double> D, Uu;
-    face_states<SCH, true>(g, dt, quirks, u1, u2, UD, UU, D, Uu);
-    if (RS == RS_SW) {
-        sw_face<true>(g, D, Uu, G);
-    } else if (RS == RS_PVRS) {
-        bool stray;
-        pvrs_face<true>(g, UD, UU, G, stray);
-        if (stray && quirks) G[3] = Fx[3];
-    } else {
-        bool up = hll_face<true>(g, D, Uu, UD, UU, G, quirks != 0);
-        if (up && quirks) {
+template <int SCH, int RS, typename T>
+__device__ __forceinline__ void yface(const GasT<T>& g, double gm1, T dt, int quirks, const double* u1,
+                                      const double* u2, double* UD, double* UU, const T* Fx, T* G) {
+    PState<T> D, Uu;
+    face_states<SCH, true>(g, gm1, dt, quirks, u1, u2, UD, UU, D, Uu);
+    bool stray;
+    bool up = riemann<true, RS>(g, D, Uu, UD, UU, G, quirks != 0, stray);
+    if (quirks) {
+        if (RS == RS_PVRS && stray) G[3] = Fx[3];
+        if (RS == RS_HLL && up) {
 #pragma unroll
             for (int k = 0; k < 4; k++) G[k] = Fx[k];
         }
@@ -230,19 +267,40 @@ __device__ __forceinline__ void yface(const GasT<double>& g, double dt, int quir
 }
 
 // Both reconstructed states of a cell in one direction (cell averages for Godunov)
-template <int SCH, int LIM>
+template <int SCH, int LIM, typename T>
 __device__ __forceinline__ void cell_states(const double* um, const double* u0, const double* up, double* lo,
                                             double* hi, int quirks) {
     if (SCH == SCH_GODUNOV) {
 #pragma unroll
         for (int k = 0; k < 4; k++) { lo[k] = u0[k]; hi[k] = u0[k]; }
-    } else {
+    } else if (sizeof(T) == sizeof(double)) {
         slope_pair<LIM>(um, u0, up, lo, hi, quirks != 0);
+    } else {
+        // mixed precision: neighbour differences in FP64 (exact cancellation of
+        // the common part), the limiter in FP32, the face states back in FP64;
+        // corrected-mode limiters only (the form has no reference counterpart)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float a = (float)(u0[k] - um[k]), b = (float)(up[k] - u0[k]);
+            float sl, sh;
+            if (LIM == LIM_VANLEER) {
+                vanleer_pair(a, b, sl, sh);
+            } else if (LIM == LIM_MINMOD) {
+                sl = lim_minmod(a, b);
+                sh = lim_minmod(b, a);
+            } else {
+                sl = lim_vanalbada(a, b);
+                sh = lim_vanalbada(b, a);
+            }
+            lo[k] = u0[k] + (double)(0.5f * sl);
+            hi[k] = u0[k] - (double)(0.5f * sh);
+        }
     }
 }
 
-template <int SCH, int RS, int LIM>
+template <int SCH, int RS, int LIM, typename T>
 __global__ __launch_bounds__(64) void k_step(StepArgs a) {
+    constexpr bool FP64 = sizeof(T) == sizeof(double);
     Ctl* ctl = a.ctl;
     const int lane = threadIdx.x;
     if (ctl->done) {
@@ -255,12 +313,15 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
     const double* __restrict__ Ui = a.buf[cur];
     double* __restrict__ Uo = a.buf[cur ^ 1];
     const long long PL = L.plane;
-    GasT<double> g;
-    g.gam = a.gam; g.gm1 = a.gm1; g.gp1 = a.gp1; g.c1 = a.c1;
-    g.r_gam = Rcp<double>{a.gam, a.r_gam};
-    g.r_gm1 = Rcp<double>{a.gm1, a.r_gm1};
-    g.r_dx = Rcp<double>{a.dx, a.r_dx};
-    g.r_dy = Rcp<double>{a.dy, a.r_dy};
+    GasT<T> g;
+    g.gam = (T)a.gam; g.gm1 = (T)a.gm1; g.gp1 = (T)a.gp1; g.c1 = (T)a.c1;
+    g.r_gam = Rcp<T>{(T)a.gam, (T)a.r_gam};
+    g.r_gm1 = Rcp<T>{(T)a.gm1, (T)a.r_gm1};
+    g.r_dx = Rcp<T>{(T)a.dx, (T)a.r_dx};
+    g.r_dy = Rcp<T>{(T)a.dy, (T)a.r_dy};
+    const T dtc = (T)dt;
+    const Rcp<double> r_dx{a.dx, a.r_dx}, r_dy{a.dy, a.r_dy}, r_gm1{a.gm1, a.r_gm1};
+    const double dtdx = dt * a.r_dx, dtdy = dt * a.r_dy;  // mixed form only
     const double dx = a.dx, dy = a.dy;
     const int ng = L.ng, nx = L.nx, ny = L.ny;
 
@@ -275,7 +336,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
 
     __shared__ double rb[2][4][66];  // row values of columns cb-2 .. cb+63
     __shared__ double hb[4][64];     // lower-face states of the x pairs
-    __shared__ double fb[4][64];     // x-face fluxes
+    __shared__ T fb[4][64];          // x-face fluxes
 
     // reference Global_Boundary corners (Test/Sod/Global_Boudary.cpp:62-75 applied to
     // ring rows): U_OLD(i<ng or i>=N_x+ng, j ring) = U_NEW(ng or N_x+ng-1, j)
@@ -295,7 +356,8 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
 
     // column window: w[0..2] = rows j, j+1, j+2 of column c at iteration j;
     // ylo = the y state of row j on its upper face (from the previous pair)
-    double w[3][4], ylo[4], ex[4] = {0, 0, 0, 0};
+    double w[3][4], ex[4] = {0, 0, 0, 0};
+    double ylo[4];
     {
         double wm[4];
         long long o = L.at(cl, j0 - 2);
@@ -308,7 +370,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
             for (int k = 0; k < 4; k++) w[r][k] = Ui[o + k * PL];
         }
         double hi_unused[4];
-        cell_states<SCH, LIM>(wm, w[0], w[1], ylo, hi_unused, a.quirks);
+        cell_states<SCH, LIM, T>(wm, w[0], w[1], ylo, hi_unused, a.quirks);
     }
     if (lane < 2) {
         long long o = L.at(ecol, j0 - 1);
@@ -316,7 +378,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
         for (int k = 0; k < 4; k++) ex[k] = Ui[o + k * PL];
     }
 
-    double Gp[4] = {0, 0, 0, 0};
+    T Gp[4] = {0, 0, 0, 0};
     double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
     int bad = 0;
     const bool upd_col = lane >= 1 && lane <= STRIP_W && c < nx + ng;
@@ -336,7 +398,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
                 for (int k = 0; k < 4; k++) nex[k] = Ui[oe + k * PL];
             }
         }
-        double F[4] = {0, 0, 0, 0};
+        T F[4] = {0, 0, 0, 0};
         // x faces of row j; the first iteration (j = j0-1) only needs the y face
         // below row j0, unless that face may take the aliased x flux
         if (j >= j0 || (a.quirks && RS != RS_SW)) {
@@ -353,19 +415,20 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
                 xm[k] = rb[b][k][lane];
                 xp[k] = rb[b][k][lane + 2];
             }
-            cell_states<SCH, LIM>(xm, w[0], xp, xlo, xhi, a.quirks);
+            cell_states<SCH, LIM, T>(xm, w[0], xp, xlo, xhi, a.quirks);
 #pragma unroll
             for (int k = 0; k < 4; k++) hb[k][lane] = xhi[k];
             __syncthreads();
             double xr[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) xr[k] = hb[k][ln];
-            xface<SCH, RS>(g, dt, a.quirks, w[0], xp, xlo, xr, F);
+            xface<SCH, RS>(g, a.gm1, dtc, a.quirks, w[0], xp, xlo, xr, F);
         }
         // y face j+1/2: this row's upper state and the lower state of row j+1
-        double ynlo[4], ynhi[4], G[4];
-        cell_states<SCH, LIM>(w[0], w[1], w[2], ynlo, ynhi, a.quirks);
-        yface<SCH, RS>(g, dt, a.quirks, w[0], w[1], ylo, ynhi, F, G);
+        double ynlo[4], ynhi[4];
+        T G[4];
+        cell_states<SCH, LIM, T>(w[0], w[1], w[2], ynlo, ynhi, a.quirks);
+        yface<SCH, RS>(g, a.gm1, dtc, a.quirks, w[0], w[1], ylo, ynhi, F, G);
 
         if (j >= j0) {
 #pragma unroll
@@ -378,11 +441,29 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
                 if (fluid) {
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        double Fm = fb[k][lane - 1];
-                        double v = w[0][k] - dq(dt * (F[k] - Fm), g.r_dx);
-                        un[k] = v - dq(dt * (G[k] - Gp[k]), g.r_dy);
+                        if (FP64) {
+                            double Fm = fb[k][lane - 1];
+                            double v = w[0][k] - dq(dt * ((double)F[k] - Fm), r_dx);
+                            un[k] = v - dq(dt * ((double)G[k] - (double)Gp[k]), r_dy);
+                        } else {  // FP32 fluxes, FP64 accumulation
+                            double Fm = (double)fb[k][lane - 1];
+                            un[k] = w[0][k] - dtdx * ((double)F[k] - Fm) - dtdy * ((double)G[k] - (double)Gp[k]);
+                        }
                     }
-                    floor_and_dt(a.gam, a.gm1, g.r_gm1, un, a.quirks, amax, umax, vmax, bad);
+                    if (a.stage == 0) {
+                        floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, amax, umax, vmax, bad);
+                    } else {
+                        // RK2 (Heun): each stage's update takes the floors; the second
+                        // stage then averages with U^n, which the output buffer holds
+                        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+                        int db = 0;
+                        floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, d0, d1, d2, db);
+                        if (a.stage == 2) {
+#pragma unroll
+                            for (int k = 0; k < 4; k++) un[k] = 0.5 * (Uo[o + k * PL] + un[k]);
+                            dt_terms(a.gam, a.gm1, un, amax, umax, vmax, bad);
+                        }
+                    }
                     msum += un[0] * dx * dy;
                 } else {
 #pragma unroll
@@ -682,28 +763,28 @@ __global__ void k_edge_ref(EdgeArgs a, const double* corner, int corner_from_inp
 
 // ------------------------------------------------------------------ launchers
 template <int SCH, int RS, int LIM>
-static hipError_t go(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL((k_step<SCH, RS, LIM>), dim3(a.nbx, a.nby), dim3(64), 0, s, a);
+static hipError_t go(int prec, const StepArgs& a, hipStream_t s) {
+    if (prec == 1) hipLaunchKernelGGL((k_step<SCH, RS, LIM, float>), dim3(a.nbx, a.nby), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_step<SCH, RS, LIM, double>), dim3(a.nbx, a.nby), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 template <int SCH, int RS>
-static hipError_t go_l(int lim, const StepArgs& a, hipStream_t s) {
-    if (lim == LIM_MINMOD) return go<SCH, RS, LIM_MINMOD>(a, s);
-    if (lim == LIM_VANLEER) return go<SCH, RS, LIM_VANLEER>(a, s);
-    return go<SCH, RS, LIM_VANALBADA>(a, s);
+static hipError_t go_l(int lim, int prec, const StepArgs& a, hipStream_t s) {
+    if (lim == LIM_MINMOD) return go<SCH, RS, LIM_MINMOD>(prec, a, s);
+    if (lim == LIM_VANLEER) return go<SCH, RS, LIM_VANLEER>(prec, a, s);
+    return go<SCH, RS, LIM_VANALBADA>(prec, a, s);
 }
 template <int SCH>
-static hipError_t go_r(int rs, int lim, const StepArgs& a, hipStream_t s) {
-    if (rs == RS_SW) return go_l<SCH, RS_SW>(lim, a, s);
-    if (rs == RS_PVRS) return go_l<SCH, RS_PVRS>(lim, a, s);
-    return go_l<SCH, RS_HLL>(lim, a, s);
+static hipError_t go_r(int rs, int lim, int prec, const StepArgs& a, hipStream_t s) {
+    if (rs == RS_SW) return go_l<SCH, RS_SW>(lim, prec, a, s);
+    if (rs == RS_PVRS) return go_l<SCH, RS_PVRS>(lim, prec, a, s);
+    return go_l<SCH, RS_HLL>(lim, prec, a, s);
 }
 hipError_t launch_step(int scheme, int rs, int lim, int prec, const StepArgs& a, hipStream_t s) {
-    (void)prec;
     if (scheme == SCH_LAX) return launch_step_lax(a, s);
-    if (scheme == SCH_MHM) return go_r<SCH_MHM>(rs, lim, a, s);
-    if (scheme == SCH_GODUNOV) return go_r<SCH_GODUNOV>(rs, LIM_MINMOD, a, s);
-    return go_r<SCH_MUSCL>(rs, lim, a, s);
+    if (scheme == SCH_MHM) return go_r<SCH_MHM>(rs, lim, prec, a, s);
+    if (scheme == SCH_GODUNOV) return go_r<SCH_GODUNOV>(rs, LIM_MINMOD, prec, a, s);
+    return go_r<SCH_MUSCL>(rs, lim, prec, a, s);
 }
 hipError_t launch_step_lax(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_step_lax, dim3(a.nbx, a.nby), dim3(64), 0, s, a);

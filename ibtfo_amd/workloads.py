@@ -62,7 +62,7 @@ MACH3_INFLOW = (1.4, 3.0, 0.0, 1.0)
 
 
 def mach3_bodies_config(ib, N_x=4096, N_y=2048, Riemann="HLL", Limiter="VanLeer", Boundary="IDWBaseGFM", quirks=0,
-              max_iter=10 ** 9, CFL_number=0.5):
+                        max_iter=10 ** 9, CFL_number=0.5):
     return ib.config(N_x=N_x, N_y=N_y, lo_x=0.0, lo_y=0.0, hi_x=2.0, hi_y=1.0, Psy_time=1e30, max_iter=max_iter,
                      CFL_number=CFL_number, gamma=1.4, Scheme="MUSCL", Riemann=Riemann, Limiter=Limiter,
                      Boundary=Boundary, Edge="inflow", quirks=quirks, edge_state=MACH3_INFLOW)

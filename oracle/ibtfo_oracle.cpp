@@ -1046,14 +1046,44 @@ void or_initialize(int kind, const double* prm, int nx, int ny, int ng, double g
         }
 }
 
+// TimeAdvance. SingleStep (SingleStep/SingleStep.cpp:5-21) is one Advance.
+// RK2 (RK2/RK2.cpp:12-48, which does not compile against the current headers)
+// is restated as its evident intent, Heun's method: U1 = Advance(U^n);
+// Boundary; Global_Boundary; U2 = Advance(U1); U_NEW = (U^n + U2) / 2 on fluid
+// cells, U_NEW = U_OLD (= U1 after Boundary) elsewhere. The reference's text
+// averages with U_OLD after Boundary has overwritten it with U1, which is not a
+// consistent two-stage method; U^n is kept here.
+void or_time_advance(int time, int scheme, int rs, int lim, int bc, int edge, const double* edge_prm, int quirks,
+                     double L, double H, int nx, int ny, int ng, double gam, double dt, double* U_OLD, double* T,
+                     double* U_NEW, const double* XY, const double* S) {
+    if (time == 0) {
+        or_advance(scheme, rs, lim, quirks, L, H, nx, ny, ng, gam, dt, U_OLD, T, U_NEW, XY);
+        return;
+    }
+    Grid g(nx, ny, ng);
+    const size_t n4 = (size_t)g.M * g.N * 4;
+    std::vector<double> U0(U_OLD, U_OLD + n4);
+    or_advance(scheme, rs, lim, quirks, L, H, nx, ny, ng, gam, dt, U_OLD, T, U_NEW, XY);
+    or_boundary(bc, nx, ny, ng, gam, U_OLD, U_NEW, XY, S, quirks);
+    or_global_boundary(edge, edge_prm, nx, ny, ng, gam, U_OLD, U_NEW);
+    std::vector<double> U2(U_NEW, U_NEW + n4);
+    or_advance(scheme, rs, lim, quirks, L, H, nx, ny, ng, gam, dt, U_OLD, T, U2.data(), XY);
+    for (int j = ng; j < ny + ng; j++)
+        for (int i = ng; i < nx + ng; i++) {
+            const size_t o = (size_t)g.c(i, j) * 4;
+            const bool fluid = XY[g.c(i, j) * 6 + 5] == 0;
+            for (int k = 0; k < 4; k++) U_NEW[o + k] = fluid ? 0.5 * (U0[o + k] + U2[o + k]) : U_OLD[o + k];
+        }
+}
+
 // Whole-run driver in the order of Test/Sod/Sod.cpp:94-181 / Test/InclineSod/InclineSod.cpp:73-165:
 // (state, geometry and scheme index already set up by the caller)
 //   [if init_boundary] Boundary ; dt = ComputeDt
-//   while (now_t < t_end && iter < max_iter): Advance(dt); Boundary; Global_Boundary;
+//   while (now_t < t_end && iter < max_iter): TimeAdvance(dt); Boundary; Global_Boundary;
 //       dt = ComputeDt; mass; now_t += dt; iter++
 // Returns the number of steps taken; per-step dt/time/mass go to the log arrays (may be null).
-int or_run(int scheme, int rs, int lim, int bc, int edge, const double* edge_prm, int quirks, double L, double H,
-           int nx, int ny, int ng, double gam, double cfl, double t_end, int max_iter, int init_boundary,
+int or_run(int time, int scheme, int rs, int lim, int bc, int edge, const double* edge_prm, int quirks, double L,
+           double H, int nx, int ny, int ng, double gam, double cfl, double t_end, int max_iter, int init_boundary,
            double* U_OLD, double* U_NEW, const double* XY, const double* S, double* T, double* dt_log,
            double* t_log, double* mass_log) {
     if (init_boundary) or_boundary(bc, nx, ny, ng, gam, U_OLD, U_NEW, XY, S, quirks);
@@ -1061,7 +1091,8 @@ int or_run(int scheme, int rs, int lim, int bc, int edge, const double* edge_prm
     double now_t = 0;
     int iter = 0;
     while (now_t < t_end && iter < max_iter) {
-        or_advance(scheme, rs, lim, quirks, L, H, nx, ny, ng, gam, dt, U_OLD, T, U_NEW, XY);
+        or_time_advance(time, scheme, rs, lim, bc, edge, edge_prm, quirks, L, H, nx, ny, ng, gam, dt, U_OLD, T,
+                        U_NEW, XY, S);
         or_boundary(bc, nx, ny, ng, gam, U_OLD, U_NEW, XY, S, quirks);
         or_global_boundary(edge, edge_prm, nx, ny, ng, gam, U_OLD, U_NEW);
         dt = or_compute_dt(L, H, nx, ny, ng, gam, cfl, U_OLD, XY);

@@ -24,6 +24,7 @@ RIEMANNS = {"HLL": 0, "HLLC": 1, "PVRS": 2, "StegerWarming": 3}
 LIMITERS = {"minmod": 0, "VanLeer": 1, "VanAlbada": 2}
 BOUNDS = {"Regular": 0, "IDWBaseGFM": 1, "LinearBaseGFM": 2, "RiemannBaseGFM": 3, "BCGFM": 4, "HBCGFM": 5}
 EDGES = {"reference": 0, "periodic": 1, "inflow": 2}
+TIMES = {"SingleStep": 0, "RK2": 1}
 # level-set hooks and initialisers (see or_level_set_hook / or_initialize)
 LS = {"sod": 0, "sodx": 0, "incline": 1, "vortex": 2, "phi": 3, "none": 0}
 INIT = {"sod": 0, "incline": 1, "vortex": 2, "sodx": 3, "uniform": 4, "shu": 5}
@@ -72,6 +73,7 @@ class Case:
     Riemann: str = "HLL"
     Limiter: str = "minmod"
     Boundary: str = "Regular"
+    Time: str = "SingleStep"
     case: str = "sodx"          # hook set: sod | sodx | incline | vortex | uniform | shu
     edge: str = "reference"     # Global_Boundary policy
     quirks: int = 1             # 1 = reference arithmetic incl. its defects, 0 = corrected
@@ -184,7 +186,7 @@ def run_oracle(case, nsteps, root_mode=0, state=None):
     t_log = np.zeros(max(nsteps, 1))
     m_log = np.zeros(max(nsteps, 1))
     ep = np.array(g.edge_prm(), dtype=np.float64)
-    n = L.or_run(SCHEMES[g.Scheme], RIEMANNS[g.Riemann], LIMITERS[g.Limiter], BOUNDS[g.Boundary],
+    n = L.or_run(TIMES[g.Time], SCHEMES[g.Scheme], RIEMANNS[g.Riemann], LIMITERS[g.Limiter], BOUNDS[g.Boundary],
                  EDGES[g.edge], ptr(ep), g.quirks, C.c_double(g.L), C.c_double(g.H), g.N_x, g.N_y,
                  g.num_ghost_cell, C.c_double(g.gamma), C.c_double(g.CFL_number), C.c_double(g.Psy_time),
                  min(g.max_iter, nsteps), g.init_boundary(), ptr(st["U_OLD"]), ptr(st["U_NEW"]), ptr(st["XY"]),

@@ -18,7 +18,7 @@ def gpu_config(case, **over):
     kw = dict(N_x=case.N_x, N_y=case.N_y, lo_x=case.lo_x, lo_y=case.lo_y, hi_x=case.hi_x, hi_y=case.hi_y,
               Psy_time=case.Psy_time, max_iter=case.max_iter, CFL_number=case.CFL_number,
               num_ghost_cell=case.num_ghost_cell, gamma=case.gamma, Scheme=case.Scheme, Riemann=case.Riemann,
-              Limiter=case.Limiter, Boundary=case.Boundary, Edge=case.edge, quirks=case.quirks,
+              Limiter=case.Limiter, Boundary=case.Boundary, Time=case.Time, Edge=case.edge, quirks=case.quirks,
               edge_state=case.edge_prm())
     kw.update(over)
     return ib.config(**kw)

@@ -201,3 +201,31 @@ def test_shapes_levelset_matches_numpy(oracle_mod):
     assert np.max(np.abs(phi - c.phi)) <= 1e-15
     st = O.setup(c)
     assert nan_equal_bits(XY[5::6], st["XY"][5::6]) == 0
+
+
+@pytest.mark.parametrize("name", ["sodx_minmod", "sodx_vanleer_mhm", "incline_idw", "incline_bcgfm", "vortex_periodic",
+                                  "bodies_inflow"])
+def test_rk2_heun_bitexact(oracle_mod, name):
+    """Time = RK2 (Heun, the reference RK2's intent): two step-kernel stages with
+    the ghost fill and edge ring between them, GPU == oracle bit for bit."""
+    O = oracle_mod
+    if name == "sodx_minmod":
+        c, n = _case(O, N_x=96, N_y=24, hi_y=0.25, Time="RK2", case="sodx"), 30
+    elif name == "sodx_vanleer_mhm":
+        c, n = _case(O, N_x=96, N_y=24, hi_y=0.25, Time="RK2", Scheme="MHM", Limiter="VanLeer", quirks=0,
+                     case="sodx"), 30
+    elif name == "incline_idw":
+        c, n = _case(O, N_x=64, N_y=64, hi_y=1.0, Boundary="IDWBaseGFM", Time="RK2", case="incline"), 20
+    elif name == "incline_bcgfm":
+        c, n = _case(O, N_x=64, N_y=64, hi_y=1.0, Boundary="BCGFM", Time="RK2", case="incline"), 20
+    elif name == "vortex_periodic":
+        c, n = _case(O, N_x=48, N_y=48, hi_x=10.0, hi_y=10.0, Psy_time=10.0, Time="RK2", Limiter="VanLeer",
+                     case="shu", edge="periodic", quirks=0, extra=dict(x0=5.0, y0=5.0, beta=5.0)), 25
+    else:
+        c, n = _bodies_case(O, nx=128, ny=64), 25
+        c.Time = "RK2"
+    g = gpu_run(c, n)
+    o = O.run_oracle(c, n, root_mode=1)
+    assert g["steps"] == o["steps"] == n
+    assert nan_equal_bits(g["U_OLD"], o["U_OLD"]) == 0
+    assert nan_equal_bits(g["dt_log"], o["dt"]) == 0
