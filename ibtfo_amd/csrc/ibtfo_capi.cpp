@@ -13,6 +13,7 @@
 #include "ibtfo_geom.h"
 #include "ibtfo_host.h"
 #include "ibtfo_internal.h"
+#include "ibtfo_ctx.h"
 
 using namespace ibtfo;
 
@@ -26,35 +27,6 @@ void set_error(const char* fmt, ...) {
 }
 }  // namespace ibtfo
 
-struct ibtfo_ctx {
-    ibtfo_config cfg;
-    Layout L;
-    int device;
-    hipStream_t stream;
-    hipStream_t own_stream;
-    double* state[2];
-    uint8_t* type;
-    double *phi, *gnx, *gny;
-    Ghost* ghosts;
-    int nghost;
-    double* gvals;
-    unsigned int* dcount;
-    Ctl* ctl;
-    double* mass_part;  // PART_STRIDE doubles per step-kernel block
-    int mass_cap;       // blocks
-    int rows_override;  // step-kernel rows per strip (0 = default shape)
-    double* corner;
-    double Li, Hi, dxc;  // cell-centre origin and spacing (Psy_coord)
-    int have_geometry, have_state, prepared;
-    StepArgs sa;
-    FillArgs fa;
-    EdgeArgs ea;
-    ReduceArgs ra;
-    FinishArgs fin;
-    hipGraphExec_t graph;
-    int graph_chunk;
-    std::vector<double> corner_host;
-};
 
 #define HIPCHK(x)                                                                  \
     do {                                                                           \
@@ -66,6 +38,26 @@ struct ibtfo_ctx {
     } while (0)
 
 static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+
+// Device memory of a context: from its arena when it belongs to a batch, else hipMalloc.
+template <class P> static hipError_t dalloc(ibtfo_ctx* x, P** p, size_t bytes) {
+    if (x->arena) {
+        *p = (P*)x->arena->alloc(bytes);
+        return *p ? hipSuccess : hipErrorOutOfMemory;
+    }
+    return hipMalloc((void**)p, bytes);
+}
+static void dfree(ibtfo_ctx* x, void* p) {
+    if (!p) return;
+    if (x->arena) x->arena->free(p);
+    else (void)hipFree(p);
+}
+// Copies ordered on the context's stream (never the null stream, which would
+// wait for every other blocking stream of the device).
+static hipError_t scopy(ibtfo_ctx* x, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, x->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(x->stream);
+}
 
 static Layout make_layout(const ibtfo_config* c) {
     Layout L;
@@ -127,18 +119,22 @@ extern "C" long long ibtfo_footprint(const ibtfo_config* c) {
     return b;
 }
 
-static void free_ctx(ibtfo_ctx* x) {
+void ibtfo::free_ctx(ibtfo_ctx* x) {
     if (!x) return;
     if (x->graph) (void)hipGraphExecDestroy(x->graph);
-    (void)hipFree(x->state[0]); (void)hipFree(x->state[1]); (void)hipFree(x->type);
-    (void)hipFree(x->phi); (void)hipFree(x->gnx); (void)hipFree(x->gny);
-    (void)hipFree(x->ghosts); (void)hipFree(x->gvals); (void)hipFree(x->dcount);
-    (void)hipFree(x->ctl); (void)hipFree(x->mass_part); (void)hipFree(x->corner);
+    for (void* p : {(void*)x->state[0], (void*)x->state[1], (void*)x->type, (void*)x->phi, (void*)x->gnx,
+                    (void*)x->gny, (void*)x->ghosts, (void*)x->gvals, (void*)x->dcount, (void*)x->ctl,
+                    (void*)x->mass_part, (void*)x->corner})
+        dfree(x, p);
     if (x->own_stream) (void)hipStreamDestroy(x->own_stream);
     delete x;
 }
 
-extern "C" int ibtfo_create(const ibtfo_config* cfg, ibtfo_ctx** out) {
+namespace ibtfo {
+// A context on `stream` (its own non-blocking stream when null) with device
+// memory from `arena` (hipMalloc when null); rows = step-kernel strip height (0 =
+// default). Everything the kernels may read before writing it is cleared.
+int ctx_create(const ibtfo_config* cfg, Arena* arena, hipStream_t stream, int rows, ibtfo_ctx** out) {
     if (!cfg || !out) { set_error("null argument"); return IBTFO_E_ARG; }
     *out = nullptr;
     int rc = validate(cfg);
@@ -146,37 +142,44 @@ extern "C" int ibtfo_create(const ibtfo_config* cfg, ibtfo_ctx** out) {
     ibtfo_ctx* x = new ibtfo_ctx();  // value-initialised: every plain member starts at zero
     x->cfg = *cfg;
     x->L = make_layout(cfg);
+    x->arena = arena;
+    x->rows_override = rows;
     const Layout& L = x->L;
     if (hipGetDevice(&x->device) != hipSuccess) { delete x; set_error("no HIP device"); return IBTFO_E_HIP; }
     auto fail = [&](const char* what) {
-        set_error("allocation of %s failed (%s)", what, hipGetErrorString(hipGetLastError()));
+        set_error("allocation of %s failed (%s)", what, arena ? "batch arena full" : hipGetErrorString(hipGetLastError()));
         free_ctx(x);
         return IBTFO_E_NOMEM;
     };
-    if (hipMalloc(&x->state[0], 4 * L.plane * sizeof(double)) != hipSuccess) return fail("state");
-    if (hipMalloc(&x->state[1], 4 * L.plane * sizeof(double)) != hipSuccess) return fail("state");
-    if (hipMalloc(&x->type, L.plane) != hipSuccess) return fail("types");
-    if (hipMalloc(&x->phi, L.plane * sizeof(double)) != hipSuccess) return fail("phi");
-    if (hipMalloc(&x->gnx, L.plane * sizeof(double)) != hipSuccess) return fail("normals");
-    if (hipMalloc(&x->gny, L.plane * sizeof(double)) != hipSuccess) return fail("normals");
-    if (hipMalloc(&x->ctl, sizeof(Ctl)) != hipSuccess) return fail("control");
-    if (hipMalloc(&x->dcount, sizeof(unsigned int)) != hipSuccess) return fail("counter");
-    int nbx, nby, rows;
-    step_grid(L, 0, &nbx, &nby, &rows);
-    x->mass_cap = std::max(nbx * nby, 4096);
-    if (hipMalloc(&x->mass_part, PART_STRIDE * (size_t)x->mass_cap * sizeof(double)) != hipSuccess) return fail("partials");
-    if (hipMalloc(&x->corner, 2 * 2 * L.ng * 4 * sizeof(double)) != hipSuccess) return fail("corner");
-    if (hipStreamCreate(&x->own_stream) != hipSuccess) return fail("stream");
-    x->stream = x->own_stream;
-    // zero everything the kernels may read before it is written
-    if (hipMemset(x->state[0], 0, 4 * L.plane * sizeof(double)) != hipSuccess ||
-        hipMemset(x->state[1], 0, 4 * L.plane * sizeof(double)) != hipSuccess ||
-        hipMemset(x->type, 2, L.plane) != hipSuccess || hipMemset(x->ctl, 0, sizeof(Ctl)) != hipSuccess ||
-        hipMemset(x->phi, 0, L.plane * sizeof(double)) != hipSuccess ||
-        hipMemset(x->gnx, 0, L.plane * sizeof(double)) != hipSuccess ||
-        hipMemset(x->gny, 0, L.plane * sizeof(double)) != hipSuccess ||
-        hipMemset(x->corner, 0, 2 * 2 * L.ng * 4 * sizeof(double)) != hipSuccess ||
-        hipDeviceSynchronize() != hipSuccess)
+    if (dalloc(x, &x->state[0], 4 * L.plane * sizeof(double)) != hipSuccess) return fail("state");
+    if (dalloc(x, &x->state[1], 4 * L.plane * sizeof(double)) != hipSuccess) return fail("state");
+    if (dalloc(x, &x->type, L.plane) != hipSuccess) return fail("types");
+    if (dalloc(x, &x->phi, L.plane * sizeof(double)) != hipSuccess) return fail("phi");
+    if (dalloc(x, &x->gnx, L.plane * sizeof(double)) != hipSuccess) return fail("normals");
+    if (dalloc(x, &x->gny, L.plane * sizeof(double)) != hipSuccess) return fail("normals");
+    if (dalloc(x, &x->ctl, sizeof(Ctl)) != hipSuccess) return fail("control");
+    if (dalloc(x, &x->dcount, sizeof(unsigned int)) != hipSuccess) return fail("counter");
+    int nbx, nby, r;
+    step_grid(L, rows, &nbx, &nby, &r);
+    x->mass_cap = arena ? nbx * nby : std::max(nbx * nby, 4096);
+    if (dalloc(x, &x->mass_part, PART_STRIDE * (size_t)x->mass_cap * sizeof(double)) != hipSuccess)
+        return fail("partials");
+    if (dalloc(x, &x->corner, 2 * 2 * L.ng * 4 * sizeof(double)) != hipSuccess) return fail("corner");
+    if (stream) {
+        x->stream = stream;
+    } else {
+        if (hipStreamCreateWithFlags(&x->own_stream, hipStreamNonBlocking) != hipSuccess) return fail("stream");
+        x->stream = x->own_stream;
+    }
+    if (hipMemsetAsync(x->state[0], 0, 4 * L.plane * sizeof(double), x->stream) != hipSuccess ||
+        hipMemsetAsync(x->state[1], 0, 4 * L.plane * sizeof(double), x->stream) != hipSuccess ||
+        hipMemsetAsync(x->type, 2, L.plane, x->stream) != hipSuccess ||
+        hipMemsetAsync(x->ctl, 0, sizeof(Ctl), x->stream) != hipSuccess ||
+        hipMemsetAsync(x->phi, 0, L.plane * sizeof(double), x->stream) != hipSuccess ||
+        hipMemsetAsync(x->gnx, 0, L.plane * sizeof(double), x->stream) != hipSuccess ||
+        hipMemsetAsync(x->gny, 0, L.plane * sizeof(double), x->stream) != hipSuccess ||
+        hipMemsetAsync(x->corner, 0, 2 * 2 * L.ng * 4 * sizeof(double), x->stream) != hipSuccess ||
+        hipStreamSynchronize(x->stream) != hipSuccess)
         return fail("initial clear");
     const double Psy_L = cfg->hi_x - cfg->lo_x;
     x->dxc = Psy_L / cfg->N_x;  // Psy_coord.cpp:21-22 (y spaced with dx as well)
@@ -184,6 +187,11 @@ extern "C" int ibtfo_create(const ibtfo_config* cfg, ibtfo_ctx** out) {
     x->Hi = -x->dxc * L.ng + cfg->lo_y + x->dxc / 2;
     *out = x;
     return IBTFO_OK;
+}
+}  // namespace ibtfo
+
+extern "C" int ibtfo_create(const ibtfo_config* cfg, ibtfo_ctx** out) {
+    return ctx_create(cfg, nullptr, nullptr, 0, out);
 }
 
 extern "C" int ibtfo_destroy(ibtfo_ctx* x) {
@@ -264,8 +272,8 @@ extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int npr
                 if (sh[0] != 0.0 && (sh[4] < 3 || sh[4] > 8)) { set_error("polygon needs 3..8 vertices"); return IBTFO_E_ARG; }
             }
             if (nprm > 1) {
-                HIPCHK(hipMalloc(&dphi, (nprm - 1) * sizeof(double)));
-                HIPCHK(hipMemcpy(dphi, prm + 1, (nprm - 1) * sizeof(double), hipMemcpyHostToDevice));
+                if (dalloc(x, &dphi, (nprm - 1) * sizeof(double)) != hipSuccess) { set_error("geometry: out of device memory"); return IBTFO_E_NOMEM; }
+                HIPCHK(scopy(x, dphi, prm + 1, (nprm - 1) * sizeof(double), hipMemcpyHostToDevice));
             }
             g.shapes = dphi;
             g.nshapes = (int)prm[0];
@@ -273,8 +281,8 @@ extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int npr
         }
         case IBTFO_LS_FIELD: {
             if (!phi_host) { set_error("IBTFO_LS_FIELD needs phi"); return IBTFO_E_ARG; }
-            HIPCHK(hipMalloc(&dphi, (size_t)L.M * L.N * sizeof(double)));
-            HIPCHK(hipMemcpy(dphi, phi_host, (size_t)L.M * L.N * sizeof(double), hipMemcpyHostToDevice));
+            if (dalloc(x, &dphi, (size_t)L.M * L.N * sizeof(double)) != hipSuccess) { set_error("geometry: out of device memory"); return IBTFO_E_NOMEM; }
+            HIPCHK(scopy(x, dphi, phi_host, (size_t)L.M * L.N * sizeof(double), hipMemcpyHostToDevice));
             g.phi_in = dphi;
             break;
         }
@@ -282,8 +290,8 @@ extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int npr
     }
     HIPCHK(geom_run(g, x->stream));
     // ghost list (upper bound: all interior cells of type 2; count first)
-    (void)hipFree(x->ghosts); x->ghosts = nullptr;
-    (void)hipFree(x->gvals); x->gvals = nullptr;
+    dfree(x, x->ghosts); x->ghosts = nullptr;
+    dfree(x, x->gvals); x->gvals = nullptr;
     x->nghost = 0;
     if (x->cfg.boundary != IBTFO_REGULAR) {
         unsigned int zero = 0, cnt = 0;
@@ -293,8 +301,12 @@ extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int npr
         HIPCHK(hipMemcpyAsync(&cnt, x->dcount, sizeof cnt, hipMemcpyDeviceToHost, x->stream));
         HIPCHK(hipStreamSynchronize(x->stream));
         if (cnt > 0) {
-            HIPCHK(hipMalloc(&x->ghosts, cnt * sizeof(Ghost)));
-            HIPCHK(hipMalloc(&x->gvals, cnt * 4 * sizeof(double)));
+            if (dalloc(x, &x->ghosts, cnt * sizeof(Ghost)) != hipSuccess ||
+                dalloc(x, &x->gvals, cnt * 4 * sizeof(double)) != hipSuccess) {
+                dfree(x, dphi);
+                set_error("geometry: out of device memory for %u ghost records", cnt);
+                return IBTFO_E_NOMEM;
+            }
             HIPCHK(hipMemcpyAsync(x->dcount, &zero, sizeof zero, hipMemcpyHostToDevice, x->stream));
             g.ghosts = x->ghosts; g.cap = (int)cnt;
             HIPCHK(geom_ghosts(g, x->stream));
@@ -302,7 +314,7 @@ extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int npr
         x->nghost = (int)cnt;
     }
     HIPCHK(hipStreamSynchronize(x->stream));
-    if (dphi) (void)hipFree(dphi);
+    dfree(x, dphi);
     x->have_geometry = 1;
     x->prepared = 0;
     return IBTFO_OK;
@@ -314,10 +326,10 @@ extern "C" int ibtfo_download_geometry(ibtfo_ctx* x, double* XY) {
     std::vector<double> phi(L.plane), nx(L.plane), ny(L.plane);
     std::vector<uint8_t> t(L.plane);
     HIPCHK(hipStreamSynchronize(x->stream));
-    HIPCHK(hipMemcpy(phi.data(), x->phi, L.plane * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(nx.data(), x->gnx, L.plane * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(ny.data(), x->gny, L.plane * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(t.data(), x->type, L.plane, hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, phi.data(), x->phi, L.plane * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, nx.data(), x->gnx, L.plane * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, ny.data(), x->gny, L.plane * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, t.data(), x->type, L.plane, hipMemcpyDeviceToHost));
     for (int j = 0; j < L.N; j++)
         for (int i = 0; i < L.M; i++) {
             double* r = XY + ((size_t)L.M * j + i) * 6;
@@ -337,7 +349,7 @@ extern "C" int ibtfo_download_scheme_index(ibtfo_ctx* x, double* S) {
     if (x->cfg.boundary == IBTFO_REGULAR || x->nghost == 0) return IBTFO_OK;
     std::vector<Ghost> gh(x->nghost);
     HIPCHK(hipStreamSynchronize(x->stream));
-    HIPCHK(hipMemcpy(gh.data(), x->ghosts, x->nghost * sizeof(Ghost), hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, gh.data(), x->ghosts, x->nghost * sizeof(Ghost), hipMemcpyDeviceToHost));
     for (const Ghost& g : gh) {
         long long rel = g.cell - L.off;
         int j = (int)(rel / L.pitch), i = (int)(rel % L.pitch);
@@ -374,18 +386,18 @@ static int upload_aos(ibtfo_ctx* x, const double* U, int buf) {
             long long o = L.at(i, j);
             for (int k = 0; k < 4; k++) planes[o + k * L.plane] = q[k];
         }
-    HIPCHK(hipMemcpy(x->state[buf], planes.data(), 4 * L.plane * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(scopy(x, x->state[buf], planes.data(), 4 * L.plane * sizeof(double), hipMemcpyHostToDevice));
     return IBTFO_OK;
 }
 
 static int read_ctl_head(ibtfo_ctx* x, Ctl& c) {
     HIPCHK(hipStreamSynchronize(x->stream));
-    HIPCHK(hipMemcpy(&c, x->ctl, offsetof(Ctl, log_dt), hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, &c, x->ctl, offsetof(Ctl, log_dt), hipMemcpyDeviceToHost));
     return IBTFO_OK;
 }
 static int write_ctl_head(ibtfo_ctx* x, const Ctl& c) {
     HIPCHK(hipStreamSynchronize(x->stream));
-    HIPCHK(hipMemcpy(x->ctl, &c, offsetof(Ctl, log_dt), hipMemcpyHostToDevice));
+    HIPCHK(scopy(x, x->ctl, &c, offsetof(Ctl, log_dt), hipMemcpyHostToDevice));
     return IBTFO_OK;
 }
 
@@ -395,7 +407,7 @@ extern "C" int ibtfo_upload_state(ibtfo_ctx* x, const double* U_OLD, const doubl
     int rc = upload_aos(x, U_OLD, 0);
     if (rc) return rc;
     corner_from_aos(x, U_NEW ? U_NEW : U_OLD);
-    HIPCHK(hipMemcpy(x->corner, x->corner_host.data(), x->corner_host.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(scopy(x, x->corner, x->corner_host.data(), x->corner_host.size() * sizeof(double), hipMemcpyHostToDevice));
     Ctl c;
     if ((rc = read_ctl_head(x, c))) return rc;
     c.cur = 0;
@@ -413,7 +425,7 @@ extern "C" int ibtfo_download_state(ibtfo_ctx* x, double* U) {
     int rc = read_ctl_head(x, c);
     if (rc) return rc;
     std::vector<double> planes(4 * L.plane);
-    HIPCHK(hipMemcpy(planes.data(), x->state[c.cur], 4 * L.plane * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, planes.data(), x->state[c.cur], 4 * L.plane * sizeof(double), hipMemcpyDeviceToHost));
     for (int j = 0; j < L.N; j++)
         for (int i = 0; i < L.M; i++) {
             double* q = U + ((size_t)L.M * j + i) * 4;
@@ -442,7 +454,7 @@ extern "C" int ibtfo_initialize(ibtfo_ctx* x, int kind, const double* prm, int n
     std::vector<double> xs(cells), ys(cells), U(cells * 4);
     std::vector<uint8_t> tp(L.plane), t(cells);
     HIPCHK(hipStreamSynchronize(x->stream));
-    HIPCHK(hipMemcpy(tp.data(), x->type, L.plane, hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, tp.data(), x->type, L.plane, hipMemcpyDeviceToHost));
     for (int j = 0; j < L.N; j++)
         for (int i = 0; i < L.M; i++) {
             size_t q = (size_t)L.M * j + i;
@@ -456,7 +468,7 @@ extern "C" int ibtfo_initialize(ibtfo_ctx* x, int kind, const double* prm, int n
 }
 
 // -------------------------------------------------------------- time loop
-static void build_args(ibtfo_ctx* x) {
+void ibtfo::build_args(ibtfo_ctx* x) {
     const ibtfo_config& c = x->cfg;
     const Layout& L = x->L;
     StepArgs& a = x->sa;
@@ -639,9 +651,9 @@ extern "C" int ibtfo_log(ibtfo_ctx* x, int count, double* dt, double* t, double*
     if (rc) return rc;
     if (count > c.iter) { set_error("only %d steps logged", c.iter); return IBTFO_E_ARG; }
     std::vector<double> ld(LOGCAP), lt(LOGCAP), lm(LOGCAP);
-    HIPCHK(hipMemcpy(ld.data(), x->ctl->log_dt, sizeof(double) * LOGCAP, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(lt.data(), x->ctl->log_t, sizeof(double) * LOGCAP, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(lm.data(), x->ctl->log_mass, sizeof(double) * LOGCAP, hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, ld.data(), x->ctl->log_dt, sizeof(double) * LOGCAP, hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, lt.data(), x->ctl->log_t, sizeof(double) * LOGCAP, hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, lm.data(), x->ctl->log_mass, sizeof(double) * LOGCAP, hipMemcpyDeviceToHost));
     for (int q = 0; q < count; q++) {
         int it = c.iter - count + q;
         int s = it % LOGCAP;
@@ -671,7 +683,7 @@ extern "C" int ibtfo_mass(ibtfo_ctx* x, double* mass) {
     int rc = run_reduce(x, 0);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(x->stream));
-    HIPCHK(hipMemcpy(mass, &x->ctl->mass_now, sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(scopy(x, mass, &x->ctl->mass_now, sizeof(double), hipMemcpyDeviceToHost));
     return IBTFO_OK;
 }
 
@@ -722,12 +734,12 @@ extern "C" int ibtfo_set_rows(ibtfo_ctx* x, int rows) {
     step_grid(x->L, rows, &nbx, &nby, &r);
     if (nbx * nby > x->mass_cap) {
         double* p = nullptr;
-        if (hipMalloc(&p, PART_STRIDE * (size_t)nbx * nby * sizeof(double)) != hipSuccess) {
+        if (dalloc(x, &p, PART_STRIDE * (size_t)nbx * nby * sizeof(double)) != hipSuccess) {
             set_error("ibtfo_set_rows: allocation of %d block partials failed", nbx * nby);
             return IBTFO_E_NOMEM;
         }
         HIPCHK(hipStreamSynchronize(x->stream));
-        (void)hipFree(x->mass_part);
+        dfree(x, x->mass_part);
         x->mass_part = p;
         x->mass_cap = nbx * nby;
     }

@@ -1,0 +1,50 @@
+// The solve context behind the C ABI's opaque ibtfo_ctx, shared by the C-ABI
+// translation unit and the batch engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/ibtfo.h"
+#include "ibtfo_host.h"
+#include "ibtfo_internal.h"
+
+struct ibtfo_ctx {
+    // one solve: its configuration, device memory, launch arguments and stream
+    ibtfo_config cfg;
+    ibtfo::Layout L;
+    int device;
+    hipStream_t stream;
+    hipStream_t own_stream;
+    double* state[2];
+    uint8_t* type;
+    double *phi, *gnx, *gny;
+    ibtfo::Ghost* ghosts;
+    int nghost;
+    double* gvals;
+    unsigned int* dcount;
+    ibtfo::Ctl* ctl;
+    double* mass_part;  // PART_STRIDE doubles per step-kernel block
+    int mass_cap;       // blocks
+    int rows_override;  // step-kernel rows per strip (0 = default shape)
+    double* corner;
+    double Li, Hi, dxc;  // cell-centre origin and spacing (Psy_coord)
+    int have_geometry, have_state, prepared;
+    ibtfo::StepArgs sa;
+    ibtfo::FillArgs fa;
+    ibtfo::EdgeArgs ea;
+    ibtfo::ReduceArgs ra;
+    ibtfo::FinishArgs fin;
+    hipGraphExec_t graph;
+    int graph_chunk;
+    std::vector<double> corner_host;
+    ibtfo::Arena* arena;  // device memory from a batch engine's arena (else hipMalloc)
+};
+
+namespace ibtfo {
+// Creates a context on `stream` (own non-blocking stream when null) with device
+// memory from `arena` (hipMalloc when null); rows = strip height (0 = default).
+int ctx_create(const ibtfo_config* cfg, Arena* arena, hipStream_t stream, int rows, ibtfo_ctx** out);
+void free_ctx(ibtfo_ctx* x);   // releases device memory and the context (no synchronisation)
+void build_args(ibtfo_ctx* x); // (re)derives the kernel arguments from the context
+}  // namespace ibtfo

@@ -126,6 +126,15 @@ struct FinishArgs {
     int reset;    // with advance = 0: start the run clock
 };
 
+constexpr int FILL_THREADS = 128;  // ghost-fill and scatter blocks
+constexpr int EDGE_THREADS = 256;  // edge-ring blocks
+
+// per-slot status record of the batch engine (k_gather)
+struct SlotStatus {
+    int iter, done, status, pad;
+    double now_t, dt, mass;
+};
+
 // kernel launchers (ibtfo_step.hip)
 hipError_t launch_step(int scheme, int rs, int lim, int prec, const StepArgs& a, hipStream_t s);
 hipError_t launch_fill(const FillArgs& a, hipStream_t s);
@@ -133,6 +142,14 @@ hipError_t launch_scatter(const FillArgs& a, hipStream_t s);
 hipError_t launch_edge(const EdgeArgs& a, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
 hipError_t launch_finish(const FinishArgs& f, hipStream_t s);
+// batched forms: one launch over many solves (maps: block -> {slot, first index})
+hipError_t launch_step_batch(int scheme, int rs, int lim, int prec, const StepArgs* slots, const int2* map,
+                             int nblocks, int stage, hipStream_t s);
+hipError_t launch_finish_batch(const FinishArgs* f, int nslots, int advance, hipStream_t s);
+hipError_t launch_fill_batch(const FillArgs* f, const int2* map, int nblocks, hipStream_t s);
+hipError_t launch_scatter_batch(const FillArgs* f, const int2* map, int nblocks, hipStream_t s);
+hipError_t launch_edge_batch(const EdgeArgs* e, const int2* map, int nblocks, hipStream_t s);
+hipError_t launch_gather(const FinishArgs* f, int nslots, SlotStatus* out, hipStream_t s);
 hipError_t launch_step_lax(const StepArgs& a, hipStream_t s);
 hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_from_input, hipStream_t s);
 

@@ -101,10 +101,10 @@ __device__ __forceinline__ void block_partials(double* part, int bid, double ama
 // the block partials, then the host loop's bookkeeping of Test/Sod/Sod.cpp:154-177:
 // dt for the next step, now_t += dt, iter++, stop test, log entry. `advance` = 0
 // for the stand-alone ComputeDt (no step taken; `reset` starts the run clock).
-__global__ __launch_bounds__(FINISH_THREADS) void k_finish(FinishArgs f) {
+__device__ __forceinline__ void finish_body(const FinishArgs& f, const int advance) {
     Ctl* ctl = f.ctl;
-    if (f.advance && ctl->done) return;  // the step kernel did not run
-    if (f.advance == 2) {
+    if (advance && ctl->done) return;  // the step kernel did not run
+    if (advance == 2) {
         if (threadIdx.x == 0) {
             ctl->cur ^= 1;
             ctl->ran = 1;
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_finish(FinishArgs f) {
         if (sB[0]) ctl->status |= 1;
         double dt = f.cfl * f.dx / (A + ((Um < Vm) ? Vm : Um));
         ctl->dt = dt;
-        if (f.advance) {
+        if (advance) {
             double tn = ctl->now_t + dt;
             int it = ctl->iter + 1;
             int slot = ctl->iter % LOGCAP;
@@ -168,6 +168,14 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_finish(FinishArgs f) {
             }
         }
     }
+}
+__global__ __launch_bounds__(FINISH_THREADS) void k_finish(FinishArgs f) { finish_body(f, f.advance); }
+
+// Batched finish: block s finishes solve s (slots with a null control block are idle).
+__global__ __launch_bounds__(FINISH_THREADS) void k_finish_batch(const FinishArgs* __restrict__ f, int nslots,
+                                                                 int advance) {
+    if ((int)blockIdx.x >= nslots || f[blockIdx.x].ctl == nullptr) return;
+    finish_body(f[blockIdx.x], advance);
 }
 
 // Primitive view of an FP64 conserved state in the compute type T. FP64:
@@ -298,13 +306,16 @@ __device__ __forceinline__ void cell_states(const double* um, const double* u0, 
     }
 }
 
+// The step body for one 64-lane block: strip (bx, by) of the solve described
+// by `a`; `stage` as StepArgs::stage. Used by the single-solve kernel and by the
+// batched kernel, whose blocks each pick their solve from a block map.
 template <int SCH, int RS, int LIM, typename T>
-__global__ __launch_bounds__(64) void k_step(StepArgs a) {
+__device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, const int by, const int stage) {
     constexpr bool FP64 = sizeof(T) == sizeof(double);
     Ctl* ctl = a.ctl;
     const int lane = threadIdx.x;
     if (ctl->done) {
-        if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0) ctl->ran = 0;
+        if (bx == 0 && by == 0 && lane == 0) ctl->ran = 0;
         return;
     }
     const Layout L = a.L;
@@ -325,10 +336,10 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
     const double dx = a.dx, dy = a.dy;
     const int ng = L.ng, nx = L.nx, ny = L.ny;
 
-    const int cb = ng + STRIP_W * blockIdx.x;  // first interior column of the strip
+    const int cb = ng + STRIP_W * bx;  // first interior column of the strip
     const int c = cb - 1 + lane;
     const int cl = min(max(c, 0), L.M - 1);
-    const int j0 = ng + a.rows * blockIdx.y;
+    const int j0 = ng + a.rows * by;
     const int j1 = min(j0 + a.rows, ny + ng);
     // lanes 0 and 1 also stage the outer halo columns cb-2 and cb+63
     const int ecol = min(max(lane == 0 ? cb - 2 : cb + 63, 0), L.M - 1);
@@ -340,7 +351,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
 
     // reference Global_Boundary corners (Test/Sod/Global_Boudary.cpp:62-75 applied to
     // ring rows): U_OLD(i<ng or i>=N_x+ng, j ring) = U_NEW(ng or N_x+ng-1, j)
-    if (a.edge_fused && blockIdx.x == 0 && blockIdx.y == 0 && lane < 4 * ng * ng) {
+    if (a.edge_fused && bx == 0 && by == 0 && lane < 4 * ng * ng) {
         int side = lane / (2 * ng * ng);
         int rem = lane % (2 * ng * ng);
         int ri = rem / ng, cc = rem % ng;
@@ -450,7 +461,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
                             un[k] = w[0][k] - dtdx * ((double)F[k] - Fm) - dtdy * ((double)G[k] - (double)Gp[k]);
                         }
                     }
-                    if (a.stage == 0) {
+                    if (stage == 0) {
                         floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, amax, umax, vmax, bad);
                     } else {
                         // RK2 (Heun): each stage's update takes the floors; the second
@@ -458,7 +469,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
                         double d0 = 0.0, d1 = 0.0, d2 = 0.0;
                         int db = 0;
                         floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, d0, d1, d2, db);
-                        if (a.stage == 2) {
+                        if (stage == 2) {
 #pragma unroll
                             for (int k = 0; k < 4; k++) un[k] = 0.5 * (Uo[o + k * PL] + un[k]);
                             dt_terms(a.gam, a.gm1, un, amax, umax, vmax, bad);
@@ -508,7 +519,25 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
             ex[k] = nex[k];
         }
     }
-    block_partials(a.mass_part, blockIdx.y * a.nbx + blockIdx.x, amax, umax, vmax, msum, bad);
+    block_partials(a.mass_part, by * a.nbx + bx, amax, umax, vmax, msum, bad);
+}
+
+template <int SCH, int RS, int LIM, typename T>
+__global__ __launch_bounds__(64) void k_step(StepArgs a) {
+    step_strip<SCH, RS, LIM, T>(a, blockIdx.x, blockIdx.y, a.stage);
+}
+
+// Batched step: one launch over the strips of many solves. Block b runs strip
+// map[b].y of solve map[b].x (row-major over that solve's nbx x nby strips);
+// map[b].x < 0 marks an idle block (the grid is sized for the batch capacity so
+// that a captured graph stays valid while solves come and go).
+template <int SCH, int RS, int LIM, typename T>
+__global__ __launch_bounds__(64) void k_step_batch(const StepArgs* __restrict__ slots, const int2* __restrict__ map,
+                                                   int stage) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const StepArgs& a = slots[m.x];
+    step_strip<SCH, RS, LIM, T>(a, m.y % a.nbx, m.y / a.nbx, stage);
 }
 
 // Lax-Friedrichs, Lax_Scheme.cpp:27-62: U_new = average of the 4 neighbours minus
@@ -584,8 +613,7 @@ __global__ __launch_bounds__(64) void k_reduce(ReduceArgs a) {
 // after the update, before any ghost is rewritten) into a compact buffer.
 // IDWBaseGFM/GFM_Bound.cpp:65-106, LinearBaseGFM/GFM_Bound.cpp:58-575,
 // BCGFM/BCGFM_Bound.cpp:81-175, HBCGFM/HBCGFM_Bound.cpp:163-178.
-__global__ void k_fill(FillArgs a) {
-    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void fill_one(const FillArgs& a, const int gi) {
     if (a.require_ran && !a.ctl->ran) return;
     if (gi >= a.nghost) return;
     const Ghost gh = a.ghosts[gi];
@@ -667,10 +695,15 @@ __global__ void k_fill(FillArgs a) {
     o[2] = rw * vw;
     o[3] = pw / gm1 + 0.5 * rw * (uw * uw + vw * vw);
 }
+__global__ void k_fill(FillArgs a) { fill_one(a, blockIdx.x * blockDim.x + threadIdx.x); }
+// Batched fill: block b covers ghosts map[b].y .. +FILL_THREADS-1 of solve map[b].x.
+__global__ void k_fill_batch(const FillArgs* __restrict__ slots, const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x >= 0) fill_one(slots[m.x], m.y + threadIdx.x);
+}
 
 // Ghost-cell fill, phase 2: write the values into the current state.
-__global__ void k_scatter(FillArgs a) {
-    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void scatter_one(const FillArgs& a, const int gi) {
     if (a.require_ran && !a.ctl->ran) return;
     if (gi >= a.nghost) return;
     double* U = a.buf[a.use_cur ? a.ctl->cur : a.fixed];
@@ -681,16 +714,20 @@ __global__ void k_scatter(FillArgs a) {
     U[c + 2 * PL] = v[2];
     U[c + 3 * PL] = v[3];
 }
+__global__ void k_scatter(FillArgs a) { scatter_one(a, blockIdx.x * blockDim.x + threadIdx.x); }
+__global__ void k_scatter_batch(const FillArgs* __restrict__ slots, const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x >= 0) scatter_one(slots[m.x], m.y + threadIdx.x);
+}
 
 // Ring of the current state for the edge policies that are not fused into the
 // step (the reference policy for Lax; periodic; inflow). Reads the interior of
 // the current state, writes its ring; one thread per ring cell.
-__global__ void k_edge(EdgeArgs a) {
+__device__ __forceinline__ void edge_one(const EdgeArgs& a, const int q) {
     if (a.require_ran && !a.ctl->ran) return;
     const Layout L = a.L;
     const int ng = L.ng, nx = L.nx, ny = L.ny;
     const int nring = L.M * L.N - nx * ny;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nring) return;
     // enumerate ring cells: bottom rows, top rows, then left/right columns of interior rows
     int i, j;
@@ -720,6 +757,12 @@ __global__ void k_edge(EdgeArgs a) {
     }
     long long o = L.at(i, j);
     for (int k = 0; k < 4; k++) U[o + k * PL] = v[k];
+}
+__global__ void k_edge(EdgeArgs a) { edge_one(a, blockIdx.x * blockDim.x + threadIdx.x); }
+// Batched edge ring: block b covers ring cells map[b].y .. +EDGE_THREADS-1 of solve map[b].x.
+__global__ void k_edge_batch(const EdgeArgs* __restrict__ slots, const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x >= 0) edge_one(slots[m.x], m.y + threadIdx.x);
 }
 
 // Reference Global_Boundary as a separate pass (used with Lax, whose step
@@ -762,29 +805,87 @@ __global__ void k_edge_ref(EdgeArgs a, const double* corner, int corner_from_inp
 }
 
 // ------------------------------------------------------------------ launchers
+// A batched launch (k_step_batch) instead of a single-solve one when non-null.
+struct BatchLaunch {
+    const StepArgs* slots;
+    const int2* map;
+    int nblocks;
+    int stage;
+};
 template <int SCH, int RS, int LIM>
-static hipError_t go(int prec, const StepArgs& a, hipStream_t s) {
-    if (prec == 1) hipLaunchKernelGGL((k_step<SCH, RS, LIM, float>), dim3(a.nbx, a.nby), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((k_step<SCH, RS, LIM, double>), dim3(a.nbx, a.nby), dim3(64), 0, s, a);
+static hipError_t go(int prec, const StepArgs* a, const BatchLaunch* b, hipStream_t s) {
+    if (b) {
+        if (prec == 1)
+            hipLaunchKernelGGL((k_step_batch<SCH, RS, LIM, float>), dim3(b->nblocks), dim3(64), 0, s, b->slots, b->map,
+                               b->stage);
+        else
+            hipLaunchKernelGGL((k_step_batch<SCH, RS, LIM, double>), dim3(b->nblocks), dim3(64), 0, s, b->slots,
+                               b->map, b->stage);
+    } else {
+        if (prec == 1) hipLaunchKernelGGL((k_step<SCH, RS, LIM, float>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
+        else hipLaunchKernelGGL((k_step<SCH, RS, LIM, double>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
+    }
     return hipGetLastError();
 }
 template <int SCH, int RS>
-static hipError_t go_l(int lim, int prec, const StepArgs& a, hipStream_t s) {
-    if (lim == LIM_MINMOD) return go<SCH, RS, LIM_MINMOD>(prec, a, s);
-    if (lim == LIM_VANLEER) return go<SCH, RS, LIM_VANLEER>(prec, a, s);
-    return go<SCH, RS, LIM_VANALBADA>(prec, a, s);
+static hipError_t go_l(int lim, int prec, const StepArgs* a, const BatchLaunch* b, hipStream_t s) {
+    if (lim == LIM_MINMOD) return go<SCH, RS, LIM_MINMOD>(prec, a, b, s);
+    if (lim == LIM_VANLEER) return go<SCH, RS, LIM_VANLEER>(prec, a, b, s);
+    return go<SCH, RS, LIM_VANALBADA>(prec, a, b, s);
 }
 template <int SCH>
-static hipError_t go_r(int rs, int lim, int prec, const StepArgs& a, hipStream_t s) {
-    if (rs == RS_SW) return go_l<SCH, RS_SW>(lim, prec, a, s);
-    if (rs == RS_PVRS) return go_l<SCH, RS_PVRS>(lim, prec, a, s);
-    return go_l<SCH, RS_HLL>(lim, prec, a, s);
+static hipError_t go_r(int rs, int lim, int prec, const StepArgs* a, const BatchLaunch* b, hipStream_t s) {
+    if (rs == RS_SW) return go_l<SCH, RS_SW>(lim, prec, a, b, s);
+    if (rs == RS_PVRS) return go_l<SCH, RS_PVRS>(lim, prec, a, b, s);
+    return go_l<SCH, RS_HLL>(lim, prec, a, b, s);
+}
+static hipError_t go_s(int scheme, int rs, int lim, int prec, const StepArgs* a, const BatchLaunch* b,
+                       hipStream_t s) {
+    if (scheme == SCH_MHM) return go_r<SCH_MHM>(rs, lim, prec, a, b, s);
+    if (scheme == SCH_GODUNOV) return go_r<SCH_GODUNOV>(rs, LIM_MINMOD, prec, a, b, s);
+    return go_r<SCH_MUSCL>(rs, lim, prec, a, b, s);
 }
 hipError_t launch_step(int scheme, int rs, int lim, int prec, const StepArgs& a, hipStream_t s) {
     if (scheme == SCH_LAX) return launch_step_lax(a, s);
-    if (scheme == SCH_MHM) return go_r<SCH_MHM>(rs, lim, prec, a, s);
-    if (scheme == SCH_GODUNOV) return go_r<SCH_GODUNOV>(rs, LIM_MINMOD, prec, a, s);
-    return go_r<SCH_MUSCL>(rs, lim, prec, a, s);
+    return go_s(scheme, rs, lim, prec, &a, nullptr, s);
+}
+hipError_t launch_step_batch(int scheme, int rs, int lim, int prec, const StepArgs* slots, const int2* map,
+                             int nblocks, int stage, hipStream_t s) {
+    if (scheme == SCH_LAX) return hipErrorInvalidValue;
+    BatchLaunch b{slots, map, nblocks, stage};
+    return go_s(scheme, rs, lim, prec, nullptr, &b, s);
+}
+hipError_t launch_finish_batch(const FinishArgs* f, int nslots, int advance, hipStream_t s) {
+    hipLaunchKernelGGL(k_finish_batch, dim3(nslots), dim3(FINISH_THREADS), 0, s, f, nslots, advance);
+    return hipGetLastError();
+}
+hipError_t launch_fill_batch(const FillArgs* f, const int2* map, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_batch, dim3(nblocks), dim3(FILL_THREADS), 0, s, f, map);
+    return hipGetLastError();
+}
+hipError_t launch_scatter_batch(const FillArgs* f, const int2* map, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_batch, dim3(nblocks), dim3(FILL_THREADS), 0, s, f, map);
+    return hipGetLastError();
+}
+hipError_t launch_edge_batch(const EdgeArgs* e, const int2* map, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_edge_batch, dim3(nblocks), dim3(EDGE_THREADS), 0, s, e, map);
+    return hipGetLastError();
+}
+// Batched status read-out: one record per slot (idle slots: ctl null -> zeros).
+__global__ void k_gather(const FinishArgs* __restrict__ f, int nslots, SlotStatus* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    SlotStatus st{};
+    const Ctl* c = f[i].ctl;
+    if (c) {
+        st.iter = c->iter; st.done = c->done; st.status = c->status;
+        st.now_t = c->now_t; st.dt = c->dt; st.mass = c->mass_now;
+    }
+    out[i] = st;
+}
+hipError_t launch_gather(const FinishArgs* f, int nslots, SlotStatus* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather, dim3((nslots + 63) / 64), dim3(64), 0, s, f, nslots, out);
+    return hipGetLastError();
 }
 hipError_t launch_step_lax(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_step_lax, dim3(a.nbx, a.nby), dim3(64), 0, s, a);
@@ -792,17 +893,17 @@ hipError_t launch_step_lax(const StepArgs& a, hipStream_t s) {
 }
 hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
     if (a.nghost == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill, dim3((a.nghost + 127) / 128), dim3(128), 0, s, a);
+    hipLaunchKernelGGL(k_fill, dim3((a.nghost + FILL_THREADS - 1) / FILL_THREADS), dim3(FILL_THREADS), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_scatter(const FillArgs& a, hipStream_t s) {
     if (a.nghost == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter, dim3((a.nghost + 127) / 128), dim3(128), 0, s, a);
+    hipLaunchKernelGGL(k_scatter, dim3((a.nghost + FILL_THREADS - 1) / FILL_THREADS), dim3(FILL_THREADS), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_edge(const EdgeArgs& a, hipStream_t s) {
     int nring = a.L.M * a.L.N - a.L.nx * a.L.ny;
-    hipLaunchKernelGGL(k_edge, dim3((nring + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_edge, dim3((nring + EDGE_THREADS - 1) / EDGE_THREADS), dim3(EDGE_THREADS), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_from_input, hipStream_t s) {

@@ -175,6 +175,86 @@ int ibtfo_autotune(ibtfo_ctx* ctx, int probe_steps, int* best_rows, double* best
  * row pitch in doubles and the offset of cell (0,0) in each plane. */
 int ibtfo_device_state(ibtfo_ctx* ctx, double** planes4, int* pitch, int* offset);
 
+/* ------------------------------------------------------------------------
+ * Batch engine: many independent solves (different grids, geometries, inflow
+ * states and step counts) served by one device-resident loop. No counterpart in
+ * the reference, which runs one solve per process.
+ *
+ * Requests queue on the host (FIFO). Each ibtfo_batch_progress call
+ *   - replays a captured HIP graph of `chunk` batched steps for the admitted
+ *     solves (one launch per kernel of the step sequence covers every solve),
+ *   - meanwhile admits queued requests whose device memory fits the remaining
+ *     HBM budget (an arena reserved at creation: no hipMalloc per request) and
+ *     builds their geometry and initial state on a second stream,
+ *   - then retires the solves that reached their step count, downloading the
+ *     state of those that asked for it and returning their memory.
+ * All solves of one batch share the kernel selection of ibtfo_batch_config;
+ * each request brings its own ibtfo_config (grid, extents, CFL, edge policy and
+ * inflow state; max_iter = its number of steps; Psy_time ends it early).
+ * ------------------------------------------------------------------------ */
+typedef struct ibtfo_batch ibtfo_batch;
+
+typedef struct ibtfo_batch_config {
+    int scheme, riemann, limiter, boundary, time, precision, quirks; /* shared kernel selection */
+    long long hbm_budget;  /* bytes of device memory for admitted solves */
+    int max_slots;         /* solves resident at once */
+    int max_step_blocks;   /* step-kernel grid capacity (strips of 62 columns x rows) */
+    int max_ghost_blocks;  /* ghost-fill grid capacity (128 ghosts per block) */
+    int max_ring_blocks;   /* edge-ring grid capacity (256 ring cells per block) */
+    int rows;              /* strip height of every solve (0: 16) */
+    int chunk;             /* steps per graph replay */
+} ibtfo_batch_config;
+
+typedef struct ibtfo_request {
+    ibtfo_config cfg;      /* the solve; slot fields must equal the batch's */
+    int ls_kind;           /* level-set hook (ibtfo_levelset_kind) and its parameters */
+    int ls_nprm;
+    const double* ls_prm;
+    const double* phi;     /* IBTFO_LS_FIELD: M*N values, else NULL */
+    int init_kind;         /* initialiser (ibtfo_init_kind) and its parameters */
+    int init_nprm;
+    const double* init_prm;
+    int keep_state;        /* 1: keep the final state for ibtfo_batch_result */
+    double arrival;        /* arrival time on the batch clock (s); < 0: now */
+} ibtfo_request;
+
+typedef struct ibtfo_result {
+    int status;            /* IBTFO_OK, or IBTFO_E_NONFINITE if a state went bad */
+    int iter;              /* steps taken */
+    double now_t, dt, mass;
+    int M, N;              /* state dimensions (with ghosts) */
+    double t_arrival, t_admit, t_done; /* batch clock (s) */
+} ibtfo_result;
+
+typedef struct ibtfo_batch_stats {
+    int active, queued, finished;
+    long long chunks, steps, cell_updates; /* interior cells x steps of finished and active solves */
+    long long arena_used, arena_capacity;
+    double busy_s;         /* time spent in progress calls */
+} ibtfo_batch_stats;
+
+/* Default batch settings: FP64 MUSCL / HLL / minmod / Regular, 8 GiB budget,
+ * 256 slots, capacities for that budget, rows 16, chunk 50. */
+int ibtfo_batch_config_default(ibtfo_batch_config* bc);
+/* Reserves the arena, the device tables and two non-blocking streams. */
+int ibtfo_batch_create(const ibtfo_batch_config* bc, ibtfo_batch** out);
+int ibtfo_batch_destroy(ibtfo_batch* b);
+/* Seconds since the batch was created (the clock of arrivals and results). */
+double ibtfo_batch_clock(const ibtfo_batch* b);
+/* Queues a request (parameters are copied); *ticket identifies it. Rejects
+ * requests that could never fit (IBTFO_E_NOMEM) or disagree with the batch's
+ * kernel selection (IBTFO_E_ARG). */
+int ibtfo_batch_submit(ibtfo_batch* b, const ibtfo_request* r, long long* ticket);
+/* One scheduling round (see above). Counts may be NULL. */
+int ibtfo_batch_progress(ibtfo_batch* b, int* active, int* queued, int* finished);
+/* Pops the ticket of a retired request: returns 1 and sets *ticket, or 0. */
+int ibtfo_batch_poll(ibtfo_batch* b, long long* ticket);
+/* Result of a retired request; U_OLD (M*N*4 doubles, AoS) only if it kept its state. */
+int ibtfo_batch_result(ibtfo_batch* b, long long ticket, ibtfo_result* res, double* U_OLD);
+/* Frees the host copy of a retired request's result. */
+int ibtfo_batch_release(ibtfo_batch* b, long long ticket);
+int ibtfo_batch_stats_get(ibtfo_batch* b, ibtfo_batch_stats* st);
+
 /* Last error message of this thread (static storage). */
 const char* ibtfo_last_error(void);
 

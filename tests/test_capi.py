@@ -75,3 +75,26 @@ def test_footprint_scales_with_grid():
     assert 3.5 < b / a < 4.5
     big = ib.lib().ibtfo_footprint(C.byref(ib.config(N_x=4096, N_y=2048)))
     assert 0.5e9 < big < 2e9
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """ctypes mirrors of the header's structs have the C compiler's sizes and offsets."""
+    import shutil
+    import subprocess
+    import pytest
+    from ibtfo_amd import serving as S
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "ibtfo.h"\nint main(void){printf("%zu %zu %zu %zu '
+                   '%zu %zu %zu %zu\\n", sizeof(ibtfo_config), sizeof(ibtfo_batch_config), sizeof(ibtfo_request), '
+                   'sizeof(ibtfo_result), sizeof(ibtfo_batch_stats), offsetof(ibtfo_request, arrival), '
+                   'offsetof(ibtfo_batch_config, hbm_budget), offsetof(ibtfo_result, t_arrival));return 0;}\n')
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    want = [C.sizeof(ib.Config), C.sizeof(S.BatchConfig), C.sizeof(S.Request), C.sizeof(S.Result),
+            C.sizeof(S.BatchStats), S.Request.arrival.offset, S.BatchConfig.hbm_budget.offset,
+            S.Result.t_arrival.offset]
+    assert got == want

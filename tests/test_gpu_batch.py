@@ -1,0 +1,146 @@
+"""Batch engine (continuous batching of independent solves): every result is
+the same alone and in the mix, bit for bit, and the same as the single-solve
+path; admission respects the HBM budget; bad requests are refused."""
+import numpy as np
+import pytest
+
+from parity_util import nan_equal_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _items(prec="FP64", n=5, steps=40, seed=7, time="SingleStep"):
+    from ibtfo_amd import serving as S
+    st = S.ellipse_stream(n=n, seed=seed, steps=steps)
+    # small grids for the test: scale each request to at most 256 x 128
+    for s in st:
+        f = max(1, max(s["N_x"] // 256, s["N_y"] // 128))
+        s["N_x"], s["N_y"] = max(64, s["N_x"] // f // 8 * 8), max(64, s["N_y"] // f // 8 * 8)
+        s["L"] = s["N_x"] / s["N_y"]
+        e = s["ellipse"]
+        s["ellipse"] = (0.3 * s["L"], 0.5, e[2], e[3], e[4])
+    return st, S.ellipse_items(st, precision=prec, keep_state=True, Time=time)
+
+
+def _bc(prec, time="SingleStep", **kw):
+    from ibtfo_amd import serving as S
+    args = dict(Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer", Boundary="IDWBaseGFM", Time=time,
+                Precision=prec, quirks=0, hbm_budget=1 << 30, max_slots=16, max_step_blocks=4096,
+                max_ghost_blocks=1024, max_ring_blocks=1024, rows=16, chunk=8)
+    args.update(kw)
+    return S.batch_config(**args)
+
+
+def _run_alone(item, prec, time="SingleStep"):
+    from ibtfo_amd import serving as S
+    b = S.Batch(_bc(prec, time))
+    t = b.submit(**item)
+    b.drain()
+    r, U = b.result(t)
+    b.close()
+    return r, U
+
+
+@pytest.mark.parametrize("prec", ["FP64", "mixed"])
+def test_alone_equals_mix_with_staggered_arrivals(prec):
+    from ibtfo_amd import serving as S
+    st, items = _items(prec)
+    alone = [_run_alone(it, prec) for it in items]
+    b = S.Batch(_bc(prec))
+    tickets = {}
+    # two requests first; the rest arrive while those are on the GPU
+    for i in (0, 1):
+        tickets[b.submit(**items[i])] = i
+    b.progress()
+    b.progress()
+    for i in range(2, len(items)):
+        tickets[b.submit(**items[i])] = i
+        b.progress()
+    order = b.drain()
+    assert b.stats().finished == len(items)
+    for t, i in tickets.items():
+        r, U = b.result(t)
+        ra, Ua = alone[i]
+        assert r.status == 0 and r.iter == items[i]["cfg"].max_iter
+        assert nan_equal_bits(U, Ua) == 0, i
+        assert r.dt == ra.dt and r.now_t == ra.now_t
+    b.close()
+
+
+def test_batch_equals_single_solve_path():
+    import ibtfo_amd as ib
+    st, items = _items("FP64", n=3)
+    for it in items:
+        r, U = _run_alone(it, "FP64")
+        s = ib.Solver(it["cfg"])
+        s.geometry(*it["levelset"])
+        s.initialize(*it["init"])
+        s.prepare(True)
+        s.step(it["cfg"].max_iter)
+        i, t, dt, rc = s.status()
+        assert i == r.iter and t == r.now_t and dt == r.dt
+        assert nan_equal_bits(s.state(), U) == 0
+        s.close()
+
+
+def test_rk2_in_batch_equals_single_solve_path():
+    import ibtfo_amd as ib
+    st, items = _items("FP64", n=2, time="RK2")
+    for it in items:
+        r, U = _run_alone(it, "FP64", time="RK2")
+        s = ib.Solver(it["cfg"])
+        s.geometry(*it["levelset"])
+        s.initialize(*it["init"])
+        s.prepare(True)
+        s.step(it["cfg"].max_iter)
+        assert nan_equal_bits(s.state(), U) == 0
+        s.close()
+
+
+def test_admission_by_hbm_budget():
+    """A budget that holds about two of the solves: all finish, never more than the
+    budget is resident, results equal the unconstrained run."""
+    import ctypes as C
+    import ibtfo_amd as ib
+    from ibtfo_amd import serving as S
+    st, items = _items("FP64", n=6, steps=24)
+    foot = [ib.lib().ibtfo_footprint(C.byref(it["cfg"])) for it in items]
+    budget = int(2.2 * max(foot))
+    b = S.Batch(_bc("FP64", hbm_budget=budget))
+    tickets = {b.submit(**it): i for i, it in enumerate(items)}
+    peak_active, peak_used = 0, 0
+    while True:
+        a, q, f = b.progress()
+        s = b.stats()
+        peak_active = max(peak_active, a)
+        peak_used = max(peak_used, s.arena_used)
+        assert s.arena_used <= s.arena_capacity
+        b.poll()
+        if a == 0 and q == 0:
+            break
+    assert 1 <= peak_active < len(items)
+    big = S.Batch(_bc("FP64"))
+    for t, i in tickets.items():
+        r, U = b.result(t)
+        t2 = big.submit(**items[i])
+        big.drain()
+        r2, U2 = big.result(t2)
+        assert nan_equal_bits(U, U2) == 0
+    b.close()
+    big.close()
+
+
+def test_refused_requests():
+    import ibtfo_amd as ib
+    from ibtfo_amd import serving as S
+    st, items = _items("FP64", n=1)
+    b = S.Batch(_bc("FP64", hbm_budget=1 << 24))
+    huge = ib.config(N_x=4096, N_y=2048, Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer", Boundary="IDWBaseGFM",
+                     quirks=0, Edge="inflow")
+    with pytest.raises(ib.IbtfoError):
+        b.submit(cfg=huge, levelset=("none", ()), init=("uniform", (1.4, 3.0, 0.0, 1.0)))
+    wrong = items[0]["cfg"]
+    wrong.limiter = ib.LIMITER["minmod"]
+    with pytest.raises(ib.IbtfoError):
+        b.submit(cfg=wrong, levelset=items[0]["levelset"], init=items[0]["init"])
+    b.close()
