@@ -106,6 +106,121 @@ def cpu_leg(args):
                     nx, ny, r["steps"], threads))
 
 
+def _plain_solve(it):
+    """One request through the single-solve path (create, geometry, initial
+    state, boundary + dt, graph-replayed steps, final status)."""
+    import ibtfo_amd as ib
+    s = ib.Solver(it["cfg"])
+    s.geometry(*it["levelset"])
+    s.initialize(*it["init"])
+    s.prepare(True)
+    n, chunk, done = it["cfg"].max_iter, 50, 0
+    while done < n:  # stops early when the solve reaches its Psy_time
+        m = min(chunk, n - done)
+        s.step(m, graph_chunk=chunk if m == chunk else 0)
+        i = s.status()[0]
+        if i < done + m:
+            break
+        done = i
+    st = s.status()
+    U = s.state() if it.get("keep_state") else None
+    types = s.xycoord()[5::6] if it.get("keep_state") else None
+    s.close()
+    return st, U, types
+
+
+def serving_leg(args):
+    """The request stream: `args.requests` seeded immersed-ellipse solves (grids
+    64x64 .. 1024x512, Mach 1.5-4, 500 steps, IDW ghost fluid). Measures the
+    plain per-solve FP64 path on a sample, the batch engine (mixed precision)
+    saturated with the whole stream, the batch's latency under Poisson arrivals
+    at 0.8x the plain path's rate, and the L1 density gate on a sample."""
+    import time as _t
+    import torch  # noqa: F401  (device initialisation outside the timed regions)
+    from ibtfo_amd import serving as S
+    stream = S.ellipse_stream(n=args.requests, seed=1234, steps=args.serve_steps)
+    items64 = S.ellipse_items(stream, precision="FP64")
+    items32 = S.ellipse_items(stream, precision="mixed")
+    cells = np.array([s["N_x"] * s["N_y"] for s in stream], dtype=float)
+    # plain per-solve FP64 path on the first `plain_sample` requests
+    k = min(args.plain_sample, len(items64))
+    _plain_solve(items64[0])  # warm-up (code objects, allocator)
+    t0 = _t.perf_counter()
+    for it in items64[:k]:
+        _plain_solve(it)
+    plain_s = _t.perf_counter() - t0
+    plain_rate = k / plain_s
+    plain_cu = cells[:k].sum() * args.serve_steps / plain_s
+    bc = S.batch_config(Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer", Boundary="IDWBaseGFM",
+                        Precision="mixed", quirks=0, hbm_budget=args.serve_budget << 30, max_slots=args.serve_slots,
+                        max_step_blocks=args.serve_step_blocks, max_ghost_blocks=4096, max_ring_blocks=8192,
+                        rows=args.serve_rows, chunk=args.serve_chunk)
+    # saturation: every request queued at t = 0
+    b = S.Batch(bc)
+    b.submit(**items32[0])
+    b.drain()  # warm-up: capture, code objects
+    b.close()
+    b = S.Batch(bc)
+    sat = S.serve(b, items32)
+    st = b.stats()
+    b.close()
+    sat_rate = len(items32) / sat["wall"]
+    sat_cu = cells.sum() * args.serve_steps / sat["wall"]
+    # Poisson arrivals at 0.8x the plain path's solves/s
+    lam = 0.8 * plain_rate
+    rng = np.random.default_rng(4321)
+    arr = np.cumsum(rng.exponential(1.0 / lam, size=len(items32)))
+    b = S.Batch(bc)
+    pois = S.serve(b, items32, arrivals=arr)
+    b.close()
+    lat = pois["latency"] * 1e3
+    # L1 density gate on a sample: mixed (batched) and FP64 (plain) against a 2x-refined FP64 solve
+    gate = []
+    for i in range(min(args.gate_sample, len(stream))):
+        sm = stream[i]
+        it64 = dict(items64[i], keep_state=True)
+        st64, U64, types = _plain_solve(it64)
+        b = S.Batch(bc)
+        t = b.submit(**dict(items32[i], keep_state=True))
+        b.drain()
+        r32, U32 = b.result(t)
+        b.close()
+        fine = dict(sm, N_x=2 * sm["N_x"], N_y=2 * sm["N_y"], steps=10 ** 9)
+        itf = S.ellipse_items([fine], precision="FP64", keep_state=True)[0]
+        itf["cfg"].Psy_time = st64[1]
+        stf, Uf, tf = _plain_solve(itf)
+        nx, ny = sm["N_x"], sm["N_y"]
+        M, N = nx + 4, ny + 4
+        rf = Uf[0::4].reshape(2 * ny + 4, 2 * nx + 4)[2:-2, 2:-2]
+        tff = tf.reshape(2 * ny + 4, 2 * nx + 4)[2:-2, 2:-2]
+        ref = 0.25 * (rf[0::2, 0::2] + rf[1::2, 0::2] + rf[0::2, 1::2] + rf[1::2, 1::2])
+        okf = (tff[0::2, 0::2] == 0) & (tff[1::2, 0::2] == 0) & (tff[0::2, 1::2] == 0) & (tff[1::2, 1::2] == 0)
+        mask = (types.reshape(N, M)[2:-2, 2:-2] == 0) & okf
+        r64 = U64[0::4].reshape(N, M)[2:-2, 2:-2]
+        r32 = U32[0::4].reshape(N, M)[2:-2, 2:-2]
+        e64 = float(np.mean(np.abs(r64[mask] - ref[mask])))
+        e32 = float(np.mean(np.abs(r32[mask] - ref[mask])))
+        gate.append(dict(grid=[nx, ny], l1_fp64=e64, l1_mixed=e32, ratio=e32 / e64))
+    return {
+        "requests": len(items32), "steps_per_request": args.serve_steps,
+        "workload": "immersed ellipse, Mach 1.5-4 inflow, grids 64x64..1024x512 (seed 1234), MUSCL + VanLeer "
+                    "+ HLL, IDW ghost fluid, batch in mixed precision",
+        "plain_fp64": {"solves_per_s": plain_rate, "cell_updates_per_s": plain_cu, "sample": k},
+        "batched_mixed_saturated": {"solves_per_s": sat_rate, "cell_updates_per_s": sat_cu,
+                                    "speedup_vs_plain_fp64": sat_rate / plain_rate, "wall_s": sat["wall"],
+                                    "p50_ms": float(np.percentile(sat["latency"] * 1e3, 50)),
+                                    "p99_ms": float(np.percentile(sat["latency"] * 1e3, 99)), "chunks": st.chunks},
+        "batched_mixed_poisson": {"rate_per_s": lam, "p50_ms": float(np.percentile(lat, 50)),
+                                  "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+                                  "solves_per_s": len(items32) / pois["wall"], "wall_s": pois["wall"]},
+        "l1_gate": {"rule": "L1(rho_mixed - rho_ref) <= 1.1 L1(rho_fp64 - rho_ref), rho_ref = 2x-refined FP64 "
+                            "solve averaged to the grid", "samples": gate,
+                    "passed": all(g["ratio"] <= 1.1 for g in gate)},
+        "batch": {"slots": args.serve_slots, "budget_gib": args.serve_budget, "rows": args.serve_rows,
+                  "chunk": args.serve_chunk},
+    }
+
+
 COPY_BYTES = 1 << 28  # calibration copy: 256 MiB read + 256 MiB written
 
 
@@ -208,6 +323,16 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-stream leg")
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--serve-steps", type=int, default=500)
+    ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--gate-sample", type=int, default=4)
+    ap.add_argument("--serve-budget", type=int, default=64, help="batch HBM budget in GiB")
+    ap.add_argument("--serve-slots", type=int, default=256)
+    ap.add_argument("--serve-step-blocks", type=int, default=65536)
+    ap.add_argument("--serve-rows", type=int, default=16)
+    ap.add_argument("--serve-chunk", type=int, default=50)
     ap.add_argument("--pmc-probe", action="store_true", help="run the hardware-counter workload and exit")
     ap.add_argument("--pmc-fetch", help="rocprofv3 output folder of the FETCH_SIZE pass (summary mode)")
     ap.add_argument("--pmc-write", help="rocprofv3 output folder of the WRITE_SIZE pass (summary mode)")
@@ -234,6 +359,7 @@ def main():
                  "kernel_ms": m["kernel_ms"], "speedup_vs_fp64": m["value"] / g["value"],
                  "l1_rho_vs_fp64": l1, "l1_rho_vs_fp64_rel": l1 / scale, "steps_compared": m["iter"],
                  "tile_rows": m["rows"]}
+    serving = None if args.no_serve else serving_leg(args)
     cpu = None
     if not args.no_cpu:
         c = cpu_leg(args)
@@ -264,6 +390,7 @@ def main():
                      "bytes_per_cell": BYTES_PER_CELL},
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
+        "serving": serving,
     }
     print(json.dumps(line))
     return 0
