@@ -61,7 +61,8 @@ def lib():
                      "ibtfo_initialize", "ibtfo_upload_state", "ibtfo_download_state", "ibtfo_download_geometry",
                      "ibtfo_download_scheme_index", "ibtfo_sch_width", "ibtfo_num_ghosts", "ibtfo_dims",
                      "ibtfo_time_steps", "ibtfo_device_state", "ibtfo_set_stream", "ibtfo_set_rows",
-                     "ibtfo_tile", "ibtfo_autotune"]:
+                     "ibtfo_tile", "ibtfo_autotune", "ibtfo_geometry_mask", "ibtfo_geometry_png",
+                     "ibtfo_png_decode"]:
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -76,7 +77,7 @@ def ptr(a):
 
 def _check(rc, what):
     if rc != OK:
-        raise IbtfoError(rc, what, lib().ibtfo_last_error().decode())
+        raise IbtfoError(rc, what, lib().ibtfo_last_error().decode(errors="replace"))
 
 
 def config(**kw):
@@ -97,6 +98,16 @@ def config(**kw):
         else:
             setattr(c, k, v)
     return c
+
+
+def png_decode(path):
+    """(H, W, 4) uint8 RGBA of a PNG file as the mask level set sees it."""
+    w, h = C.c_int(), C.c_int()
+    _check(lib().ibtfo_png_decode(os.fsencode(path), None, C.c_longlong(0), C.byref(w), C.byref(h)), "ibtfo_png_decode")
+    out = np.zeros((h.value, w.value, 4), dtype=np.uint8)
+    _check(lib().ibtfo_png_decode(os.fsencode(path), out.ctypes.data_as(C.c_void_p), C.c_longlong(out.size),
+                                  C.byref(w), C.byref(h)), "ibtfo_png_decode")
+    return out
 
 
 def config_from_inp(path):
@@ -126,6 +137,16 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+    def geometry_mask(self, rgba):
+        """Mask level set from an (H, W, 4) uint8 image, top row first."""
+        img = np.ascontiguousarray(rgba, dtype=np.uint8)
+        assert img.ndim == 3 and img.shape[2] == 4
+        _check(lib().ibtfo_geometry_mask(self.h, img.ctypes.data_as(C.c_void_p), img.shape[1], img.shape[0]),
+               "ibtfo_geometry_mask")
+
+    def geometry_png(self, path):
+        _check(lib().ibtfo_geometry_png(self.h, os.fsencode(path)), "ibtfo_geometry_png")
 
     def geometry(self, kind="none", prm=(), phi=None):
         k = LEVELSET[kind] if isinstance(kind, str) else int(kind)

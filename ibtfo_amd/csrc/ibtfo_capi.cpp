@@ -226,6 +226,42 @@ extern "C" int ibtfo_sch_width(const ibtfo_ctx* x) {
 extern "C" int ibtfo_num_ghosts(const ibtfo_ctx* x) { return x ? x->nghost : -1; }
 
 // --------------------------------------------------------------- geometry
+// Runs the geometry kernels for a prepared GeomArgs (hook constants and inputs
+// set) and builds the ghost list; frees `dphi` (a device input of the hook).
+static int geometry_finish(ibtfo_ctx* x, GeomArgs& g, double* dphi) {
+    hipError_t ge = geom_run(g, x->stream);
+    if (ge != hipSuccess) { dfree(x, dphi); HIPCHK(ge); }
+    // ghost list (upper bound: all interior cells of type 2; count first)
+    dfree(x, x->ghosts); x->ghosts = nullptr;
+    dfree(x, x->gvals); x->gvals = nullptr;
+    x->nghost = 0;
+    if (x->cfg.boundary != IBTFO_REGULAR) {
+        unsigned int zero = 0, cnt = 0;
+        HIPCHK(hipMemcpyAsync(x->dcount, &zero, sizeof zero, hipMemcpyHostToDevice, x->stream));
+        g.count = x->dcount; g.cap = 0; g.ghosts = nullptr;
+        HIPCHK(geom_ghosts(g, x->stream));
+        HIPCHK(hipMemcpyAsync(&cnt, x->dcount, sizeof cnt, hipMemcpyDeviceToHost, x->stream));
+        HIPCHK(hipStreamSynchronize(x->stream));
+        if (cnt > 0) {
+            if (dalloc(x, &x->ghosts, cnt * sizeof(Ghost)) != hipSuccess ||
+                dalloc(x, &x->gvals, cnt * 4 * sizeof(double)) != hipSuccess) {
+                dfree(x, dphi);
+                set_error("geometry: out of device memory for %u ghost records", cnt);
+                return IBTFO_E_NOMEM;
+            }
+            HIPCHK(hipMemcpyAsync(x->dcount, &zero, sizeof zero, hipMemcpyHostToDevice, x->stream));
+            g.ghosts = x->ghosts; g.cap = (int)cnt;
+            HIPCHK(geom_ghosts(g, x->stream));
+        }
+        x->nghost = (int)cnt;
+    }
+    HIPCHK(hipStreamSynchronize(x->stream));
+    dfree(x, dphi);
+    x->have_geometry = 1;
+    x->prepared = 0;
+    return IBTFO_OK;
+}
+
 extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int nprm, const double* phi_host) {
     if (!x) return IBTFO_E_ARG;
     const Layout& L = x->L;
@@ -236,6 +272,7 @@ extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int npr
     g.kind = kind;
     g.phi = x->phi; g.nx = x->gnx; g.ny = x->gny; g.type = x->type;
     g.bc = x->cfg.boundary;
+    g.unit_normals = x->cfg.quirks == 0;
     double* dphi = nullptr;
     auto need = [&](int n) {
         if (nprm < n || !prm) { set_error("level-set hook %d needs %d parameters", kind, n); return false; }
@@ -288,36 +325,79 @@ extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int npr
         }
         default: set_error("unknown level-set kind %d", kind); return IBTFO_E_ARG;
     }
-    HIPCHK(geom_run(g, x->stream));
-    // ghost list (upper bound: all interior cells of type 2; count first)
-    dfree(x, x->ghosts); x->ghosts = nullptr;
-    dfree(x, x->gvals); x->gvals = nullptr;
-    x->nghost = 0;
-    if (x->cfg.boundary != IBTFO_REGULAR) {
-        unsigned int zero = 0, cnt = 0;
-        HIPCHK(hipMemcpyAsync(x->dcount, &zero, sizeof zero, hipMemcpyHostToDevice, x->stream));
-        g.count = x->dcount; g.cap = 0; g.ghosts = nullptr;
-        HIPCHK(geom_ghosts(g, x->stream));
-        HIPCHK(hipMemcpyAsync(&cnt, x->dcount, sizeof cnt, hipMemcpyDeviceToHost, x->stream));
-        HIPCHK(hipStreamSynchronize(x->stream));
-        if (cnt > 0) {
-            if (dalloc(x, &x->ghosts, cnt * sizeof(Ghost)) != hipSuccess ||
-                dalloc(x, &x->gvals, cnt * 4 * sizeof(double)) != hipSuccess) {
-                dfree(x, dphi);
-                set_error("geometry: out of device memory for %u ghost records", cnt);
-                return IBTFO_E_NOMEM;
-            }
-            HIPCHK(hipMemcpyAsync(x->dcount, &zero, sizeof zero, hipMemcpyHostToDevice, x->stream));
-            g.ghosts = x->ghosts; g.cap = (int)cnt;
-            HIPCHK(geom_ghosts(g, x->stream));
-        }
-        x->nghost = (int)cnt;
+    return geometry_finish(x, g, dphi);
+}
+
+namespace ibtfo {
+int png_read_file(const char* path, std::vector<unsigned char>& rgba, int* width, int* height);
+}
+
+// Test/PNGGeo/Level_Set.cpp:11-364 with the image already decoded: phi from the
+// mask on the device (mask_run), then normals, classification and the ghost
+// list as for any level set.
+extern "C" int ibtfo_geometry_mask(ibtfo_ctx* x, const unsigned char* rgba, int width, int height) {
+    if (!x || !rgba || width < 3 || height < 3) { set_error("ibtfo_geometry_mask: bad image"); return IBTFO_E_ARG; }
+    const Layout& L = x->L;
+    const ibtfo_config& c = x->cfg;
+    MaskArgs m;
+    memset(&m, 0, sizeof m);
+    m.W = width; m.H = height; m.M = L.M; m.N = L.N;
+    m.Li = x->Li; m.Hi = x->Hi; m.dxc = x->dxc;
+    const double Psy_L = c.hi_x - c.lo_x, Psy_H = c.hi_y - c.lo_y;
+    m.exo = -c.num_ghost_cell * (Psy_L / c.N_x);  // Level_Set.cpp:82-95
+    m.eyo = -c.num_ghost_cell * (Psy_H / c.N_y);
+    m.dxp = (Psy_L / c.N_x) * (c.N_x + 2 * c.num_ghost_cell) / width;
+    m.dyp = (Psy_H / c.N_y) * (c.N_y + 2 * c.num_ghost_cell) / height;
+    const size_t WH = (size_t)width * height, WN = (size_t)width * L.N, NW1 = (size_t)L.N * (width + 1);
+    unsigned char *drgba = nullptr, *dedge = nullptr;
+    double *dv = nullptr, *dg1 = nullptr, *dz = nullptr, *dphi = nullptr;
+    int *dr1 = nullptr, *dr2 = nullptr, *dvb = nullptr;
+    auto release = [&]() {
+        for (void* p : {(void*)drgba, (void*)dedge, (void*)dv, (void*)dg1, (void*)dz, (void*)dr1, (void*)dr2,
+                        (void*)dvb})
+            dfree(x, p);
+    };
+    if (dalloc(x, &drgba, WH * 4) != hipSuccess || dalloc(x, &dedge, WH) != hipSuccess ||
+        dalloc(x, &dv, WH * sizeof(double)) != hipSuccess || dalloc(x, &dg1, WN * sizeof(double)) != hipSuccess ||
+        dalloc(x, &dr1, WN * sizeof(int)) != hipSuccess || dalloc(x, &dr2, WN * sizeof(int)) != hipSuccess ||
+        dalloc(x, &dvb, NW1 * sizeof(int)) != hipSuccess ||
+        dalloc(x, &dz, NW1 * sizeof(double)) != hipSuccess ||
+        dalloc(x, &dphi, (size_t)L.M * L.N * sizeof(double)) != hipSuccess) {
+        release();
+        dfree(x, dphi);
+        set_error("ibtfo_geometry_mask: out of device memory");
+        return IBTFO_E_NOMEM;
     }
-    HIPCHK(hipStreamSynchronize(x->stream));
-    dfree(x, dphi);
-    x->have_geometry = 1;
-    x->prepared = 0;
-    return IBTFO_OK;
+    m.rgba = drgba; m.v = dv; m.edge = dedge; m.g1 = dg1; m.r1 = dr1; m.r2 = dr2; m.vbuf = dvb; m.zbuf = dz;
+    m.phi = dphi;
+    hipError_t e = scopy(x, drgba, rgba, WH * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = mask_run(m, x->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+    release();
+    if (e != hipSuccess) {
+        dfree(x, dphi);
+        set_error("ibtfo_geometry_mask: %s", hipGetErrorString(e));
+        return IBTFO_E_HIP;
+    }
+    GeomArgs g;
+    memset(&g, 0, sizeof g);
+    g.L = L;
+    g.Li = x->Li; g.Hi = x->Hi; g.dx = x->dxc;
+    g.kind = IBTFO_LS_FIELD;
+    g.phi_in = dphi;
+    g.phi = x->phi; g.nx = x->gnx; g.ny = x->gny; g.type = x->type;
+    g.bc = c.boundary;
+    g.unit_normals = c.quirks == 0;
+    return geometry_finish(x, g, dphi);
+}
+
+extern "C" int ibtfo_geometry_png(ibtfo_ctx* x, const char* path) {
+    if (!x || !path) return IBTFO_E_ARG;
+    std::vector<unsigned char> img;
+    int w = 0, h = 0;
+    int rc = png_read_file(path, img, &w, &h);
+    if (rc) return rc;
+    return ibtfo_geometry_mask(x, img.data(), w, h);
 }
 
 extern "C" int ibtfo_download_geometry(ibtfo_ctx* x, double* XY) {

@@ -18,12 +18,31 @@ struct GeomArgs {
     double *phi, *nx, *ny;  // planes
     uint8_t* type;
     int bc;
+    int unit_normals;       // corrected mode: normals scaled to unit length
     Ghost* ghosts;
     unsigned int* count;
     int cap;
 };
 
+// Mask level set (Test/PNGGeo/Level_Set.cpp:31-149): phi per cell from an RGBA image
+struct MaskArgs {
+    const unsigned char* rgba;  // W x H x 4, top row first
+    int W, H;
+    double* v;                  // W x H signs, bottom row first
+    unsigned char* edge;        // W x H edge flags, bottom row first
+    int M, N;                   // cells with ghosts
+    double Li, Hi, dxc;         // cell centres (Psy_coord)
+    double exo, eyo, dxp, dyp;  // edge pixel x: exo + dxp * x, y: eyo + dyp * y
+    double* g1;                 // W x N: squared y distance to the column's nearest edge
+    int* r1;                    // W x N: that edge's pixel row (-1: none)
+    int* r2;                    // W x N: the column's other candidate row (-1: none)
+    int* vbuf;                  // N x (W+1) envelope scratch
+    double* zbuf;               // N x (W+1)
+    double* phi;                // M x N, reference cell order
+};
+
 hipError_t geom_run(GeomArgs& g, hipStream_t s);
+hipError_t mask_run(MaskArgs& m, hipStream_t s);
 hipError_t geom_ghosts(GeomArgs& g, hipStream_t s);
 
 }  // namespace ibtfo

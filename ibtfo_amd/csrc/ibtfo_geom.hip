@@ -92,6 +92,10 @@ __global__ void k_geom_classify(GeomArgs g) {
         nxv = (g.phi[L.at(i + 1, j)] - g.phi[L.at(i - 1, j)]) / ((xc(g, i + 1) - xc(g, i - 1)));
         nyv = (g.phi[L.at(i, j + 1)] - g.phi[L.at(i, j - 1)]) / ((yc(g, j + 1) - yc(g, j - 1)));
     }
+    if (g.unit_normals) {  // corrected mode: unit normals (no reference counterpart)
+        const double r = ib_sqrt(nxv * nxv + nyv * nyv);
+        if (r > 0) { nxv = nxv / r; nyv = nyv / r; }
+    }
     g.nx[o] = nxv;
     g.ny[o] = nyv;
     const int ng = L.ng;
@@ -233,6 +237,145 @@ hipError_t geom_run(GeomArgs& g, hipStream_t s) {
 hipError_t geom_ghosts(GeomArgs& g, hipStream_t s) {
     dim3 blk(128), grd((g.L.M + 127) / 128, g.L.N);
     hipLaunchKernelGGL(k_geom_ghosts, grd, blk, 0, s, g);
+    return hipGetLastError();
+}
+
+
+
+// ------------------------------------------------------------ mask level set
+// Test/PNGGeo/Level_Set.cpp:31-149 on the device. The reference computes, for
+// every cell, the distance to every edge pixel (O(cells x edges)); here the
+// nearest edge pixel is found by a separable exact search (per pixel column the
+// nearest edge row to each cell row, then per cell row the lower envelope of the
+// parabolas (x_q - x_e)^2 + g(x_e) over pixel columns, Felzenszwalb and
+// Huttenlocher, "Distance Transforms of Sampled Functions", 2012), and the
+// distance to that pixel is evaluated with the reference's expression.
+
+// Level_Set.cpp:36-48: +1 where R + G + B > 0, else -1; image rows flipped so that
+// row 0 is the bottom of the domain
+__global__ void k_mask_sign(MaskArgs m) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= m.W) return;
+    const unsigned char* px = m.rgba + ((long long)y * m.W + x) * 4;
+    const double s = (double)px[0] + (double)px[1] + (double)px[2];
+    m.v[(long long)(m.H - y - 1) * m.W + x] = (s > 0) * 2 - 1;
+}
+
+// Level_Set.cpp:51-76: edge pixel where max(|v - v_left|, |v - v_below|) > 0.1,
+// pixels 1..W-2 x 1..H-2 only
+__global__ void k_mask_edge(MaskArgs m) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= m.W) return;
+    unsigned char e = 0;
+    if (x >= 1 && x < m.W - 1 && y >= 1 && y < m.H - 1) {
+        const double c = m.v[(long long)y * m.W + x];
+        const double dx = c - m.v[(long long)y * m.W + x - 1];
+        const double dy = c - m.v[(long long)(y - 1) * m.W + x];
+        const double g = (ib_fabs(dx) > ib_fabs(dy)) ? ib_fabs(dx) : ib_fabs(dy);
+        e = g > 0.1;
+    }
+    m.edge[(long long)y * m.W + x] = e;
+}
+
+__device__ __forceinline__ double edge_x(const MaskArgs& m, int x) { return m.exo + m.dxp * x; }
+__device__ __forceinline__ double edge_y(const MaskArgs& m, int y) { return m.eyo + m.dyp * y; }
+
+// Per pixel column: for every cell row j, the nearest edge row of the column to
+// the cell-centre height y_j (two-pointer sweep; both sequences increase).
+__global__ void k_edt_cols(MaskArgs m) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= m.W) return;
+    auto next_edge = [&](int from) {
+        for (int y = from; y < m.H; y++)
+            if (m.edge[(long long)y * m.W + x]) return y;
+        return -1;
+    };
+    int prv = -1, nxt = next_edge(0);
+    for (int j = 0; j < m.N; j++) {
+        const double yq = m.Hi + j * m.dxc;
+        while (nxt >= 0 && edge_y(m, nxt) <= yq) {
+            prv = nxt;
+            nxt = next_edge(nxt + 1);
+        }
+        double best = HUGE_VAL;
+        int arg = -1;
+        if (prv >= 0) {
+            double d = edge_y(m, prv) - yq;
+            best = d * d;
+            arg = prv;
+        }
+        if (nxt >= 0) {
+            double d = edge_y(m, nxt) - yq;
+            if (d * d < best) { best = d * d; arg = nxt; }
+        }
+        m.g1[(long long)x * m.N + j] = best;
+        m.r1[(long long)x * m.N + j] = arg;
+        m.r2[(long long)x * m.N + j] = arg == prv ? nxt : prv;  // the other neighbour (near ties)
+    }
+}
+
+// Per cell row j: lower envelope of f_x(q) = (q - x_e(x))^2 + g1(x, j) over the
+// pixel columns that hold an edge, evaluated at the cell centres x_i; then
+// phi(i, j) = sign * min(100, distance to the chosen edge pixel) with the
+// sign sampled at pixel (int((i+0.5) W / M), int((j+0.5) H / N)).
+__global__ void k_edt_rows(MaskArgs m) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m.N) return;
+    int* v = m.vbuf + (long long)j * (m.W + 1);
+    double* z = m.zbuf + (long long)j * (m.W + 1);
+    int k = -1;
+    for (int x = 0; x < m.W; x++) {
+        const double f = m.g1[(long long)x * m.N + j];
+        if (!(f < HUGE_VAL)) continue;
+        const double p = edge_x(m, x);
+        double s = -HUGE_VAL;
+        while (k >= 0) {
+            const int u = v[k];
+            const double pu = edge_x(m, u), fu = m.g1[(long long)u * m.N + j];
+            s = ((f + p * p) - (fu + pu * pu)) / (2.0 * (p - pu));
+            if (s <= z[k]) k--;
+            else break;
+        }
+        k++;
+        v[k] = x;
+        z[k] = k == 0 ? -HUGE_VAL : s;
+    }
+    const double Y = m.Hi + j * m.dxc;
+    int q = 0;
+    for (int i = 0; i < m.M; i++) {
+        const double X = m.Li + i * m.dxc;
+        double dmin = 100.0;  // Level_Set.cpp:131 starts every search at 100
+        if (k >= 0) {
+            while (q < k && z[q + 1] < X) q++;
+            // the envelope's parabola at X and its neighbours, each with the column's
+            // two nearest edge rows: points whose distances tie with the nearest in
+            // exact arithmetic but round differently are all evaluated with the
+            // reference's expression, whose minimum it keeps
+            for (int c = (q > 0 ? q - 1 : q); c <= (q < k ? q + 1 : q); c++) {
+                const int ex = v[c];
+                const double ax = edge_x(m, ex);
+                for (int rr = 0; rr < 2; rr++) {
+                    const int ey = rr == 0 ? m.r1[(long long)ex * m.N + j] : m.r2[(long long)ex * m.N + j];
+                    if (ey < 0) continue;
+                    const double ay = edge_y(m, ey);
+                    const double d = ib_sqrt((ax - X) * (ax - X) + (ay - Y) * (ay - Y));
+                    if (d < dmin) dmin = d;
+                }
+            }
+        }
+        const int xs = (int)((i + 0.5) * (m.W) / (m.M));
+        const int ys = (int)((j + 0.5) * (m.H) / (m.N));
+        const int pn = m.v[(long long)ys * m.W + xs] >= 0 ? 1 : -1;
+        m.phi[(long long)j * m.M + i] = pn * dmin;
+    }
+}
+
+hipError_t mask_run(MaskArgs& m, hipStream_t s) {
+    dim3 pg((m.W + 255) / 256, m.H);
+    hipLaunchKernelGGL(k_mask_sign, pg, dim3(256), 0, s, m);
+    hipLaunchKernelGGL(k_mask_edge, pg, dim3(256), 0, s, m);
+    hipLaunchKernelGGL(k_edt_cols, dim3((m.W + 63) / 64), dim3(64), 0, s, m);
+    hipLaunchKernelGGL(k_edt_rows, dim3((m.N + 63) / 64), dim3(64), 0, s, m);
     return hipGetLastError();
 }
 

@@ -633,6 +633,9 @@ __device__ __forceinline__ void fill_one(const FillArgs& a, const int gi) {
         double r[4], u[4], v[4], p[4];
         const long long dd[4] = {d1, d1 + P, d1 + P + 1, d1 + 1};
         for (int q = 0; q < 4; q++) prim(dd[q], (rbits >> q) & 1, r[q], u[q], v[q], p[q]);
+        if (!a.quirks)  // corrected mode: a donor at the point itself (1/0 weight) is the value
+            for (int q = 0; q < 4; q++)
+                if (__builtin_isinf(w[q])) { R = r[q]; Uu = u[q]; V = v[q]; Pp = p[q]; return; }
         double s = w[0] + w[1] + w[2] + w[3];
         R = (w[0] * r[0] + w[1] * r[1] + w[2] * r[2] + w[3] * r[3]) / s;
         Uu = (w[0] * u[0] + w[1] * u[1] + w[2] * u[2] + w[3] * u[3]) / s;

@@ -117,6 +117,18 @@ long long ibtfo_footprint(const ibtfo_config* cfg);
  * + Scheme_Index (mirror points, donor indices, weights) + compacted ghost list.
  * phi_host: IBTFO_LS_FIELD only (M*N doubles, reference cell order), else NULL. */
 int ibtfo_geometry(ibtfo_ctx* ctx, int ls_kind, const double* prm, int nprm, const double* phi_host);
+/* Mask level set, Test/PNGGeo/Level_Set.cpp:11-364 (the reference reads Demo.png
+ * from the working directory with libpng, Src/Utils/PngProcess.cpp:4-72): phi =
+ * sign * distance to the nearest edge pixel of the mask (+ where R+G+B > 0),
+ * computed on the device, then normals, classification and ghost records as
+ * ibtfo_geometry. rgba: width x height x 4 bytes, top row first. */
+int ibtfo_geometry_mask(ibtfo_ctx* ctx, const unsigned char* rgba, int width, int height);
+/* The same from a PNG file (8-bit RGBA after the reference reader's transforms:
+ * 16-bit cut to the high byte, palette and gray expanded, filler alpha). */
+int ibtfo_geometry_png(ibtfo_ctx* ctx, const char* path);
+/* Decodes a PNG file into rgba (cap bytes; NULL to query the size only). */
+int ibtfo_png_decode(const char* path, unsigned char* rgba, long long cap, int* width, int* height);
+
 /* Initialize hook; evaluated on the host, uploaded as both U_OLD and U_NEW. */
 int ibtfo_initialize(ibtfo_ctx* ctx, int init_kind, const double* prm, int nprm);
 /* Boundary once (the drivers' call before the loop for GFM variants,

@@ -719,6 +719,21 @@ void or_classify(int nx, int ny, int ng, double* XY) {
             if (i < ng || i >= nx + ng || j < ng || j >= ny + ng) at(i, j, 5) = 2;
 }
 
+// Corrected mode only (no reference counterpart): unit normals. The reference's
+// normals are the unnormalised central differences of phi, which the mirror
+// points and the velocity reflection use as if they were unit vectors; for a
+// level set that is not a signed distance (e.g. the mask's distance to edge
+// pixels) |n|^2 reaches 2 and the reflection then amplifies the ghost velocity
+// every step.
+void or_unit_normals(int nx, int ny, int ng, double* XY) {
+    Grid g(nx, ny, ng);
+    for (size_t q = 0; q < (size_t)g.M * g.N; q++) {
+        double a = XY[q * 6 + 3], b = XY[q * 6 + 4];
+        double r = std::sqrt(a * a + b * b);
+        if (r > 0) { XY[q * 6 + 3] = a / r; XY[q * 6 + 4] = b / r; }
+    }
+}
+
 }  // extern "C"
 
 namespace {
@@ -854,6 +869,9 @@ void or_boundary(int bc, int nx, int ny, int ng, double gam, double* U_OLD, doub
                     double r[4], u[4], v[4], p[4], t[4];
                     donors(kp, r, u, v, p, t);
                     double a1 = s[ka], a2 = s[ka + 1], a3 = s[ka + 2], a4 = s[ka + 3];
+                    if (!quirks)  // corrected mode: a donor at the point itself (1/0 weight) is the value
+                        for (int q = 0; q < 4; q++)
+                            if (std::isinf(s[ka + q])) { R = r[q]; Uu = u[q]; V = v[q]; P = p[q]; return; }
                     R = (a1 * r[0] + a2 * r[1] + a3 * r[2] + a4 * r[3]) / (a1 + a2 + a3 + a4);
                     Uu = (a1 * u[0] + a2 * u[1] + a3 * u[2] + a4 * u[3]) / (a1 + a2 + a3 + a4);
                     V = (a1 * v[0] + a2 * v[1] + a3 * v[2] + a4 * v[3]) / (a1 + a2 + a3 + a4);
@@ -1043,6 +1061,58 @@ void or_initialize(int kind, const double* prm, int nx, int ny, int ng, double g
                 prim_to_cons(gam, rho, prm[4] - f * dyv, prm[5] + f * dxv, pr, U);
             }
             for (int k = 0; k < 4; k++) U_NEW[g.c(i, j) * 4 + k] = U[k];
+        }
+}
+
+// Test/PNGGeo/Level_Set.cpp:31-149: phi per cell (reference cell order j*M+i)
+// from an RGBA image (top row first), by the reference's brute-force search over
+// all edge pixels. The edge-magnitude slot is set for pixels 1..W-2 x 1..H-2
+// only; the reference leaves the border pixels' slot uninitialised and scans
+// all pixels, which is taken here as 0 (no edge).
+void or_mask_phi(const unsigned char* rgba, int W, int H, int nx, int ny, int ng, double lo_x, double lo_y,
+                 double hi_x, double hi_y, const double* XY, double* phi) {
+    const double Psy_L = hi_x - lo_x, Psy_H = hi_y - lo_y;
+    std::vector<double> v((size_t)W * H), g((size_t)W * H, 0.0);
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            const unsigned char* px = rgba + ((size_t)y * W + x) * 4;
+            double R = px[0], G = px[1], B = px[2];
+            v[(size_t)(H - y - 1) * W + x] = ((R + G + B) > 0) * 2 - 1;
+        }
+    int edg_num = 0;
+    for (int y = 1; y < H - 1; y++)
+        for (int x = 1; x < W - 1; x++) {
+            double dx = v[(size_t)y * W + x] - v[(size_t)y * W + x - 1];
+            double dy = v[(size_t)y * W + x] - v[(size_t)(y - 1) * W + x];
+            g[(size_t)y * W + x] = (std::fabs(dx) > std::fabs(dy)) ? std::fabs(dx) : std::fabs(dy);
+            if (g[(size_t)y * W + x] > 0.1) edg_num++;
+        }
+    std::vector<double> ex(edg_num), ey(edg_num);
+    int k = 0;
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            double dxp = (Psy_L / nx) * (nx + 2 * ng) / W;
+            double dyp = (Psy_H / ny) * (ny + 2 * ng) / H;
+            if (g[(size_t)y * W + x] > 0.1) {
+                ex[k] = -ng * (Psy_L / nx) + dxp * x;
+                ey[k] = -ng * (Psy_H / ny) + dyp * y;
+                k++;
+            }
+        }
+    const int M = nx + 2 * ng, N = ny + 2 * ng;
+    for (int i = 0; i < M; i++)
+        for (int j = 0; j < N; j++) {
+            int x = (i + 0.5) * (W) / (M);
+            int y = (j + 0.5) * (H) / (N);
+            const double sgn = v[(size_t)y * W + x];
+            const double X = XY[((size_t)j * M + i) * 6 + 0], Y = XY[((size_t)j * M + i) * 6 + 1];
+            double min_distance = 100;
+            for (int e = 0; e < edg_num; e++) {
+                double d = std::sqrt((ex[e] - X) * (ex[e] - X) + (ey[e] - Y) * (ey[e] - Y));
+                if (d < min_distance) min_distance = d;
+            }
+            int pn = sgn >= 0 ? 1 : -1;
+            phi[(size_t)j * M + i] = pn * min_distance;
         }
 }
 

@@ -168,11 +168,28 @@ def setup(case):
     prm = np.array(g.ls_prm(), dtype=np.float64)
     L.or_level_set_hook(ls_kind, ptr(prm), g.N_x, g.N_y, g.num_ghost_cell, ptr(XY), ptr(phi))
     L.or_classify(g.N_x, g.N_y, g.num_ghost_cell, ptr(XY))
+    if not g.quirks:
+        L.or_unit_normals(g.N_x, g.N_y, g.num_ghost_cell, ptr(XY))
     L.or_scheme_index(bc, g.N_x, g.N_y, g.num_ghost_cell, ptr(XY), ptr(S))
     ip = np.array(g.init_prm(), dtype=np.float64)
     L.or_initialize(INIT[g.case], ptr(ip), g.N_x, g.N_y, g.num_ghost_cell, C.c_double(g.gamma),
                     ptr(U_OLD), ptr(U_NEW), ptr(XY))
     return dict(XY=XY, U_OLD=U_OLD, U_NEW=U_NEW, S=S)
+
+
+def mask_phi(case, rgba):
+    """Test/PNGGeo/Level_Set.cpp:31-149 on the CPU: phi per cell (reference
+    cell order) from an (H, W, 4) uint8 image, top row first."""
+    L = lib()
+    g = case
+    XY = np.zeros(g.M * g.N * 6)
+    L.or_psy_coord(C.c_double(g.lo_x), C.c_double(g.lo_y), C.c_double(g.L), C.c_double(g.H),
+                   g.N_x, g.N_y, g.num_ghost_cell, ptr(XY))
+    img = np.ascontiguousarray(rgba, dtype=np.uint8)
+    phi = np.zeros(g.M * g.N)
+    L.or_mask_phi(img.ctypes.data_as(C.c_void_p), img.shape[1], img.shape[0], g.N_x, g.N_y, g.num_ghost_cell,
+                  C.c_double(g.lo_x), C.c_double(g.lo_y), C.c_double(g.hi_x), C.c_double(g.hi_y), ptr(XY), ptr(phi))
+    return phi
 
 
 def run_oracle(case, nsteps, root_mode=0, state=None):
