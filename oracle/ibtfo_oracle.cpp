@@ -39,7 +39,7 @@ namespace {
 enum { SCH_LAX = 0, SCH_MUSCL = 1, SCH_MHM = 2, SCH_GODUNOV = 3 };
 enum { RS_HLL = 0, RS_HLLC = 1, RS_PVRS = 2, RS_SW = 3 };
 enum { LIM_MINMOD = 0, LIM_VANLEER = 1, LIM_VANALBADA = 2 };
-enum { BC_REGULAR = 0, BC_IDW = 1, BC_LINEAR = 2, BC_RIEMANN = 3, BC_BCGFM = 4, BC_HBCGFM = 5 };
+enum { BC_REGULAR = 0, BC_IDW = 1, BC_LINEAR = 2, BC_RIEMANN = 3, BC_BCGFM = 4, BC_HBCGFM = 5, BC_CUT = 6 };
 enum { EDGE_REFERENCE = 0, EDGE_PERIODIC = 1, EDGE_INFLOW = 2 };
 
 // Sound speed. The reference computes std::pow(x, 0.5) (HLL/Riemann_solver.cpp:144,
@@ -47,6 +47,10 @@ enum { EDGE_REFERENCE = 0, EDGE_PERIODIC = 1, EDGE_INFLOW = 2 };
 // swaps in sqrt so a run can be compared bit-for-bit with an IEEE sqrt device path.
 int g_root_mode = 0;
 inline double root(double x) { return g_root_mode ? std::sqrt(x) : std::pow(x, 0.5); }
+
+// cut-cell records of the current solve (or_set_cut), 8 doubles per cell:
+// alpha, aperture W, E, S, N, wall vector x, y, merge target (-1: none)
+const double* g_cut = nullptr;
 
 struct Grid {
     int nx, ny, ng, M, N;
@@ -441,21 +445,43 @@ void update_x(Ctx& c, double dt, const double* U, double* Un, const double* XY) 
     const Grid& g = c.g;
 #pragma omp parallel for
     for (int j = g.ng; j < g.N - g.ng; j++)
-        for (int i = g.ng; i < g.nx + g.ng; i++)
-            if (XY[g.c(i, j) * 6 + 5] == 0)
+        for (int i = g.ng; i < g.nx + g.ng; i++) {
+            const double t = XY[g.c(i, j) * 6 + 5];
+            if (t == 0) {
                 for (int k = 0; k < 4; k++)
                     Un[g.c(i, j) * 4 + k] = U[g.c(i, j) * 4 + k] -
                         dt * (c.slot(i, j, S_FOLD)[k] - c.slot(i - 1, j, S_FOLD)[k]) / c.dx;
+            } else if (t == 3 && g_cut) {  // cut cell: x part of the aperture-weighted balance
+                const double* cc = g_cut + g.c(i, j) * 8;
+                for (int k = 0; k < 4; k++)
+                    Un[g.c(i, j) * 4 + k] = U[g.c(i, j) * 4 + k] -
+                        dt * ((cc[2] * c.slot(i, j, S_FOLD)[k] - cc[1] * c.slot(i - 1, j, S_FOLD)[k]) * c.dy) /
+                            (cc[0] * c.dx * c.dy);
+            }
+        }
 }
-void update_y(Ctx& c, double dt, double* Un, const double* XY) {
+void update_y(Ctx& c, double dt, const double* U, double* Un, const double* XY) {
     const Grid& g = c.g;
 #pragma omp parallel for
     for (int j = g.ng; j < g.N - g.ng; j++)
-        for (int i = g.ng; i < g.nx + g.ng; i++)
-            if (XY[g.c(i, j) * 6 + 5] == 0)
+        for (int i = g.ng; i < g.nx + g.ng; i++) {
+            const double t = XY[g.c(i, j) * 6 + 5];
+            if (t == 0) {
                 for (int k = 0; k < 4; k++)
                     Un[g.c(i, j) * 4 + k] = Un[g.c(i, j) * 4 + k] -
                         dt * (c.slot(i, j, S_GOLD)[k] - c.slot(i, j - 1, S_GOLD)[k]) / c.dy;
+            } else if (t == 3 && g_cut) {  // y part and the wall's pressure force
+                const double* cc = g_cut + g.c(i, j) * 8;
+                const double* q = U + g.c(i, j) * 4;
+                const double rho = q[0], u = q[1] / q[0], v = q[2] / q[0];
+                const double p = (c.gam - 1) * (q[3] - 0.5 * rho * (u * u + v * v));
+                const double wall[4] = {0.0, p * cc[5], p * cc[6], 0.0};
+                for (int k = 0; k < 4; k++)
+                    Un[g.c(i, j) * 4 + k] = Un[g.c(i, j) * 4 + k] -
+                        dt * ((cc[4] * c.slot(i, j, S_GOLD)[k] - cc[3] * c.slot(i, j - 1, S_GOLD)[k]) * c.dx +
+                              wall[k]) / (cc[0] * c.dx * c.dy);
+            }
+        }
 }
 // Lax_Scheme.cpp:27-62 (16-double record: U_TMP slot 0, F_OLD slot 2, G_OLD slot 3)
 void advance_lax(Ctx& c, double dt, const double* U, double* Un, const double* XY) {
@@ -516,7 +542,9 @@ void floors(Ctx& c, double* Un, const double* XY) {
 #pragma omp parallel for
     for (int j = g.ng; j < g.N - g.ng; j++)
         for (int i = g.ng; i < g.nx + g.ng; i++)
-            if (XY[g.c(i, j) * 6 + 5] == 0) apply_floor(c.gam, Un + g.c(i, j) * 4);
+            if (XY[g.c(i, j) * 6 + 5] == 0 ||
+                (XY[g.c(i, j) * 6 + 5] == 3 && !(g_cut && g_cut[g.c(i, j) * 8] < 0.5)))  // small cells: after merging
+                apply_floor(c.gam, Un + g.c(i, j) * 4);
 }
 
 void advance(Ctx& c, int scheme, int rs, int lim, double dt, const double* U, double* Un, const double* XY) {
@@ -532,7 +560,7 @@ void advance(Ctx& c, int scheme, int rs, int lim, double dt, const double* U, do
     if (scheme == SCH_MHM) predictor(c, dt, 1);
     if (!c.quirks && scheme != SCH_GODUNOV) positivity_fallback(c, U, 1);
     riemann_y(c, rs);
-    update_y(c, dt, Un, XY);
+    update_y(c, dt, U, Un, XY);
     if (!c.quirks) floors(c, Un, XY);
 }
 
@@ -615,7 +643,7 @@ double or_compute_dt(double L, double H, int nx, int ny, int ng, double gam, dou
     double a_max = 0.0, u_max = 0.0, v_max = 0.0;
     for (int j = ng; j < g.N - ng; j++)
         for (int i = ng; i < nx + ng; i++)
-            if (XY[g.c(i, j) * 6 + 5] < 1) {
+            if (XY[g.c(i, j) * 6 + 5] < 1 || XY[g.c(i, j) * 6 + 5] == 3) {
                 const double* q = U + g.c(i, j) * 4;
                 double rho = q[0], u = q[1] / q[0], v = q[2] / q[0];
                 double p = (gam - 1) * (q[3] - 0.5 * rho * (u * u + v * v));
@@ -634,6 +662,7 @@ double or_mass(double L, double H, int nx, int ny, int ng, const double* U, cons
     for (int j = 0; j < g.N; j++)
         for (int i = ng; i < nx + ng; i++)
             if (XY[g.c(i, j) * 6 + 5] < 0.5) mass += U[g.c(i, j) * 4] * dx * dy;
+            else if (XY[g.c(i, j) * 6 + 5] == 3 && g_cut) mass += g_cut[g.c(i, j) * 8] * U[g.c(i, j) * 4] * dx * dy;
     return mass;
 }
 
@@ -734,6 +763,144 @@ void or_unit_normals(int nx, int ny, int ng, double* XY) {
     }
 }
 
+// ------------------------------------------------------------------ cut cells
+// No reference counterpart (the reference has no cut-cell boundary). Geometry
+// from the cell-centred level set: corner values are the means of the four
+// surrounding cells' phi; a face's aperture is the fluid (phi >= 0) fraction of
+// the segment between its corners under linear interpolation; the volume
+// fraction alpha is the area of the fluid polygon cut from the cell by those
+// crossings (shoelace); the wall vector is S = -((aE - aW) dy, (aN - aS) dx),
+// which closes the cell's faces so that a uniform pressure exerts no net force.
+// Types: 0 all corners fluid (alpha = 1), 3 cut (0 < alpha < 1), 1 no fluid,
+// 2 an inactive cell within ng of an active one (filled like an IDW ghost) and
+// the domain ring. A cut cell with alpha < 1/2 is merged with a neighbour with
+// alpha >= 1/2: along the normal's dominant axis first, then the other axis,
+// then the opposite directions; with none it becomes inactive.
+static double cut_frac(double p, double q) {
+    if (p >= 0 && q >= 0) return 1.0;
+    if (p < 0 && q < 0) return 0.0;
+    return p >= 0 ? p / (p - q) : q / (q - p);
+}
+void or_set_cut(const double* cut) { g_cut = cut; }
+void or_cut_geometry(int nx, int ny, int ng, double L, double H, double* XY, double* cut) {
+    Grid g(nx, ny, ng);
+    const double dx = L / nx, dy = H / ny;
+    auto phi = [&](int i, int j) { return XY[g.c(i, j) * 6 + 2]; };
+    auto corner = [&](int i, int j) {  // corner (i+1/2, j+1/2)
+        return 0.25 * (((phi(i, j) + phi(i + 1, j)) + phi(i, j + 1)) + phi(i + 1, j + 1));
+    };
+    auto interior = [&](int i, int j) { return i >= ng && i < nx + ng && j >= ng && j < ny + ng; };
+    for (size_t q = 0; q < (size_t)g.M * g.N * 8; q++) cut[q] = 0.0;
+    for (int j = 0; j < g.N; j++)
+        for (int i = 0; i < g.M; i++) {
+            double* cc = cut + g.c(i, j) * 8;
+            cc[7] = -1;
+            if (!interior(i, j)) { XY[g.c(i, j) * 6 + 5] = 2; continue; }
+            const double sw = corner(i - 1, j - 1), se = corner(i, j - 1), ne = corner(i, j), nw = corner(i - 1, j);
+            const double aw = cut_frac(sw, nw), ae = cut_frac(se, ne), as = cut_frac(sw, se), an = cut_frac(nw, ne);
+            double alpha;
+            if (sw >= 0 && se >= 0 && ne >= 0 && nw >= 0) alpha = 1.0;
+            else if (sw < 0 && se < 0 && ne < 0 && nw < 0) alpha = 0.0;
+            else {
+                // polygon around the unit square SW(0,0) SE(1,0) NE(1,1) NW(0,1)
+                const double cx[4] = {0, 1, 1, 0}, cy[4] = {0, 0, 1, 1}, cv[4] = {sw, se, ne, nw};
+                double px[8], py[8];
+                int n = 0;
+                for (int e = 0; e < 4; e++) {
+                    const int f = (e + 1) % 4;
+                    if (cv[e] >= 0) { px[n] = cx[e]; py[n] = cy[e]; n++; }
+                    if ((cv[e] >= 0) != (cv[f] >= 0)) {
+                        const double t = cv[e] / (cv[e] - cv[f]);
+                        px[n] = cx[e] + t * (cx[f] - cx[e]);
+                        py[n] = cy[e] + t * (cy[f] - cy[e]);
+                        n++;
+                    }
+                }
+                double a2 = 0.0;
+                for (int k = 0; k < n; k++) {
+                    const int l = (k + 1) % n;
+                    a2 += px[k] * py[l] - px[l] * py[k];
+                }
+                alpha = 0.5 * a2;
+            }
+            cc[0] = alpha; cc[1] = aw; cc[2] = ae; cc[3] = as; cc[4] = an;
+            cc[5] = -((ae - aw) * dy);
+            cc[6] = -((an - as) * dx);
+            XY[g.c(i, j) * 6 + 5] = alpha >= 1.0 ? 0 : (alpha > 0.0 ? 3 : 1);
+        }
+    // small cells choose a merge target
+    for (int j = ng; j < ny + ng; j++)
+        for (int i = ng; i < nx + ng; i++) {
+            double* cc = cut + g.c(i, j) * 8;
+            if (XY[g.c(i, j) * 6 + 5] != 3 || cc[0] >= 0.5) continue;
+            const double n_x = XY[g.c(i, j) * 6 + 3], n_y = XY[g.c(i, j) * 6 + 4];
+            const int sx = n_x >= 0 ? 1 : -1, sy = n_y >= 0 ? 1 : -1;
+            int ci[4], cj[4];
+            if (std::fabs(n_x) >= std::fabs(n_y)) {
+                ci[0] = i + sx; cj[0] = j; ci[1] = i; cj[1] = j + sy; ci[2] = i - sx; cj[2] = j; ci[3] = i; cj[3] = j - sy;
+            } else {
+                ci[0] = i; cj[0] = j + sy; ci[1] = i + sx; cj[1] = j; ci[2] = i; cj[2] = j - sy; ci[3] = i - sx; cj[3] = j;
+            }
+            for (int q = 0; q < 4; q++) {
+                if (!interior(ci[q], cj[q])) continue;
+                const double tq = XY[g.c(ci[q], cj[q]) * 6 + 5];
+                if ((tq == 0 || tq == 3) && cut[g.c(ci[q], cj[q]) * 8] >= 0.5) {
+                    cc[7] = (double)g.c(ci[q], cj[q]);
+                    break;
+                }
+            }
+            if (cc[7] < 0) XY[g.c(i, j) * 6 + 5] = 1;  // no partner: inactive
+        }
+    // inactive cells within ng of an active cell along an axis are ghosts
+    std::vector<double> t5(g.M * g.N);
+    for (size_t q = 0; q < t5.size(); q++) t5[q] = XY[q * 6 + 5];
+    for (int j = ng; j < ny + ng; j++)
+        for (int i = ng; i < nx + ng; i++) {
+            if (t5[g.c(i, j)] != 1) continue;
+            bool near = false;
+            for (int k = 1; k <= ng && !near; k++) {
+                const int ci[4] = {i - k, i + k, i, i}, cj[4] = {j, j, j - k, j + k};
+                for (int q = 0; q < 4; q++)
+                    if (interior(ci[q], cj[q]) && (t5[g.c(ci[q], cj[q])] == 0 || t5[g.c(ci[q], cj[q])] == 3))
+                        near = true;
+            }
+            if (near) XY[g.c(i, j) * 6 + 5] = 2;
+        }
+}
+
+// Small-cell merging after an update: each target and the small cells that
+// chose it (neighbours W, E, S, N in that order) take the alpha-weighted mean.
+void or_cut_merge(int nx, int ny, int ng, double gam, int quirks, const double* XY, const double* cut, double* U) {
+    Grid g(nx, ny, ng);
+    std::vector<double> out(U, U + (size_t)g.M * g.N * 4);
+    for (int j = ng; j < ny + ng; j++)
+        for (int i = ng; i < nx + ng; i++) {
+            const size_t t = g.c(i, j);
+            const double tt = XY[t * 6 + 5];
+            if (!((tt == 0 || tt == 3) && cut[t * 8] >= 0.5)) continue;
+            const size_t nb[4] = {g.c(i - 1, j), g.c(i + 1, j), g.c(i, j - 1), g.c(i, j + 1)};
+            size_t mem[4];
+            int n = 0;
+            for (int q = 0; q < 4; q++)
+                if (XY[nb[q] * 6 + 5] == 3 && cut[nb[q] * 8 + 7] == (double)t) mem[n++] = nb[q];
+            if (!n) continue;
+            double asum = cut[t * 8];
+            for (int q = 0; q < n; q++) asum = asum + cut[mem[q] * 8];
+            double v[4];
+            for (int k = 0; k < 4; k++) {
+                v[k] = cut[t * 8] * U[t * 4 + k];
+                for (int q = 0; q < n; q++) v[k] = v[k] + cut[mem[q] * 8] * U[mem[q] * 4 + k];
+                v[k] = v[k] / asum;
+            }
+            if (!quirks) apply_floor(gam, v);  // the small cells' provisional states were not floored
+            for (int k = 0; k < 4; k++) {
+                out[t * 4 + k] = v[k];
+                for (int q = 0; q < n; q++) out[mem[q] * 4 + k] = v[k];
+            }
+        }
+    std::copy(out.begin(), out.end(), U);
+}
+
 }  // extern "C"
 
 namespace {
@@ -742,14 +909,14 @@ namespace {
 static void mirror_idx(const Grid& g, double* S, int W, const double* XY, int k) {
     for (int j = g.ng; j < g.N - g.ng; j++)
         for (int i = g.ng; i < g.nx + g.ng; i++)
-            if (XY[g.c(i, j) * 6 + 5] > 1) {
+            if (XY[g.c(i, j) * 6 + 5] == 2) {
                 double xm = S[g.c(i, j) * W + k];
                 for (int ii = 0; ii < g.M - 1; ii++)
                     if (xm < XY[g.c(ii + 1, j) * 6] && xm >= XY[g.c(ii, j) * 6]) S[g.c(i, j) * W + k + 2] = ii;
             }
     for (int j = g.ng; j < g.N - g.ng; j++)
         for (int i = g.ng; i < g.nx + g.ng; i++)
-            if (XY[g.c(i, j) * 6 + 5] > 1) {
+            if (XY[g.c(i, j) * 6 + 5] == 2) {
                 double ym = S[g.c(i, j) * W + k + 1];
                 for (int jj = 0; jj < g.N - 1; jj++)
                     if (ym < XY[g.c(0, jj + 1) * 6 + 1] && ym >= XY[g.c(0, jj) * 6 + 1]) S[g.c(i, j) * W + k + 3] = jj;
@@ -761,7 +928,7 @@ template <class Keep>
 static void idw_weights(const Grid& g, double* S, int W, const double* XY, int kp, int ka, Keep keep) {
     for (int j = g.ng; j < g.N - g.ng; j++)
         for (int i = g.ng; i < g.nx + g.ng; i++)
-            if (XY[g.c(i, j) * 6 + 5] > 1) {
+            if (XY[g.c(i, j) * 6 + 5] == 2) {
                 double* s = S + g.c(i, j) * W;
                 int IDX = (int)s[kp + 2], IDY = (int)s[kp + 3];
                 const int di[4] = {0, 0, 1, 1}, dj[4] = {0, 1, 1, 0};
@@ -783,7 +950,7 @@ extern "C" {
 
 int or_sch_width(int bc) {
     switch (bc) {
-        case BC_IDW: return 8;
+        case BC_IDW: case BC_CUT: return 8;
         case BC_LINEAR: case BC_RIEMANN: return 4;
         case BC_BCGFM: case BC_HBCGFM: return 16;
         default: return 1;
@@ -800,8 +967,8 @@ void or_scheme_index(int bc, int nx, int ny, int ng, const double* XY, double* S
         for (int i = ng; i < nx + ng; i++) {
             const double* p = XY + g.c(i, j) * 6;
             double* s = S + g.c(i, j) * W;
-            bool ghost = p[5] > 1;
-            if (bc == BC_IDW || bc == BC_LINEAR || bc == BC_RIEMANN) {
+            bool ghost = p[5] == 2;
+            if (bc == BC_IDW || bc == BC_CUT || bc == BC_LINEAR || bc == BC_RIEMANN) {
                 if (ghost) { s[0] = p[0] + 2 * p[3] * std::fabs(p[2]); s[1] = p[1] + 2 * p[4] * std::fabs(p[2]); }
                 else { s[0] = 0; s[1] = 0; }
             } else {
@@ -822,6 +989,7 @@ void or_scheme_index(int bc, int nx, int ny, int ng, const double* XY, double* S
         }
     mirror_idx(g, S, W, XY, 0);
     if (bc == BC_IDW) idw_weights(g, S, W, XY, 0, 4, [](double t) { return t < 1; });
+    if (bc == BC_CUT) idw_weights(g, S, W, XY, 0, 4, [](double t) { return t == 0 || t == 3; });
     if (bc == BC_BCGFM || bc == BC_HBCGFM) {
         mirror_idx(g, S, W, XY, 8);
         idw_weights(g, S, W, XY, 0, 4, [](double t) { return t != 1; });
@@ -843,14 +1011,14 @@ void or_boundary(int bc, int nx, int ny, int ng, double gam, double* U_OLD, doub
     }
     for (int j = ng; j < g.N - ng; j++)
         for (int i = ng; i < nx + ng; i++)
-            if (XY[g.c(i, j) * 6 + 5] == 0)
+            if (XY[g.c(i, j) * 6 + 5] == 0 || (bc == BC_CUT && XY[g.c(i, j) * 6 + 5] == 3))
                 for (int k = 0; k < 4; k++) U_OLD[g.c(i, j) * 4 + k] = U_NEW[g.c(i, j) * 4 + k];
     for (size_t q = 0; q < (size_t)g.M * g.N * 4; q++) U_NEW[q] = U_OLD[q];
     double dx = XY[g.c(1, 1) * 6] - XY[g.c(0, 1) * 6];
     double dy = XY[g.c(1, 2) * 6 + 1] - XY[g.c(1, 1) * 6 + 1];
     for (int j = ng; j < g.N - ng; j++)
         for (int i = ng; i < nx + ng; i++) {
-            if (!(XY[g.c(i, j) * 6 + 5] > 1)) continue;
+            if (!(XY[g.c(i, j) * 6 + 5] == 2)) continue;
             const double* s = S + g.c(i, j) * W;
             double n_x = XY[g.c(i, j) * 6 + 3], n_y = XY[g.c(i, j) * 6 + 4];
             // donors 1..4 = (IDX,IDY), (IDX,IDY+1), (IDX+1,IDY+1), (IDX+1,IDY)
@@ -864,7 +1032,7 @@ void or_boundary(int bc, int nx, int ny, int ng, double gam, double* U_OLD, doub
                 }
             };
             double rw, uw, vw, pw;
-            if (bc == BC_IDW || bc == BC_BCGFM || bc == BC_HBCGFM) {
+            if (bc == BC_IDW || bc == BC_CUT || bc == BC_BCGFM || bc == BC_HBCGFM) {
                 auto idw = [&](int kp, int ka, double& R, double& Uu, double& V, double& P) {
                     double r[4], u[4], v[4], p[4], t[4];
                     donors(kp, r, u, v, p, t);
@@ -877,7 +1045,7 @@ void or_boundary(int bc, int nx, int ny, int ng, double gam, double* U_OLD, doub
                     V = (a1 * v[0] + a2 * v[1] + a3 * v[2] + a4 * v[3]) / (a1 + a2 + a3 + a4);
                     P = (a1 * p[0] + a2 * p[1] + a3 * p[2] + a4 * p[3]) / (a1 + a2 + a3 + a4);
                 };
-                if (bc == BC_IDW) {
+                if (bc == BC_IDW || bc == BC_CUT) {
                     if (!quirks && s[4] + s[5] + s[6] + s[7] == 0) continue;  // no fluid donor: keep the ghost
                     idw(0, 4, rw, uw, vw, pw);
                     double u_n = uw * n_x + vw * n_y;
@@ -1128,6 +1296,7 @@ void or_time_advance(int time, int scheme, int rs, int lim, int bc, int edge, co
                      double* U_NEW, const double* XY, const double* S) {
     if (time == 0) {
         or_advance(scheme, rs, lim, quirks, L, H, nx, ny, ng, gam, dt, U_OLD, T, U_NEW, XY);
+        if (bc == BC_CUT && g_cut) or_cut_merge(nx, ny, ng, gam, quirks, XY, g_cut, U_NEW);
         return;
     }
     Grid g(nx, ny, ng);

@@ -22,7 +22,8 @@ REF_DIR = os.path.join(HERE, "_ref")
 SCHEMES = {"Lax": 0, "MUSCL": 1, "MHM": 2, "Godunov": 3}
 RIEMANNS = {"HLL": 0, "HLLC": 1, "PVRS": 2, "StegerWarming": 3}
 LIMITERS = {"minmod": 0, "VanLeer": 1, "VanAlbada": 2}
-BOUNDS = {"Regular": 0, "IDWBaseGFM": 1, "LinearBaseGFM": 2, "RiemannBaseGFM": 3, "BCGFM": 4, "HBCGFM": 5}
+BOUNDS = {"Regular": 0, "IDWBaseGFM": 1, "LinearBaseGFM": 2, "RiemannBaseGFM": 3, "BCGFM": 4, "HBCGFM": 5,
+          "CutCell": 6}
 EDGES = {"reference": 0, "periodic": 1, "inflow": 2}
 TIMES = {"SingleStep": 0, "RK2": 1}
 # level-set hooks and initialisers (see or_level_set_hook / or_initialize)
@@ -170,11 +171,15 @@ def setup(case):
     L.or_classify(g.N_x, g.N_y, g.num_ghost_cell, ptr(XY))
     if not g.quirks:
         L.or_unit_normals(g.N_x, g.N_y, g.num_ghost_cell, ptr(XY))
+    cut = None
+    if g.Boundary == "CutCell":
+        cut = np.zeros(cells * 8)
+        L.or_cut_geometry(g.N_x, g.N_y, g.num_ghost_cell, C.c_double(g.L), C.c_double(g.H), ptr(XY), ptr(cut))
     L.or_scheme_index(bc, g.N_x, g.N_y, g.num_ghost_cell, ptr(XY), ptr(S))
     ip = np.array(g.init_prm(), dtype=np.float64)
     L.or_initialize(INIT[g.case], ptr(ip), g.N_x, g.N_y, g.num_ghost_cell, C.c_double(g.gamma),
                     ptr(U_OLD), ptr(U_NEW), ptr(XY))
-    return dict(XY=XY, U_OLD=U_OLD, U_NEW=U_NEW, S=S)
+    return dict(XY=XY, U_OLD=U_OLD, U_NEW=U_NEW, S=S, cut=cut)
 
 
 def mask_phi(case, rgba):
@@ -203,12 +208,14 @@ def run_oracle(case, nsteps, root_mode=0, state=None):
     t_log = np.zeros(max(nsteps, 1))
     m_log = np.zeros(max(nsteps, 1))
     ep = np.array(g.edge_prm(), dtype=np.float64)
+    L.or_set_cut(ptr(st.get("cut")))
     n = L.or_run(TIMES[g.Time], SCHEMES[g.Scheme], RIEMANNS[g.Riemann], LIMITERS[g.Limiter], BOUNDS[g.Boundary],
                  EDGES[g.edge], ptr(ep), g.quirks, C.c_double(g.L), C.c_double(g.H), g.N_x, g.N_y,
                  g.num_ghost_cell, C.c_double(g.gamma), C.c_double(g.CFL_number), C.c_double(g.Psy_time),
                  min(g.max_iter, nsteps), g.init_boundary(), ptr(st["U_OLD"]), ptr(st["U_NEW"]), ptr(st["XY"]),
                  ptr(st["S"]), ptr(T), ptr(dt_log), ptr(t_log), ptr(m_log))
     L.or_set_root_mode(0)
+    L.or_set_cut(None)
     st.update(steps=n, dt=dt_log[:n], t=t_log[:n], mass=m_log[:n])
     return st
 

@@ -1,0 +1,113 @@
+"""Cut-cell boundary in the CPU oracle (no reference counterpart): geometry
+closed forms and the invariants that fix the method (free stream at rest,
+reduction to the regular scheme without a body, conservation of the
+alpha-weighted mass, and a stable supersonic run)."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+
+def _circle_case(O, n=64, r=0.21, cx=0.5, cy=0.5, **kw):
+    args = dict(N_x=n, N_y=n, hi_x=1.0, hi_y=1.0, Boundary="CutCell", case="uniform", quirks=0, edge="periodic",
+                Psy_time=1e30, CFL_number=0.4, extra=dict(rho=1.0, u=0.0, v=0.0, p=1.0))
+    args.update(kw)
+    c = O.Case(**args)
+    h = 1.0 / n
+    x = -2 * h + h / 2 + np.arange(c.M) * h
+    X, Y = np.meshgrid(x, x[: c.N])
+    c.phi = (np.hypot(X - cx, Y - cy) - r).reshape(-1)
+    return c
+
+
+def test_geometry_area_perimeter_and_apertures(oracle_mod):
+    O = oracle_mod
+    n, r = 128, 0.21
+    c = _circle_case(O, n=n, r=r)
+    st = O.setup(c)
+    cut = st["cut"].reshape(c.N, c.M, 8)
+    t = st["XY"][5::6].reshape(c.N, c.M)
+    h = 1.0 / n
+    inner = np.s_[2:-2, 2:-2]
+    alpha = np.where((t[inner] == 0), 1.0, np.where(t[inner] == 3, cut[inner][..., 0], 0.0))
+    area = alpha.sum() * h * h
+    assert abs(area - (1 - math.pi * r * r)) < 2e-4, area
+    wall = np.hypot(cut[..., 5], cut[..., 6])[t == 3].sum()
+    assert abs(wall - 2 * math.pi * r) < 0.01 * 2 * math.pi * r, wall
+    # a face has one aperture: the east aperture of a cell is the west one of its neighbour
+    act = (t == 0) | (t == 3)
+    both = act[:, :-1] & act[:, 1:]
+    assert np.array_equal(cut[:, :-1, 2][both], cut[:, 1:, 1][both])
+    bothy = act[:-1, :] & act[1:, :]
+    assert np.array_equal(cut[:-1, :, 4][bothy], cut[1:, :, 3][bothy])
+    # every cut cell with alpha < 1/2 has a merge target with alpha >= 1/2
+    small = (t == 3) & (cut[..., 0] < 0.5)
+    tg = cut[..., 7][small].astype(int)
+    assert np.all(tg >= 0)
+    assert np.all(cut.reshape(-1, 8)[tg, 0] >= 0.5)
+
+
+def test_gas_at_rest_stays_at_rest(oracle_mod):
+    O = oracle_mod
+    c = _circle_case(O, n=48)
+    r = O.run_oracle(c, 60)
+    U = r["U_OLD"].reshape(c.N, c.M, 4)
+    t = r["XY"][5::6].reshape(c.N, c.M)
+    act = (t == 0) | (t == 3)
+    assert r["steps"] == 60
+    assert np.max(np.abs(U[act][:, 1:3])) < 1e-13
+    assert np.max(np.abs(U[act][:, 0] - 1.0)) < 1e-13
+
+
+def test_no_body_is_the_regular_scheme(oracle_mod):
+    O = oracle_mod
+    kw = dict(N_x=40, N_y=10, hi_y=0.25, case="sodx", quirks=0, edge="periodic", Psy_time=0.1)
+    a = O.run_oracle(O.Case(Boundary="CutCell", **kw), 40)
+    b = O.run_oracle(O.Case(Boundary="Regular", **kw), 40)
+    assert a["steps"] == b["steps"]
+    assert np.array_equal(a["U_OLD"], b["U_OLD"])
+
+
+def test_alpha_weighted_mass_is_conserved(oracle_mod):
+    """Periodic box, circular body, a pressure jump across the box: the fluid
+    mass sum(alpha rho) dx dy changes only by round-off."""
+    O = oracle_mod
+    c = _circle_case(O, n=64, r=0.17, cx=0.62, cy=0.47)
+    st = O.setup(c)
+    U = st["U_OLD"].reshape(c.N, c.M, 4)
+    h = 1.0 / 64
+    x = -2 * h + h / 2 + np.arange(c.M) * h
+    left = x[None, :] < 0.3
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    U[..., 0] = rho
+    U[..., 1] = 0
+    U[..., 2] = 0
+    U[..., 3] = p / 0.4
+    st["U_NEW"][:] = st["U_OLD"]
+
+    def mass(U, XY, cut):
+        t = XY[5::6]
+        a = np.where(t == 0, 1.0, np.where(t == 3, cut[0::8], 0.0))
+        return float(np.sum(a * U[0::4]))
+
+    m0 = mass(st["U_OLD"], st["XY"], st["cut"])
+    r = O.run_oracle(c, 80, state=st)
+    m1 = mass(r["U_OLD"], r["XY"], r["cut"])
+    assert r["steps"] == 80
+    assert abs(m1 - m0) <= 1e-12 * m0, (m0, m1)
+    assert abs(r["mass"][-1] / (h * h) - m1) <= 1e-12 * m1
+
+
+def test_mach3_past_cylinder_is_stable(oracle_mod):
+    O = oracle_mod
+    c = _circle_case(O, n=96, r=0.1, cx=0.4, cy=0.5, edge="inflow", Limiter="VanLeer", CFL_number=0.5,
+                     extra=dict(rho=1.4, u=3.0, v=0.0, p=1.0))
+    r = O.run_oracle(c, 300)
+    U = r["U_OLD"].reshape(c.N, c.M, 4)
+    t = r["XY"][5::6].reshape(c.N, c.M)
+    act = (t == 0) | (t == 3)
+    assert r["steps"] == 300 and np.isfinite(U[act]).all()
+    rho = U[act][:, 0]
+    assert rho.min() > 1e-3 and rho.max() < 8  # near-vacuum wake of the impulsive start; ~5.8 behind the bow shock
