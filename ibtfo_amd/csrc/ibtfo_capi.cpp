@@ -88,11 +88,15 @@ static int validate(const ibtfo_config* c) {
     if (c->N_x < 4 || c->N_y < 2 * c->num_ghost_cell) { set_error("grid too small"); return IBTFO_E_ARG; }
     if ((long long)c->N_x * c->N_y > (1ll << 31)) { set_error("grid too large"); return IBTFO_E_ARG; }
     if (c->scheme < 0 || c->scheme > 3 || c->riemann < 0 || c->riemann > 3 || c->limiter < 0 || c->limiter > 2 ||
-        c->boundary < 0 || c->boundary > 5 || c->edge < 0 || c->edge > 2) {
+        c->boundary < 0 || c->boundary > 6 || c->edge < 0 || c->edge > 2) {
         set_error("slot value out of range");
         return IBTFO_E_ARG;
     }
     if (c->time != IBTFO_SINGLESTEP && c->time != IBTFO_RK2) { set_error("unknown Time slot"); return IBTFO_E_ARG; }
+    if (c->boundary == IBTFO_CUTCELL && (c->quirks || c->time != IBTFO_SINGLESTEP || c->scheme == IBTFO_LAX)) {
+        set_error("the cut-cell boundary needs quirks = 0, Time = SingleStep and a reconstructing scheme");
+        return IBTFO_E_ARG;
+    }
     if (c->time == IBTFO_RK2 && c->scheme == IBTFO_LAX) {
         set_error("Time=RK2 needs a reconstructing scheme (MUSCL, MHM, Godunov)");
         return IBTFO_E_ARG;
@@ -124,7 +128,7 @@ void ibtfo::free_ctx(ibtfo_ctx* x) {
     if (x->graph) (void)hipGraphExecDestroy(x->graph);
     for (void* p : {(void*)x->state[0], (void*)x->state[1], (void*)x->type, (void*)x->phi, (void*)x->gnx,
                     (void*)x->gny, (void*)x->ghosts, (void*)x->gvals, (void*)x->dcount, (void*)x->ctl,
-                    (void*)x->mass_part, (void*)x->corner})
+                    (void*)x->mass_part, (void*)x->corner, (void*)x->cuts, (void*)x->cut_index, (void*)x->groups})
         dfree(x, p);
     if (x->own_stream) (void)hipStreamDestroy(x->own_stream);
     delete x;
@@ -217,7 +221,7 @@ extern "C" int ibtfo_dims(const ibtfo_ctx* x, int* M, int* N) {
 
 extern "C" int ibtfo_sch_width(const ibtfo_ctx* x) {
     switch (x->cfg.boundary) {
-        case IBTFO_IDW_GFM: return 8;
+        case IBTFO_IDW_GFM: case IBTFO_CUTCELL: return 8;
         case IBTFO_LINEAR_GFM: case IBTFO_RIEMANN_GFM: return 4;
         case IBTFO_BCGFM: case IBTFO_HBCGFM: return 16;
         default: return 1;
@@ -228,9 +232,71 @@ extern "C" int ibtfo_num_ghosts(const ibtfo_ctx* x) { return x ? x->nghost : -1;
 // --------------------------------------------------------------- geometry
 // Runs the geometry kernels for a prepared GeomArgs (hook constants and inputs
 // set) and builds the ghost list; frees `dphi` (a device input of the hook).
+// Cut-cell geometry after geom_run: dense scratch, records, types, merge
+// targets, ghosts; then the compact records, the index plane and the groups.
+static int cut_geometry(ibtfo_ctx* x, GeomArgs& g) {
+    const Layout& L = x->L;
+    dfree(x, x->cuts); x->cuts = nullptr;
+    dfree(x, x->cut_index); x->cut_index = nullptr;
+    dfree(x, x->groups); x->groups = nullptr;
+    x->ncut = x->ngroups = 0;
+    double* dense = nullptr;
+    if (dalloc(x, &dense, (size_t)L.M * L.N * 8 * sizeof(double)) != hipSuccess ||
+        dalloc(x, &x->cut_index, (size_t)L.plane * sizeof(int)) != hipSuccess) {
+        dfree(x, dense);
+        set_error("cut-cell geometry: out of device memory");
+        return IBTFO_E_NOMEM;
+    }
+    g.dcut = dense;
+    g.cut_index = x->cut_index;
+    g.ccount = x->dcount;
+    unsigned int zero[2] = {0, 0}, cnt[2] = {0, 0};
+    unsigned int* gcount = nullptr;
+    if (dalloc(x, &gcount, sizeof(unsigned int)) != hipSuccess) { dfree(x, dense); return IBTFO_E_NOMEM; }
+    g.gcount = gcount;
+    auto pass = [&](int fill) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(x->dcount, &zero[0], sizeof(unsigned), hipMemcpyHostToDevice, x->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(gcount, &zero[1], sizeof(unsigned), hipMemcpyHostToDevice, x->stream);
+        if (e == hipSuccess && !fill) e = cut_run(g, x->stream);
+        if (e == hipSuccess) e = cut_compact(g, x->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&cnt[0], x->dcount, sizeof(unsigned), hipMemcpyDeviceToHost, x->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&cnt[1], gcount, sizeof(unsigned), hipMemcpyDeviceToHost, x->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+        return e;
+    };
+    g.ccap = g.gcap = 0;
+    hipError_t e = pass(0);
+    int rc = IBTFO_OK;
+    if (e == hipSuccess) {
+        x->ncut = (int)cnt[0];
+        x->ngroups = (int)cnt[1];
+        if ((x->ncut && dalloc(x, &x->cuts, x->ncut * sizeof(CutCell)) != hipSuccess) ||
+            (x->ngroups && dalloc(x, &x->groups, x->ngroups * sizeof(MergeGroup)) != hipSuccess)) {
+            set_error("cut-cell geometry: out of device memory for %d records", x->ncut);
+            rc = IBTFO_E_NOMEM;
+        } else {
+            g.cuts = x->cuts;
+            g.ccap = x->ncut;
+            g.groups = x->groups;
+            g.gcap = x->ngroups;
+            e = pass(1);
+        }
+    }
+    dfree(x, dense);
+    dfree(x, gcount);
+    g.dcut = nullptr;
+    if (rc) return rc;
+    HIPCHK(e);
+    return IBTFO_OK;
+}
+
 static int geometry_finish(ibtfo_ctx* x, GeomArgs& g, double* dphi) {
     hipError_t ge = geom_run(g, x->stream);
     if (ge != hipSuccess) { dfree(x, dphi); HIPCHK(ge); }
+    if (x->cfg.boundary == IBTFO_CUTCELL) {
+        int rc = cut_geometry(x, g);
+        if (rc) { dfree(x, dphi); return rc; }
+    }
     // ghost list (upper bound: all interior cells of type 2; count first)
     dfree(x, x->ghosts); x->ghosts = nullptr;
     dfree(x, x->gvals); x->gvals = nullptr;
@@ -271,8 +337,11 @@ extern "C" int ibtfo_geometry(ibtfo_ctx* x, int kind, const double* prm, int npr
     g.Li = x->Li; g.Hi = x->Hi; g.dx = x->dxc;
     g.kind = kind;
     g.phi = x->phi; g.nx = x->gnx; g.ny = x->gny; g.type = x->type;
-    g.bc = x->cfg.boundary;
+    g.bc = x->cfg.boundary == IBTFO_CUTCELL ? IBTFO_IDW_GFM : x->cfg.boundary;
     g.unit_normals = x->cfg.quirks == 0;
+    g.cut = x->cfg.boundary == IBTFO_CUTCELL;
+    g.cdx = (x->cfg.hi_x - x->cfg.lo_x) / x->cfg.N_x;
+    g.cdy = (x->cfg.hi_y - x->cfg.lo_y) / x->cfg.N_y;
     double* dphi = nullptr;
     auto need = [&](int n) {
         if (nprm < n || !prm) { set_error("level-set hook %d needs %d parameters", kind, n); return false; }
@@ -386,8 +455,11 @@ extern "C" int ibtfo_geometry_mask(ibtfo_ctx* x, const unsigned char* rgba, int 
     g.kind = IBTFO_LS_FIELD;
     g.phi_in = dphi;
     g.phi = x->phi; g.nx = x->gnx; g.ny = x->gny; g.type = x->type;
-    g.bc = c.boundary;
+    g.bc = c.boundary == IBTFO_CUTCELL ? IBTFO_IDW_GFM : c.boundary;
     g.unit_normals = c.quirks == 0;
+    g.cut = c.boundary == IBTFO_CUTCELL;
+    g.cdx = Psy_L / c.N_x;
+    g.cdy = Psy_H / c.N_y;
     return geometry_finish(x, g, dphi);
 }
 
@@ -570,12 +642,14 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     a.edge_fused = (c.edge == IBTFO_EDGE_REFERENCE && c.scheme != IBTFO_LAX) ? 1 : 0;
     a.corner_from_input = c.boundary != IBTFO_REGULAR;
     step_grid(L, x->rows_override, &a.nbx, &a.nby, &a.rows);
+    a.cut = x->cfg.boundary == IBTFO_CUTCELL ? x->cuts : nullptr;
+    a.cut_index = x->cut_index;
 
     FillArgs& f = x->fa;
     memset(&f, 0, sizeof f);
     f.buf[0] = x->state[0]; f.buf[1] = x->state[1];
     f.L = L; f.ctl = x->ctl; f.ghosts = x->ghosts; f.nghost = x->nghost; f.out = x->gvals;
-    f.bc = c.boundary; f.gam = c.gamma; f.gm1 = c.gamma - 1;
+    f.bc = c.boundary == IBTFO_CUTCELL ? IBTFO_IDW_GFM : c.boundary; f.gam = c.gamma; f.gm1 = c.gamma - 1;
     f.dxg = (x->Li + 1 * x->dxc) - (x->Li + 0 * x->dxc);
     f.dyg = (x->Hi + 2 * x->dxc) - (x->Hi + 1 * x->dxc);
     f.use_cur = 1; f.require_ran = 1; f.ring_prev = 1; f.quirks = c.quirks;
@@ -600,6 +674,17 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     q.gam = c.gamma; q.gm1 = c.gamma - 1; q.dx = a.dx; q.dy = a.dy; q.cfl = c.CFL_number;
     q.t_end = c.Psy_time; q.max_iter = c.max_iter;
     q.nblocks = std::min(x->mass_cap, std::max(1, (L.nx * L.ny + 255) / 256));
+    q.buf[0] = x->state[0]; q.buf[1] = x->state[1];
+    q.from_step = 0;
+    q.cut = a.cut;
+    q.cut_index = x->cut_index;
+
+    MergeArgs& mg = x->ma;
+    memset(&mg, 0, sizeof mg);
+    mg.buf[0] = x->state[0]; mg.buf[1] = x->state[1];
+    mg.plane = L.plane;
+    mg.ctl = x->ctl; mg.groups = x->groups; mg.ngroups = x->ngroups;
+    mg.gm1 = c.gamma - 1; mg.r_gm1 = 1.0 / (c.gamma - 1); mg.quirks = c.quirks;
 }
 
 // Boundary and Global_Boundary after an update (the state just made current):
@@ -641,6 +726,17 @@ static int enqueue_step(ibtfo_ctx* x, hipEvent_t* ev = nullptr) {
         if (ev) HIPCHK(hipEventRecord(ev[0], x->stream));
         HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, x->sa, x->stream));
         if (ev) HIPCHK(hipEventRecord(ev[1], x->stream));
+    }
+    if (c.boundary == IBTFO_CUTCELL) {
+        // small cells merged, then ComputeDt / mass over the merged state
+        HIPCHK(launch_merge(x->ma, x->stream));
+        ReduceArgs r = x->ra;
+        r.from_step = 1;
+        HIPCHK(launch_reduce_partials(r, x->stream));
+        FinishArgs f = x->fin;
+        f.nblocks = r.nblocks;
+        HIPCHK(launch_finish(f, x->stream));
+        return enqueue_boundaries(x);
     }
     HIPCHK(launch_finish(x->fin, x->stream));
     return enqueue_boundaries(x);

@@ -35,6 +35,11 @@ struct ibtfo_ctx {
     ibtfo::EdgeArgs ea;
     ibtfo::ReduceArgs ra;
     ibtfo::FinishArgs fin;
+    ibtfo::MergeArgs ma;
+    ibtfo::CutCell* cuts;     // cut-cell records, index plane and merge groups
+    int* cut_index;
+    ibtfo::MergeGroup* groups;
+    int ncut, ngroups;
     hipGraphExec_t graph;
     int graph_chunk;
     std::vector<double> corner_host;

@@ -19,6 +19,17 @@ struct GeomArgs {
     uint8_t* type;
     int bc;
     int unit_normals;       // corrected mode: normals scaled to unit length
+    // cut cells (bc = IDW fill with active donors = types 0 and 3)
+    int cut;
+    double cdx, cdy;        // cell sizes (Psy_L / N_x, Psy_H / N_y)
+    double* dcut;           // dense scratch: M x N records of 8 doubles
+    CutCell* cuts;          // compact records of the type-3 cells
+    int* cut_index;         // plane: record of each type-3 cell
+    unsigned int* ccount;
+    int ccap;
+    MergeGroup* groups;
+    unsigned int* gcount;
+    int gcap;
     Ghost* ghosts;
     unsigned int* count;
     int cap;
@@ -43,6 +54,11 @@ struct MaskArgs {
 
 hipError_t geom_run(GeomArgs& g, hipStream_t s);
 hipError_t mask_run(MaskArgs& m, hipStream_t s);
+// cut-cell geometry on top of geom_run's phi and normals: records, types, merge
+// targets, ghosts (dense scratch g.dcut), then the compaction passes (counting
+// when ccap / gcap are 0)
+hipError_t cut_run(GeomArgs& g, hipStream_t s);
+hipError_t cut_compact(GeomArgs& g, hipStream_t s);
 hipError_t geom_ghosts(GeomArgs& g, hipStream_t s);
 
 }  // namespace ibtfo

@@ -172,7 +172,7 @@ __global__ void k_geom_ghosts(GeomArgs g) {
         for (int q = 0; q < 4; q++) {
             int ii = gh.idx + di[q], jj = gh.idy + dj[q];
             uint8_t t = g.type[L.at(ii, jj)];
-            bool keep = (bc == 1) ? (t < 1) : (t != 1);
+            bool keep = (bc == 1) ? (g.cut ? (t == 0 || t == 3) : (t < 1)) : (t != 1);
             if (keep) {
                 double ddx = xc(g, ii) - gh.xm, ddy = yc(g, jj) - gh.ym;
                 gh.a[q] = 1 / ib_sqrt(ddx * ddx + ddy * ddy);
@@ -241,6 +241,171 @@ hipError_t geom_ghosts(GeomArgs& g, hipStream_t s) {
 }
 
 
+
+// ------------------------------------------------------------ cut cells
+// No reference counterpart. From the cell-centred phi: corner values are the
+// means of the four surrounding cells' phi, a face's aperture is its fluid
+// fraction under linear interpolation between its corners, the volume fraction
+// is the area of the fluid polygon those crossings cut from the cell, and the
+// wall vector S = -((aE - aW) dy, (aN - aS) dx) closes the faces.
+__device__ __forceinline__ double cut_frac(double p, double q) {
+    if (p >= 0 && q >= 0) return 1.0;
+    if (p < 0 && q < 0) return 0.0;
+    return p >= 0 ? p / (p - q) : q / (q - p);
+}
+__device__ __forceinline__ double corner_phi(const GeomArgs& g, int i, int j) {  // corner (i+1/2, j+1/2)
+    const Layout& L = g.L;
+    return 0.25 * (((g.phi[L.at(i, j)] + g.phi[L.at(i + 1, j)]) + g.phi[L.at(i, j + 1)]) + g.phi[L.at(i + 1, j + 1)]);
+}
+__device__ __forceinline__ long long cell_q(const Layout& L, int i, int j) { return (long long)j * L.M + i; }
+
+__global__ void k_cut_cells(GeomArgs g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const Layout L = g.L;
+    if (i >= L.M) return;
+    double* cc = g.dcut + cell_q(L, i, j) * 8;
+    for (int k = 0; k < 7; k++) cc[k] = 0.0;
+    cc[7] = -1.0;
+    const int ng = L.ng;
+    if (i < ng || i >= L.nx + ng || j < ng || j >= L.ny + ng) { g.type[L.at(i, j)] = 2; return; }
+    const double sw = corner_phi(g, i - 1, j - 1), se = corner_phi(g, i, j - 1), ne = corner_phi(g, i, j),
+                 nw = corner_phi(g, i - 1, j);
+    const double aw = cut_frac(sw, nw), ae = cut_frac(se, ne), as = cut_frac(sw, se), an = cut_frac(nw, ne);
+    double alpha;
+    if (sw >= 0 && se >= 0 && ne >= 0 && nw >= 0) alpha = 1.0;
+    else if (sw < 0 && se < 0 && ne < 0 && nw < 0) alpha = 0.0;
+    else {
+        const double cx[4] = {0, 1, 1, 0}, cy[4] = {0, 0, 1, 1}, cv[4] = {sw, se, ne, nw};
+        double px[8], py[8];
+        int n = 0;
+        for (int e = 0; e < 4; e++) {
+            const int f = (e + 1) % 4;
+            if (cv[e] >= 0) { px[n] = cx[e]; py[n] = cy[e]; n++; }
+            if ((cv[e] >= 0) != (cv[f] >= 0)) {
+                const double t = cv[e] / (cv[e] - cv[f]);
+                px[n] = cx[e] + t * (cx[f] - cx[e]);
+                py[n] = cy[e] + t * (cy[f] - cy[e]);
+                n++;
+            }
+        }
+        double a2 = 0.0;
+        for (int k = 0; k < n; k++) {
+            const int l = (k + 1) % n;
+            a2 += px[k] * py[l] - px[l] * py[k];
+        }
+        alpha = 0.5 * a2;
+    }
+    cc[0] = alpha; cc[1] = aw; cc[2] = ae; cc[3] = as; cc[4] = an;
+    cc[5] = -((ae - aw) * g.cdy);
+    cc[6] = -((an - as) * g.cdx);
+    g.type[L.at(i, j)] = alpha >= 1.0 ? 0 : (alpha > 0.0 ? 3 : 1);
+}
+
+// A cut cell with alpha < 1/2 picks a neighbour with alpha >= 1/2: along the
+// normal's dominant axis first, then the other axis, then the opposite
+// directions; without one it becomes inactive.
+__global__ void k_cut_target(GeomArgs g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const Layout L = g.L;
+    const int ng = L.ng;
+    if (i < ng || i >= L.nx + ng || j < ng || j >= L.ny + ng) return;
+    const long long o = L.at(i, j);
+    double* cc = g.dcut + cell_q(L, i, j) * 8;
+    if (g.type[o] != 3 || cc[0] >= 0.5) return;
+    const double n_x = g.nx[o], n_y = g.ny[o];
+    const int sx = n_x >= 0 ? 1 : -1, sy = n_y >= 0 ? 1 : -1;
+    int ci[4], cj[4];
+    if (ib_fabs(n_x) >= ib_fabs(n_y)) {
+        ci[0] = i + sx; cj[0] = j; ci[1] = i; cj[1] = j + sy; ci[2] = i - sx; cj[2] = j; ci[3] = i; cj[3] = j - sy;
+    } else {
+        ci[0] = i; cj[0] = j + sy; ci[1] = i + sx; cj[1] = j; ci[2] = i; cj[2] = j - sy; ci[3] = i - sx; cj[3] = j;
+    }
+    for (int q = 0; q < 4; q++) {
+        if (ci[q] < ng || ci[q] >= L.nx + ng || cj[q] < ng || cj[q] >= L.ny + ng) continue;
+        const uint8_t tq = g.type[L.at(ci[q], cj[q])];
+        if ((tq == 0 || tq == 3) && g.dcut[cell_q(L, ci[q], cj[q]) * 8] >= 0.5) {
+            cc[7] = (double)cell_q(L, ci[q], cj[q]);
+            return;
+        }
+    }
+    g.type[o] = 1;  // no partner: inactive
+}
+
+// inactive cells within ng of an active cell along an axis are ghosts
+__global__ void k_cut_ghosts(GeomArgs g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const Layout L = g.L;
+    const int ng = L.ng;
+    if (i < ng || i >= L.nx + ng || j < ng || j >= L.ny + ng) return;
+    const long long o = L.at(i, j);
+    if (g.type[o] != 1) return;
+    for (int k = 1; k <= ng; k++) {
+        const int ci[4] = {i - k, i + k, i, i}, cj[4] = {j, j, j - k, j + k};
+        for (int q = 0; q < 4; q++) {
+            if (ci[q] < ng || ci[q] >= L.nx + ng || cj[q] < ng || cj[q] >= L.ny + ng) continue;
+            const uint8_t tq = g.type[L.at(ci[q], cj[q])];
+            if (tq == 0 || tq == 3) { g.type[o] = 2; return; }
+        }
+    }
+}
+
+// compaction: records of the type-3 cells, and merge groups gathered by each
+// target from its W, E, S, N neighbours (counting only when the capacity is 0)
+__global__ void k_cut_compact(GeomArgs g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const Layout L = g.L;
+    const int ng = L.ng;
+    if (i < ng || i >= L.nx + ng || j < ng || j >= L.ny + ng) return;
+    const long long o = L.at(i, j);
+    const uint8_t t = g.type[o];
+    const double* cc = g.dcut + cell_q(L, i, j) * 8;
+    if (t == 3) {
+        const unsigned int slot = atomicAdd(g.ccount, 1u);
+        if (slot < (unsigned int)g.ccap) {
+            CutCell r;
+            r.alpha = cc[0]; r.aw = cc[1]; r.ae = cc[2]; r.as = cc[3]; r.an = cc[4]; r.sbx = cc[5]; r.sby = cc[6];
+            r.target = cc[7] < 0 ? -1 : L.at((int)((long long)cc[7] % L.M), (int)((long long)cc[7] / L.M));
+            g.cuts[slot] = r;
+            g.cut_index[o] = (int)slot;
+        }
+    }
+    if ((t == 0 || t == 3) && cc[0] >= 0.5) {
+        const int ni[4] = {i - 1, i + 1, i, i}, nj[4] = {j, j, j - 1, j + 1};
+        MergeGroup gr;
+        gr.n = 0;
+        gr.pad = 0;
+        gr.target = o;
+        gr.at = cc[0];
+        const double me = (double)cell_q(L, i, j);
+        for (int q = 0; q < 4; q++) {
+            const long long nq = L.at(ni[q], nj[q]);
+            const double* cn = g.dcut + cell_q(L, ni[q], nj[q]) * 8;
+            if (g.type[nq] == 3 && cn[7] == me) {
+                gr.member[gr.n] = nq;
+                gr.am[gr.n] = cn[0];
+                gr.n++;
+            }
+        }
+        for (int q = gr.n; q < 4; q++) { gr.member[q] = o; gr.am[q] = 0.0; }
+        if (gr.n > 0) {
+            const unsigned int slot = atomicAdd(g.gcount, 1u);
+            if (slot < (unsigned int)g.gcap) g.groups[slot] = gr;
+        }
+    }
+}
+
+hipError_t cut_run(GeomArgs& g, hipStream_t s) {
+    dim3 blk(128), grd((g.L.M + 127) / 128, g.L.N);
+    hipLaunchKernelGGL(k_cut_cells, grd, blk, 0, s, g);
+    hipLaunchKernelGGL(k_cut_target, grd, blk, 0, s, g);
+    hipLaunchKernelGGL(k_cut_ghosts, grd, blk, 0, s, g);
+    return hipGetLastError();
+}
+hipError_t cut_compact(GeomArgs& g, hipStream_t s) {
+    dim3 blk(128), grd((g.L.M + 127) / 128, g.L.N);
+    hipLaunchKernelGGL(k_cut_compact, grd, blk, 0, s, g);
+    return hipGetLastError();
+}
 
 // ------------------------------------------------------------ mask level set
 // Test/PNGGeo/Level_Set.cpp:31-149 on the device. The reference computes, for

@@ -40,6 +40,23 @@ struct Ctl {
     double log_dt[LOGCAP], log_t[LOGCAP], log_mass[LOGCAP];
 };
 
+// Cut-cell record (cut cells only; no reference counterpart): volume fraction,
+// face apertures W, E, S, N, wall vector (the closure of the faces) and the
+// merge target of a small cell (-1: none).
+struct CutCell {
+    double alpha, aw, ae, as, an, sbx, sby;
+    long long target;
+};
+// A merge group: a target cell and the small cut cells that chose it
+// (neighbours W, E, S, N in that order), with their volume fractions.
+struct MergeGroup {
+    long long target;
+    long long member[4];
+    double at;
+    double am[4];
+    int n, pad;
+};
+
 struct StepArgs {
     double* buf[2];     // state buffers
     Layout L;
@@ -57,6 +74,19 @@ struct StepArgs {
     int nbx, nby;           // grid
     int stage;              // 0: single-step update; RK2 (Heun): 1 = U1 = U^n + dt L(U^n),
                             // 2 = (U^n + U1 + dt L(U1)) / 2, written over U^n in place
+    const CutCell* cut;     // cut-cell records (type-3 cells), else null
+    const int* cut_index;   // plane: record of each type-3 cell
+};
+
+// Small-cell merging after a cut-cell step (one thread per group).
+struct MergeArgs {
+    double* buf[2];
+    long long plane;
+    Ctl* ctl;
+    const MergeGroup* groups;
+    int ngroups;
+    double gm1, r_gm1;
+    int quirks;
 };
 
 // Ghost-cell records (compacted list of interior cells of type 2), built once.
@@ -101,7 +131,11 @@ struct EdgeArgs {
 };
 
 struct ReduceArgs {
-    const double* U;  // state buffer
+    const double* U;  // state buffer (when from_step = 0)
+    double* buf[2];   // from_step = 1: the buffer the step just wrote, buf[ctl->cur ^ 1]
+    int from_step;
+    const CutCell* cut;  // cut cells count with their volume fraction
+    const int* cut_index;
     Layout L;
     const uint8_t* type;
     Ctl* ctl;
@@ -141,6 +175,8 @@ hipError_t launch_fill(const FillArgs& a, hipStream_t s);
 hipError_t launch_scatter(const FillArgs& a, hipStream_t s);
 hipError_t launch_edge(const EdgeArgs& a, hipStream_t s);
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
+hipError_t launch_reduce_partials(const ReduceArgs& a, hipStream_t s);  // without the finish
+hipError_t launch_merge(const MergeArgs& a, hipStream_t s);
 hipError_t launch_finish(const FinishArgs& f, hipStream_t s);
 // batched forms: one launch over many solves (maps: block -> {slot, first index})
 hipError_t launch_step_batch(int scheme, int rs, int lim, int prec, const StepArgs* slots, const int2* map,

@@ -476,6 +476,22 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
                         }
                     }
                     msum += un[0] * dx * dy;
+                } else if (a.cut && a.type[o] == 3) {
+                    // cut cell: aperture-weighted face fluxes and the wall's pressure
+                    // force S p (S closes the cell's faces), over the fluid volume
+                    // alpha dx dy; small cells are merged (and floored) afterwards
+                    const CutCell cc = a.cut[a.cut_index[o]];
+                    const double rho = w[0][0], u0 = w[0][1] / rho, v0 = w[0][2] / rho;
+                    const double pw = a.gm1 * (w[0][3] - 0.5 * rho * (u0 * u0 + v0 * v0));
+                    const double wall[4] = {0.0, pw * cc.sbx, pw * cc.sby, 0.0};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const double Fk = (double)F[k], Fm = (double)fb[k][lane - 1];
+                        const double Gk = (double)G[k], Gm = (double)Gp[k];
+                        double v = w[0][k] - dt * ((cc.ae * Fk - cc.aw * Fm) * dy) / (cc.alpha * dx * dy);
+                        un[k] = v - dt * ((cc.an * Gk - cc.as * Gm) * dx + wall[k]) / (cc.alpha * dx * dy);
+                    }
+                    if (cc.alpha >= 0.5) floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, amax, umax, vmax, bad);
                 } else {
 #pragma unroll
                     for (int k = 0; k < 4; k++) un[k] = w[0][k];
@@ -594,19 +610,51 @@ __global__ __launch_bounds__(64) void k_step_lax(StepArgs a) {
 // CaculateMass.cpp:27-38); with `reset`, also starts the run clock.
 __global__ __launch_bounds__(64) void k_reduce(ReduceArgs a) {
     const Layout L = a.L;
+    if (a.from_step && a.ctl->done) return;  // no step was taken
+    const double* Us = a.from_step ? a.buf[a.ctl->cur ^ 1] : a.U;
     const int nint = L.nx * L.ny;
     double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
     int bad = 0;
     for (int q = blockIdx.x * 64 + threadIdx.x; q < nint; q += a.nblocks * 64) {
         int i = L.ng + q % L.nx, j = L.ng + q / L.nx;
         long long o = L.at(i, j);
-        if (a.type[o] == 0) {
-            double U[4] = {a.U[o], a.U[o + L.plane], a.U[o + 2 * L.plane], a.U[o + 3 * L.plane]};
+        const uint8_t t = a.type[o];
+        if (t == 0 || (t == 3 && a.cut)) {
+            double U[4] = {Us[o], Us[o + L.plane], Us[o + 2 * L.plane], Us[o + 3 * L.plane]};
             dt_terms(a.gam, a.gm1, U, amax, umax, vmax, bad);
-            msum += U[0] * a.dx * a.dy;
+            if (t == 0) msum += U[0] * a.dx * a.dy;
+            else msum += a.cut[a.cut_index[o]].alpha * U[0] * a.dx * a.dy;
         }
     }
     block_partials(a.mass_part, blockIdx.x, amax, umax, vmax, msum, bad);
+}
+
+// Small-cell merging (cut cells): each group's target and members take the
+// volume-fraction-weighted mean of their updated states; the corrected-mode
+// floors, which the small cells skipped in the step, apply to the mean.
+__global__ void k_merge(MergeArgs a) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= a.ngroups || a.ctl->done) return;
+    const MergeGroup gr = a.groups[gi];
+    double* U = a.buf[a.ctl->cur ^ 1];
+    const long long PL = a.plane;
+    double asum = gr.at;
+    for (int q = 0; q < gr.n; q++) asum = asum + gr.am[q];
+    double v[4];
+    for (int k = 0; k < 4; k++) {
+        double s = gr.at * U[gr.target + k * PL];
+        for (int q = 0; q < gr.n; q++) s = s + gr.am[q] * U[gr.member[q] + k * PL];
+        v[k] = s / asum;
+    }
+    if (!a.quirks) {
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+        int db = 0;
+        floor_and_dt(a.gm1 + 1.0, a.gm1, Rcp<double>{a.gm1, a.r_gm1}, v, 0, d0, d1, d2, db);
+    }
+    for (int k = 0; k < 4; k++) {
+        U[gr.target + k * PL] = v[k];
+        for (int q = 0; q < gr.n; q++) U[gr.member[q] + k * PL] = v[k];
+    }
 }
 
 // Ghost-cell fill, phase 1: values from the donors of U_NEW (the state right
@@ -912,6 +960,15 @@ hipError_t launch_edge(const EdgeArgs& a, hipStream_t s) {
 hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_from_input, hipStream_t s) {
     int nring = a.L.M * a.L.N - a.L.nx * a.L.ny;
     hipLaunchKernelGGL(k_edge_ref, dim3((nring + 255) / 256), dim3(256), 0, s, a, corner, corner_from_input);
+    return hipGetLastError();
+}
+hipError_t launch_reduce_partials(const ReduceArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce, dim3(a.nblocks), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_merge(const MergeArgs& a, hipStream_t s) {
+    if (a.ngroups == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_merge, dim3((a.ngroups + 127) / 128), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {

@@ -46,7 +46,9 @@ enum ibtfo_riemann { IBTFO_HLL = 0, IBTFO_HLLC = 1, IBTFO_PVRS = 2, IBTFO_STEGER
 enum ibtfo_limiter { IBTFO_MINMOD = 0, IBTFO_VANLEER = 1, IBTFO_VANALBADA = 2 };
 enum ibtfo_boundary {
     IBTFO_REGULAR = 0, IBTFO_IDW_GFM = 1, IBTFO_LINEAR_GFM = 2, IBTFO_RIEMANN_GFM = 3,
-    IBTFO_BCGFM = 4, IBTFO_HBCGFM = 5
+    IBTFO_BCGFM = 4, IBTFO_HBCGFM = 5,
+    IBTFO_CUTCELL = 6 /* cut cells with small-cell merging (no reference counterpart; corrected mode,
+                         single-step, not in batches) */
 };
 enum ibtfo_time { IBTFO_SINGLESTEP = 0, IBTFO_RK2 = 1 };
 /* Global_Boundary policies: the reference hook (Test/Sod/Global_Boudary.cpp:24-75),
