@@ -309,7 +309,7 @@ __device__ __forceinline__ void cell_states(const double* um, const double* u0, 
 // The step body for one 64-lane block: strip (bx, by) of the solve described
 // by `a`; `stage` as StepArgs::stage. Used by the single-solve kernel and by the
 // batched kernel, whose blocks each pick their solve from a block map.
-template <int SCH, int RS, int LIM, typename T>
+template <int SCH, int RS, int LIM, typename T, bool CUT>
 __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, const int by, const int stage) {
     constexpr bool FP64 = sizeof(T) == sizeof(double);
     Ctl* ctl = a.ctl;
@@ -476,7 +476,7 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
                         }
                     }
                     msum += un[0] * dx * dy;
-                } else if (a.cut && a.type[o] == 3) {
+                } else if (CUT && a.type[o] == 3) {
                     // cut cell: aperture-weighted face fluxes and the wall's pressure
                     // force S p (S closes the cell's faces), over the fluid volume
                     // alpha dx dy; small cells are merged (and floored) afterwards
@@ -538,9 +538,9 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
     block_partials(a.mass_part, by * a.nbx + bx, amax, umax, vmax, msum, bad);
 }
 
-template <int SCH, int RS, int LIM, typename T>
+template <int SCH, int RS, int LIM, typename T, bool CUT>
 __global__ __launch_bounds__(64) void k_step(StepArgs a) {
-    step_strip<SCH, RS, LIM, T>(a, blockIdx.x, blockIdx.y, a.stage);
+    step_strip<SCH, RS, LIM, T, CUT>(a, blockIdx.x, blockIdx.y, a.stage);
 }
 
 // Batched step: one launch over the strips of many solves. Block b runs strip
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(64) void k_step_batch(const StepArgs* __restrict__ 
     const int2 m = map[blockIdx.x];
     if (m.x < 0) return;
     const StepArgs& a = slots[m.x];
-    step_strip<SCH, RS, LIM, T>(a, m.y % a.nbx, m.y / a.nbx, stage);
+    step_strip<SCH, RS, LIM, T, false>(a, m.y % a.nbx, m.y / a.nbx, stage);
 }
 
 // Lax-Friedrichs, Lax_Scheme.cpp:27-62: U_new = average of the 4 neighbours minus
@@ -872,9 +872,12 @@ static hipError_t go(int prec, const StepArgs* a, const BatchLaunch* b, hipStrea
         else
             hipLaunchKernelGGL((k_step_batch<SCH, RS, LIM, double>), dim3(b->nblocks), dim3(64), 0, s, b->slots,
                                b->map, b->stage);
+    } else if (a->cut) {  // cut-cell variant (its extra registers stay out of the plain kernel)
+        if (prec == 1) hipLaunchKernelGGL((k_step<SCH, RS, LIM, float, true>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
+        else hipLaunchKernelGGL((k_step<SCH, RS, LIM, double, true>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
     } else {
-        if (prec == 1) hipLaunchKernelGGL((k_step<SCH, RS, LIM, float>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
-        else hipLaunchKernelGGL((k_step<SCH, RS, LIM, double>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
+        if (prec == 1) hipLaunchKernelGGL((k_step<SCH, RS, LIM, float, false>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
+        else hipLaunchKernelGGL((k_step<SCH, RS, LIM, double, false>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
     }
     return hipGetLastError();
 }
