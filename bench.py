@@ -2,14 +2,18 @@
 """Benchmark: FP64 cell-updates/s of the Euler step on one MI355X.
 
 Workload: 4096 x 2048 cells on [0,2] x [0,1], Mach 3 inflow (rho, u, v, p) =
-(1.4, 3, 0, 1) past 9 seeded bodies (seed 20261202: circles and convex polygons,
-exact signed-distance level set), MUSCL + van Leer limiter + HLL flux, IDW
-ghost-fluid boundary, supersonic inflow / zero-gradient edges, FP64. A step =
-the step kernel (reconstruction, fluxes, update, per-block dt/mass partials),
-the one-block dt/clock kernel, the two ghost-fill kernels and the edge kernel.
-The state is initialised on the device before the timed region and dt never
-leaves the device. The step kernel's tile height is picked by the library's
-autotuner; the timed loop replays a captured HIP graph of the step sequence.
+(1.4, 3, 0, 1) past 9 seeded bodies (seed 20261202: circles and convex
+polygons) given as a binary mask image (one pixel per cell), whose level set is
+built on the device (mask -> edge pixels -> signed distance), cut-cell boundary
+with small-cell merging, MUSCL + van Leer limiter, HLL flux (the headline) and
+Steger-Warming flux (second leg), supersonic inflow / zero-gradient edges, FP64.
+A step = the step kernel (reconstruction, fluxes, update, per-block dt/mass
+partials), the small-cell merge kernel, the post-merge dt/mass partials of the
+cut cells, the one-block dt/clock kernel, the two ghost-fill kernels and the
+edge kernel. The state is initialised on the device before the timed region and
+dt never leaves the device. The step kernel's tile height is picked by the
+library's autotuner; the timed loop replays a captured HIP graph of the step
+sequence.
 
 Prints one JSON line. `value` = N_x*N_y*steps / t_loop (HIP events around K
 steps on the solve's stream). `roofline` is for the step kernel alone: 65
@@ -31,19 +35,45 @@ sys.path.insert(0, ROOT)
 
 BYTES_PER_CELL = 65.0
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = "traffic_step_mach3_mask_cutcell.json"
 
 
-def gpu_leg(args, precision="FP64", keep_state=False):
+def workload_name(args):
+    geo = ("binary mask, level set built on the device" if args.geometry == "mask"
+           else "exact signed-distance level set")
+    return ("Mach 3 inflow past 9 seeded bodies (seed 20261202; %s), %dx%d cells on [0,2]x[0,1], "
+            "MUSCL + %s + %s, %s, inflow/zero-gradient edges, FP64"
+            % (geo, args.nx, args.ny, args.limiter, args.riemann, args.boundary))
+
+
+_MASK = {}
+
+
+def workload_geometry(args, s):
+    """The bodies on a solver: the binary mask (level set built on the device)
+    or the exact signed distance of the same shapes."""
+    from ibtfo_amd import workloads as W
+    if args.geometry == "mask":
+        key = (args.nx, args.ny)
+        if key not in _MASK:
+            _MASK[key] = W.bodies_mask(args.nx, args.ny)
+        s.geometry_mask(_MASK[key])
+    else:
+        s.geometry("shapes", W.bodies_shapes())
+
+
+def gpu_leg(args, precision="FP64", keep_state=False, riemann=None):
     import ibtfo_amd as ib
     from ibtfo_amd import workloads as W
-    cfg = W.mach3_bodies_config(ib, N_x=args.nx, N_y=args.ny, Riemann=args.riemann, Limiter=args.limiter,
-                                Boundary=args.boundary)
+    cfg = W.mach3_bodies_config(ib, N_x=args.nx, N_y=args.ny, Riemann=riemann or args.riemann,
+                                Limiter=args.limiter, Boundary=args.boundary)
     cfg.precision = ib.PRECISION[precision]
     s = ib.Solver(cfg)
-    s.geometry("shapes", W.bodies_shapes())
+    workload_geometry(args, s)
     s.initialize("uniform", W.MACH3_INFLOW)
     s.prepare(True)
     nghost = s.num_ghosts()
+    ncut, ngroups = s.cut_counts()
     tuned = None
     if args.rows > 0:
         s.set_rows(args.rows)
@@ -63,12 +93,14 @@ def gpu_leg(args, precision="FP64", keep_state=False):
     value = cells * args.steps / (total_ms / 1e3)
     achieved = BYTES_PER_CELL * cells / (kern_ms / 1e3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_step_mach3_bodies.json")
-    if os.path.exists(tpath):
+    tpath = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
+    if os.path.exists(tpath) and precision == "FP64" and riemann is None:
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tr = json.load(f)
+        if tr.get("workload") == workload_name(args):
+            traffic = tr.get("bytes_per_launch")
     out = dict(value=value, ms_per_step=ms_step, kernel_ms=kern_ms, achieved=achieved, traffic=traffic,
-               iter=it, t=t, dt=dt, status=rc, nghost=nghost, rows=rows, grid=[nbx, nby],
+               iter=it, t=t, dt=dt, status=rc, nghost=nghost, ncut=ncut, ngroups=ngroups, rows=rows, grid=[nbx, nby],
                tuned=tuned is not None)
     if keep_state:
         out["rho"] = s.state()[0::4].copy()
@@ -91,12 +123,15 @@ def cpu_leg(args):
                   CFL_number=0.5, Scheme="MUSCL", Riemann=args.riemann, Limiter=args.limiter,
                   Boundary=args.boundary, case="uniform", edge="inflow", quirks=0,
                   extra=dict(rho=1.4, u=3.0, v=0.0, p=1.0))
-    M, N = case.M, case.N
-    dx = 2.0 / nx
-    Li = -dx * 2 + 0.0 + dx / 2
-    x = (Li + np.arange(M) * dx)[None, :].repeat(N, 0)
-    y = (Li + np.arange(N) * dx)[:, None].repeat(M, 1)
-    case.phi = W.shapes_phi(W.bodies_shapes(), x, y).reshape(-1)
+    if args.geometry == "mask":
+        case.phi = O.mask_phi(case, W.bodies_mask(nx, ny))
+    else:
+        M, N = case.M, case.N
+        dx = 2.0 / nx
+        Li = -dx * 2 + 0.0 + dx / 2
+        x = (Li + np.arange(M) * dx)[None, :].repeat(N, 0)
+        y = (Li + np.arange(N) * dx)[:, None].repeat(M, 1)
+        case.phi = W.shapes_phi(W.bodies_shapes(), x, y).reshape(-1)
     st = O.setup(case)
     t0 = time.perf_counter()
     r = O.run_oracle(case, steps, state=st)
@@ -235,7 +270,7 @@ def pmc_probe(args):
     cfg = W.mach3_bodies_config(ib, N_x=args.nx, N_y=args.ny, Riemann=args.riemann, Limiter=args.limiter,
                                 Boundary=args.boundary)
     s = ib.Solver(cfg)
-    s.geometry("shapes", W.bodies_shapes())
+    workload_geometry(args, s)
     s.initialize("uniform", W.MACH3_INFLOW)
     s.prepare(True)
     if args.rows > 0:
@@ -298,6 +333,19 @@ def pmc_summary(args):
                bytes_per_launch=f_step * 1024.0 * sf + w_step * 1024.0 * sw,
                algorithmic_bytes_per_launch=BYTES_PER_CELL * cells, cells=cells)
     out["bytes_per_cell"] = out["bytes_per_launch"] / cells
+    out["workload"] = workload_name(args)
+    out["tile_rows"] = args.rows or "default"
+    rows = " --rows %d" % args.rows if args.rows else ""
+    out["commands"] = [
+        "cd /tmp && export TMPDIR=/tmp",
+        "timeout -s KILL 120 rocprofv3 --pmc FETCH_SIZE -d $REPO/gpurun_out/pmc_fetch -o fetch --output-format csv "
+        "-- python3 $REPO/bench.py --pmc-probe --steps 5" + rows,
+        "timeout -s KILL 120 rocprofv3 --pmc WRITE_SIZE -d $REPO/gpurun_out/pmc_write -o write --output-format csv "
+        "-- python3 $REPO/bench.py --pmc-probe --steps 5" + rows,
+        "python3 bench.py --pmc-fetch gpurun_out/pmc_fetch --pmc-write gpurun_out/pmc_write --pmc-out "
+        "profiles/%s%s" % (TRAFFIC_FILE, rows)]
+    out["note"] = ("FETCH_SIZE / WRITE_SIZE are KiB per dispatch (mean over the step-kernel dispatches); each is "
+                   "scaled by copy_bytes / (1024 * the same counter for a 256 MiB elementwise copy)")
     print(json.dumps(out, indent=1))
     if args.pmc_out:
         with open(args.pmc_out, "w") as f:
@@ -314,7 +362,10 @@ def main():
     ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--riemann", default="HLL")
     ap.add_argument("--limiter", default="VanLeer")
-    ap.add_argument("--boundary", default="IDWBaseGFM")
+    ap.add_argument("--boundary", default="CutCell")
+    ap.add_argument("--geometry", default="mask", choices=["mask", "shapes"],
+                    help="bodies as a binary mask (level set on the device) or as exact signed distances")
+    ap.add_argument("--no-sw", action="store_true", help="skip the Steger-Warming leg")
     ap.add_argument("--rows", type=int, default=0, help="fixed rows per strip (skips the autotuner)")
     ap.add_argument("--no-tune", action="store_true", help="default tile shape, no autotuning")
     ap.add_argument("--tune-steps", type=int, default=20)
@@ -359,6 +410,13 @@ def main():
                  "kernel_ms": m["kernel_ms"], "speedup_vs_fp64": m["value"] / g["value"],
                  "l1_rho_vs_fp64": l1, "l1_rho_vs_fp64_rel": l1 / scale, "steps_compared": m["iter"],
                  "tile_rows": m["rows"]}
+    sw = None
+    if not args.no_sw and args.riemann != "StegerWarming":
+        w = gpu_leg(args, riemann="StegerWarming")
+        sw = {"value": w["value"], "unit": "cell-updates/s", "ms_per_step": w["ms_per_step"],
+              "kernel": "k_step<MUSCL,StegerWarming,VanLeer>", "kernel_ms": w["kernel_ms"],
+              "roofline_frac": w["achieved"] / HBM_PEAK_GBS, "tile_rows": w["rows"], "steps_taken": w["iter"],
+              "status": w["status"]}
     serving = None if args.no_serve else serving_leg(args)
     cpu = None
     if not args.no_cpu:
@@ -377,10 +435,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "Mach 3 inflow past 9 seeded bodies (seed 20261202), %dx%d cells on [0,2]x[0,1], "
-                               "MUSCL + %s + %s, %s, inflow/zero-gradient edges, FP64"
-                               % (args.nx, args.ny, args.limiter, args.riemann, args.boundary),
-                   "N_x": args.nx, "N_y": args.ny, "ghost_cells": g["nghost"], "steps_taken": g["iter"],
+        "config": {"workload": workload_name(args),
+                   "N_x": args.nx, "N_y": args.ny, "ghost_cells": g["nghost"], "cut_cells": g["ncut"], "merge_groups": g["ngroups"],
+                   "steps_taken": g["iter"], "status": g["status"],
                    "tile": {"columns": 62, "rows": g["rows"], "grid": g["grid"], "autotuned": g["tuned"]},
                    "launch": "hipGraph of %d steps" % args.graph_chunk if args.graph_chunk > 0 else "stream",
                    "parallelism": "single GPU, whole domain resident"},
@@ -389,6 +446,7 @@ def main():
                      "kernel": "k_step<MUSCL,HLL,VanLeer>", "kernel_ms": g["kernel_ms"],
                      "bytes_per_cell": BYTES_PER_CELL},
         "cpu_baseline": cpu,
+        "steger_warming": sw,
         "mixed_precision": mixed,
         "serving": serving,
     }

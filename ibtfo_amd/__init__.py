@@ -63,7 +63,7 @@ def lib():
                      "ibtfo_download_scheme_index", "ibtfo_sch_width", "ibtfo_num_ghosts", "ibtfo_dims",
                      "ibtfo_time_steps", "ibtfo_device_state", "ibtfo_set_stream", "ibtfo_set_rows",
                      "ibtfo_tile", "ibtfo_autotune", "ibtfo_geometry_mask", "ibtfo_geometry_png",
-                     "ibtfo_png_decode"]:
+                     "ibtfo_png_decode", "ibtfo_cut_counts"]:
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -205,6 +205,12 @@ class Solver:
 
     def num_ghosts(self):
         return lib().ibtfo_num_ghosts(self.h)
+
+    def cut_counts(self):
+        """(cut cells, small-cell merge groups) of the cut-cell boundary."""
+        a, b = C.c_int(), C.c_int()
+        _check(lib().ibtfo_cut_counts(self.h, C.byref(a), C.byref(b)), "ibtfo_cut_counts")
+        return a.value, b.value
 
     def compute_dt(self):
         v = C.c_double()

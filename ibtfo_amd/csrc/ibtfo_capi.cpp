@@ -128,7 +128,8 @@ void ibtfo::free_ctx(ibtfo_ctx* x) {
     if (x->graph) (void)hipGraphExecDestroy(x->graph);
     for (void* p : {(void*)x->state[0], (void*)x->state[1], (void*)x->type, (void*)x->phi, (void*)x->gnx,
                     (void*)x->gny, (void*)x->ghosts, (void*)x->gvals, (void*)x->dcount, (void*)x->ctl,
-                    (void*)x->mass_part, (void*)x->corner, (void*)x->cuts, (void*)x->cut_index, (void*)x->groups})
+                    (void*)x->mass_part, (void*)x->corner, (void*)x->cuts, (void*)x->cut_index, (void*)x->groups,
+                    (void*)x->rblocks, (void*)x->fold})
         dfree(x, p);
     if (x->own_stream) (void)hipStreamDestroy(x->own_stream);
     delete x;
@@ -162,6 +163,7 @@ int ctx_create(const ibtfo_config* cfg, Arena* arena, hipStream_t stream, int ro
     if (dalloc(x, &x->gnx, L.plane * sizeof(double)) != hipSuccess) return fail("normals");
     if (dalloc(x, &x->gny, L.plane * sizeof(double)) != hipSuccess) return fail("normals");
     if (dalloc(x, &x->ctl, sizeof(Ctl)) != hipSuccess) return fail("control");
+    if (dalloc(x, &x->fold, FOLD_MAX * PART_STRIDE * sizeof(double)) != hipSuccess) return fail("partials");
     if (dalloc(x, &x->dcount, sizeof(unsigned int)) != hipSuccess) return fail("counter");
     int nbx, nby, r;
     step_grid(L, rows, &nbx, &nby, &r);
@@ -228,6 +230,12 @@ extern "C" int ibtfo_sch_width(const ibtfo_ctx* x) {
     }
 }
 extern "C" int ibtfo_num_ghosts(const ibtfo_ctx* x) { return x ? x->nghost : -1; }
+extern "C" int ibtfo_cut_counts(const ibtfo_ctx* x, int* cut_cells, int* merge_groups) {
+    if (!x) { set_error("null context"); return IBTFO_E_ARG; }
+    if (cut_cells) *cut_cells = x->ncut;
+    if (merge_groups) *merge_groups = x->ngroups;
+    return IBTFO_OK;
+}
 
 // --------------------------------------------------------------- geometry
 // Runs the geometry kernels for a prepared GeomArgs (hook constants and inputs
@@ -287,7 +295,62 @@ static int cut_geometry(ibtfo_ctx* x, GeomArgs& g) {
     g.dcut = nullptr;
     if (rc) return rc;
     HIPCHK(e);
+    // cells the merge may change or that carry a volume fraction: every cut cell
+    // and every group member, as (i, j), for the post-merge partials' block list
+    x->touched_ij.clear();
+    dfree(x, x->rblocks); x->rblocks = nullptr;
+    x->nrblocks = x->rb_cap = 0;
+    x->rb_nbx = x->rb_rows = -1;
+    if (x->ncut) {
+        std::vector<uint8_t> tp(L.plane);
+        std::vector<MergeGroup> gr(x->ngroups);
+        HIPCHK(scopy(x, tp.data(), x->type, L.plane, hipMemcpyDeviceToHost));
+        if (x->ngroups) HIPCHK(scopy(x, gr.data(), x->groups, x->ngroups * sizeof(MergeGroup), hipMemcpyDeviceToHost));
+        auto add = [&](long long o) {
+            const long long r = o - L.off;
+            x->touched_ij.push_back((int)(r % L.pitch));
+            x->touched_ij.push_back((int)(r / L.pitch));
+        };
+        for (int j = L.ng; j < L.ny + L.ng; j++)
+            for (int i = L.ng; i < L.nx + L.ng; i++)
+                if (tp[L.at(i, j)] == 3) add(L.at(i, j));
+        for (const MergeGroup& m : gr) {
+            add(m.target);
+            for (int q = 0; q < m.n; q++) add(m.member[q]);
+        }
+        x->rb_cap = (int)(x->touched_ij.size() / 2);
+        if (dalloc(x, &x->rblocks, x->rb_cap * sizeof(int)) != hipSuccess) {
+            set_error("cut-cell geometry: out of device memory");
+            return IBTFO_E_NOMEM;
+        }
+    }
     return IBTFO_OK;
+}
+
+// The step blocks holding a touched cell (cut cell or merge-group member) for
+// the current step grid, uploaded when the grid changed. Set-up time only (it
+// synchronises the stream); a failure is reported by the next step.
+static void cut_blocks(ibtfo_ctx* x) {
+    const StepArgs& a = x->sa;
+    if (x->cfg.boundary != IBTFO_CUTCELL || (a.nbx == x->rb_nbx && a.rows == x->rb_rows)) return;
+    std::vector<char> mark((size_t)a.nbx * a.nby, 0);
+    std::vector<int> list;
+    const Layout& L = x->L;
+    for (size_t q = 0; q + 1 < x->touched_ij.size(); q += 2) {
+        const int i = x->touched_ij[q], j = x->touched_ij[q + 1];
+        if (i < L.ng || i >= L.nx + L.ng || j < L.ng || j >= L.ny + L.ng) continue;  // ring cells: no partials
+        const int b = ((j - L.ng) / a.rows) * a.nbx + (i - L.ng) / STRIP_W;
+        if (!mark[b]) { mark[b] = 1; list.push_back(b); }
+    }
+    x->rb_error = 0;
+    if ((int)list.size() > x->rb_cap) { x->rb_error = 1; return; }
+    if (!list.empty() && scopy(x, x->rblocks, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        x->rb_error = 1;
+        return;
+    }
+    x->nrblocks = (int)list.size();
+    x->rb_nbx = a.nbx;
+    x->rb_rows = a.rows;
 }
 
 static int geometry_finish(ibtfo_ctx* x, GeomArgs& g, double* dphi) {
@@ -667,6 +730,7 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     fn.ctl = x->ctl; fn.part = x->mass_part; fn.nblocks = a.nbx * a.nby;
     fn.cfl = c.CFL_number; fn.dx = a.dx; fn.t_end = c.Psy_time; fn.max_iter = c.max_iter;
     fn.advance = 1; fn.reset = 0;
+    fn.fold = x->fold;
 
     ReduceArgs& q = x->ra;
     memset(&q, 0, sizeof q);
@@ -685,6 +749,7 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     mg.plane = L.plane;
     mg.ctl = x->ctl; mg.groups = x->groups; mg.ngroups = x->ngroups;
     mg.gm1 = c.gamma - 1; mg.r_gm1 = 1.0 / (c.gamma - 1); mg.quirks = c.quirks;
+    cut_blocks(x);
 }
 
 // Boundary and Global_Boundary after an update (the state just made current):
@@ -730,12 +795,18 @@ static int enqueue_step(ibtfo_ctx* x, hipEvent_t* ev = nullptr) {
     if (c.boundary == IBTFO_CUTCELL) {
         // small cells merged, then ComputeDt / mass over the merged state
         HIPCHK(launch_merge(x->ma, x->stream));
+        if (x->rb_error || x->rb_nbx != x->sa.nbx || x->rb_rows != x->sa.rows) {
+            set_error("cut-cell block list not built for the current step grid");
+            return IBTFO_E_STATE;
+        }
         ReduceArgs r = x->ra;
         r.from_step = 1;
+        r.blocks = x->rblocks;
+        r.nblocks = x->nrblocks;
+        r.nbx = x->sa.nbx;
+        r.rows = x->sa.rows;
         HIPCHK(launch_reduce_partials(r, x->stream));
-        FinishArgs f = x->fin;
-        f.nblocks = r.nblocks;
-        HIPCHK(launch_finish(f, x->stream));
+        HIPCHK(launch_finish(x->fin, x->stream));
         return enqueue_boundaries(x);
     }
     HIPCHK(launch_finish(x->fin, x->stream));

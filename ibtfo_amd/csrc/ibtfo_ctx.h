@@ -26,6 +26,7 @@ struct ibtfo_ctx {
     ibtfo::Ctl* ctl;
     double* mass_part;  // PART_STRIDE doubles per step-kernel block
     int mass_cap;       // blocks
+    double* fold;       // FOLD_MAX records of the two-level finish
     int rows_override;  // step-kernel rows per strip (0 = default shape)
     double* corner;
     double Li, Hi, dxc;  // cell-centre origin and spacing (Psy_coord)
@@ -40,6 +41,11 @@ struct ibtfo_ctx {
     int* cut_index;
     ibtfo::MergeGroup* groups;
     int ncut, ngroups;
+    // step blocks whose partials the merge invalidates (they hold a cut cell or a
+    // group member), for the current step grid; rebuilt when the grid changes
+    std::vector<int> touched_ij;  // (i, j) of every cut cell and group member
+    int* rblocks;
+    int nrblocks, rb_cap, rb_nbx, rb_rows, rb_error;
     hipGraphExec_t graph;
     int graph_chunk;
     std::vector<double> corner_host;

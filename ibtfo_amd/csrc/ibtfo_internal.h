@@ -144,6 +144,9 @@ struct ReduceArgs {
     int max_iter;
     int nblocks;
     int reset;  // 1: set iter = 0, now_t = 0 (start of a run)
+    // from_step = 1: recompute the partials of the listed step blocks only
+    const int* blocks;
+    int nbx, rows;
 };
 
 // the dt / mass reduction over the step kernel's per-block partials
@@ -158,7 +161,14 @@ struct FinishArgs {
     int advance;  // 1: a step was taken (advance the clock); 0: stand-alone ComputeDt;
                   // 2: RK2 stage boundary (make the stage-1 buffer current, no reduction)
     int reset;    // with advance = 0: start the run clock
+    // two-level reduction (advance = 1 with many partials): nfold blocks each fold
+    // `fold_per` consecutive partials into fold[] first, then one block finishes
+    double* fold;
+    int nfold, fold_per;
 };
+constexpr int FOLD_THREADS = 256;
+constexpr int FOLD_MAX = 128;      // fold records
+constexpr int FOLD_MIN_PARTS = 2048;  // fewer partials: one block reads them all
 
 constexpr int FILL_THREADS = 128;  // ghost-fill and scatter blocks
 constexpr int EDGE_THREADS = 256;  // edge-ring blocks

@@ -116,14 +116,26 @@ __device__ __forceinline__ void finish_body(const FinishArgs& f, const int advan
     const int t = threadIdx.x;
     double A = 0.0, Um = 0.0, Vm = 0.0, m = 0.0;
     int bad = 0;
-    for (int q = t; q < f.nblocks; q += FINISH_THREADS) {
-        const double* r = f.part + PART_STRIDE * (long long)q;
-        double a2 = r[0], u = r[1], v = r[2];
-        A = (A < a2) ? a2 : A;
-        Um = (Um < u) ? u : Um;
-        Vm = (Vm < v) ? v : Vm;
-        m += r[3];
-        bad |= r[4] != 0.0;
+    // batches of FB records per thread, all loads issued before any is used, so
+    // one memory round trip covers FB records (the loop is latency bound)
+    constexpr int FB = 8;
+    for (int q0 = t; q0 < f.nblocks; q0 += FB * FINISH_THREADS) {
+        double ra[FB], ru[FB], rv[FB], rm[FB], rb[FB];
+#pragma unroll
+        for (int b = 0; b < FB; b++) {
+            const int q = q0 + b * FINISH_THREADS;
+            const double* r = f.part + PART_STRIDE * (long long)(q < f.nblocks ? q : q0);
+            ra[b] = r[0]; ru[b] = r[1]; rv[b] = r[2]; rm[b] = r[3]; rb[b] = r[4];
+        }
+#pragma unroll
+        for (int b = 0; b < FB; b++) {
+            if (q0 + b * FINISH_THREADS >= f.nblocks) break;
+            A = (A < ra[b]) ? ra[b] : A;
+            Um = (Um < ru[b]) ? ru[b] : Um;
+            Vm = (Vm < rv[b]) ? rv[b] : Vm;
+            m += rm[b];
+            bad |= rb[b] != 0.0;
+        }
     }
     sA[t] = A; sU[t] = Um; sV[t] = Vm; sM[t] = m; sB[t] = bad;
     __syncthreads();
@@ -170,6 +182,54 @@ __device__ __forceinline__ void finish_body(const FinishArgs& f, const int advan
     }
 }
 __global__ __launch_bounds__(FINISH_THREADS) void k_finish(FinishArgs f) { finish_body(f, f.advance); }
+
+// Partials of a 256-thread block: wave reductions, then the four waves in order.
+__device__ __forceinline__ void block4_partials(double* part, int bid, double amax, double umax, double vmax,
+                                                double msum, int bad) {
+    __shared__ double sw[4][4];
+    __shared__ int sb[4];
+    amax = wave_max(amax);
+    umax = wave_max(umax);
+    vmax = wave_max(vmax);
+    msum = wave_sum(msum);
+    bad = __any(bad);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sw[w][0] = amax; sw[w][1] = umax; sw[w][2] = vmax; sw[w][3] = msum; sb[w] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double A = sw[0][0], U = sw[0][1], V = sw[0][2], m = sw[0][3];
+        int b = sb[0];
+        for (int q = 1; q < 4; q++) {
+            A = (A < sw[q][0]) ? sw[q][0] : A;
+            U = (U < sw[q][1]) ? sw[q][1] : U;
+            V = (V < sw[q][2]) ? sw[q][2] : V;
+            m = m + sw[q][3];
+            b |= sb[q];
+        }
+        double* r = part + PART_STRIDE * (long long)bid;
+        r[0] = A; r[1] = U; r[2] = V; r[3] = m; r[4] = b ? 1.0 : 0.0;
+    }
+}
+
+// First level of the two-level finish: block g folds partials
+// [g * fold_per, (g + 1) * fold_per) into fold record g.
+__global__ __launch_bounds__(FOLD_THREADS) void k_fold(FinishArgs f) {
+    if (f.ctl->done) return;  // the step kernel did not run; k_finish returns too
+    const int q0 = blockIdx.x * f.fold_per, q1 = min(q0 + f.fold_per, f.nblocks);
+    double A = 0.0, U = 0.0, V = 0.0, m = 0.0;
+    int bad = 0;
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += FOLD_THREADS) {
+        const double* r = f.part + PART_STRIDE * (long long)q;
+        A = (A < r[0]) ? r[0] : A;
+        U = (U < r[1]) ? r[1] : U;
+        V = (V < r[2]) ? r[2] : V;
+        m += r[3];
+        bad |= r[4] != 0.0;
+    }
+    block4_partials(f.fold, blockIdx.x, A, U, V, m, bad);
+}
 
 // Batched finish: block s finishes solve s (slots with a null control block are idle).
 __global__ __launch_bounds__(FINISH_THREADS) void k_finish_batch(const FinishArgs* __restrict__ f, int nslots,
@@ -492,6 +552,8 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
                         un[k] = v - dt * ((cc.an * Gk - cc.as * Gm) * dx + wall[k]) / (cc.alpha * dx * dy);
                     }
                     if (cc.alpha >= 0.5) floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, amax, umax, vmax, bad);
+                    // (a block holding a cut cell has its partials recomputed after
+                    // the merge, cut-cell mass included)
                 } else {
 #pragma unroll
                     for (int k = 0; k < 4; k++) un[k] = w[0][k];
@@ -608,10 +670,36 @@ __global__ __launch_bounds__(64) void k_step_lax(StepArgs a) {
 
 // ComputeDt + CaculateMass of the current state (ComputeDt.cpp:46-67,
 // CaculateMass.cpp:27-38); with `reset`, also starts the run clock.
-__global__ __launch_bounds__(64) void k_reduce(ReduceArgs a) {
+__global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
     const Layout L = a.L;
-    if (a.from_step && a.ctl->done) return;  // no step was taken
-    const double* Us = a.from_step ? a.buf[a.ctl->cur ^ 1] : a.U;
+    if (a.from_step) {
+        // after the small-cell merge: the partials of the step blocks holding a
+        // cut cell or a merge-group member, over the merged state (the other
+        // blocks' partials, written by the step kernel, are unchanged by the merge)
+        if (a.ctl->done) return;  // no step was taken
+        const double* Us = a.buf[a.ctl->cur ^ 1];
+        const int bid = a.blocks[blockIdx.x];
+        const int bx = bid % a.nbx, by = bid / a.nbx;
+        const int lane = threadIdx.x & 63;
+        const int i = L.ng + STRIP_W * bx + lane;
+        const int j0 = L.ng + a.rows * by, j1 = min(j0 + a.rows, L.ny + L.ng);
+        double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
+        int bad = 0;
+        if (lane < STRIP_W && i < L.nx + L.ng)
+            for (int j = j0 + (threadIdx.x >> 6); j < j1; j += 4) {
+                const long long o = L.at(i, j);
+                const uint8_t t = a.type[o];
+                double U[4] = {Us[o], Us[o + L.plane], Us[o + 2 * L.plane], Us[o + 3 * L.plane]};
+                if (t == 0 || t == 3) {
+                    dt_terms(a.gam, a.gm1, U, amax, umax, vmax, bad);
+                    if (t == 0) msum += U[0] * a.dx * a.dy;
+                    else msum += a.cut[a.cut_index[o]].alpha * U[0] * a.dx * a.dy;
+                }
+            }
+        block4_partials(a.mass_part, bid, amax, umax, vmax, msum, bad);
+        return;
+    }
+    const double* Us = a.U;
     const int nint = L.nx * L.ny;
     double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
     int bad = 0;
@@ -638,12 +726,28 @@ __global__ void k_merge(MergeArgs a) {
     const MergeGroup gr = a.groups[gi];
     double* U = a.buf[a.ctl->cur ^ 1];
     const long long PL = a.plane;
+    // every load first (absent members read the target, unused), then the sums
+    // in group order
+    double ut[4], um[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) ut[k] = U[gr.target + k * PL];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const long long o = q < gr.n ? gr.member[q] : gr.target;
+#pragma unroll
+        for (int k = 0; k < 4; k++) um[q][k] = U[o + k * PL];
+    }
     double asum = gr.at;
-    for (int q = 0; q < gr.n; q++) asum = asum + gr.am[q];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (q < gr.n) asum = asum + gr.am[q];
     double v[4];
+#pragma unroll
     for (int k = 0; k < 4; k++) {
-        double s = gr.at * U[gr.target + k * PL];
-        for (int q = 0; q < gr.n; q++) s = s + gr.am[q] * U[gr.member[q] + k * PL];
+        double s = gr.at * ut[k];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (q < gr.n) s = s + gr.am[q] * um[q][k];
         v[k] = s / asum;
     }
     if (!a.quirks) {
@@ -651,9 +755,12 @@ __global__ void k_merge(MergeArgs a) {
         int db = 0;
         floor_and_dt(a.gm1 + 1.0, a.gm1, Rcp<double>{a.gm1, a.r_gm1}, v, 0, d0, d1, d2, db);
     }
+#pragma unroll
     for (int k = 0; k < 4; k++) {
         U[gr.target + k * PL] = v[k];
-        for (int q = 0; q < gr.n; q++) U[gr.member[q] + k * PL] = v[k];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (q < gr.n) U[gr.member[q] + k * PL] = v[k];
     }
 }
 
@@ -966,12 +1073,13 @@ hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_f
     return hipGetLastError();
 }
 hipError_t launch_reduce_partials(const ReduceArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce, dim3(a.nblocks), dim3(64), 0, s, a);
+    if (a.nblocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce, dim3(a.nblocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s) {
     if (a.ngroups == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_merge, dim3((a.ngroups + 127) / 128), dim3(128), 0, s, a);
+    hipLaunchKernelGGL(k_merge, dim3((a.ngroups + 63) / 64), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
@@ -985,6 +1093,18 @@ hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
     return launch_finish(f, s);
 }
 hipError_t launch_finish(const FinishArgs& f, hipStream_t s) {
+    if (f.advance == 1 && f.fold && f.nblocks >= FOLD_MIN_PARTS) {
+        FinishArgs g = f;
+        g.nfold = std::min(FOLD_MAX, (f.nblocks + FOLD_THREADS - 1) / FOLD_THREADS);
+        g.fold_per = (f.nblocks + g.nfold - 1) / g.nfold;
+        hipLaunchKernelGGL(k_fold, dim3(g.nfold), dim3(FOLD_THREADS), 0, s, g);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        g.part = f.fold;
+        g.nblocks = g.nfold;
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, g);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, f);
     return hipGetLastError();
 }

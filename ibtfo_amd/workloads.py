@@ -57,11 +57,31 @@ def shapes_phi(prm, x, y):
     return best
 
 
+def bodies_mask(N_x, N_y, ng=2, L=2.0, H=1.0, seed=20261202):
+    """The bodies of bodies_shapes(seed) as a binary mask image: RGBA, one pixel
+    per cell including the ghost ring ((N_x + 2 ng) x (N_y + 2 ng) pixels, top
+    row first), black (0, 0, 0, 255) where the pixel's point lies inside a body,
+    white elsewhere. Pixel (x, y), counted from the bottom row, stands at
+    (-ng dx + x dx, -ng dy + y dy), where the mask level set places it
+    (Test/PNGGeo/Level_Set.cpp:82-95)."""
+    dx, dy = L / N_x, H / N_y
+    M, N = N_x + 2 * ng, N_y + 2 * ng
+    prm = bodies_shapes(seed, H)
+    img = np.full((N, M, 4), 255, np.uint8)
+    px = -ng * dx + np.arange(M) * dx
+    for y0 in range(0, N, 256):  # row blocks keep the temporaries small at 4096 x 2048
+        py = -ng * dy + np.arange(y0, min(N, y0 + 256)) * dy
+        X, Y = np.meshgrid(px, py)
+        solid = shapes_phi(prm, X, Y) < 0.0
+        img[N - 1 - y0 - np.arange(len(py))[:, None], np.arange(M)[None, :], :3] = np.where(solid, 0, 255)[..., None]
+    return img
+
+
 # Mach 3 inflow (rho, u, v, p) of the 4096x2048 immersed-body workload
 MACH3_INFLOW = (1.4, 3.0, 0.0, 1.0)
 
 
-def mach3_bodies_config(ib, N_x=4096, N_y=2048, Riemann="HLL", Limiter="VanLeer", Boundary="IDWBaseGFM", quirks=0,
+def mach3_bodies_config(ib, N_x=4096, N_y=2048, Riemann="HLL", Limiter="VanLeer", Boundary="CutCell", quirks=0,
                         max_iter=10 ** 9, CFL_number=0.5):
     return ib.config(N_x=N_x, N_y=N_y, lo_x=0.0, lo_y=0.0, hi_x=2.0, hi_y=1.0, Psy_time=1e30, max_iter=max_iter,
                      CFL_number=CFL_number, gamma=1.4, Scheme="MUSCL", Riemann=Riemann, Limiter=Limiter,

@@ -147,6 +147,10 @@ int ibtfo_download_geometry(ibtfo_ctx* ctx, double* XYCOORD);
 int ibtfo_download_scheme_index(ibtfo_ctx* ctx, double* SCHEME_IDX);
 int ibtfo_sch_width(const ibtfo_ctx* ctx);
 int ibtfo_num_ghosts(const ibtfo_ctx* ctx);
+/* Cut-cell boundary: number of cut cells (type 3) and of small-cell merge
+ * groups after ibtfo_geometry* (0 for other boundaries). No reference
+ * counterpart: the reference has no cut-cell method. */
+int ibtfo_cut_counts(const ibtfo_ctx* ctx, int* cut_cells, int* merge_groups);
 
 /* Up to nsteps steps of the reference loop body (Test/Sod/Sod.cpp:154-177):
  * TimeAdvance(dt); Boundary; Global_Boundary; dt = ComputeDt; mass; now_t += dt;

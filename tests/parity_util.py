@@ -87,3 +87,17 @@ def rel_linf(a, b, ncomp=4):
         scale = np.max(np.abs(b[m, k])) if m.any() else 1.0
         out.append(float(np.max(np.abs(a[m, k] - b[m, k])) / (scale if scale > 0 else 1.0)) if m.any() else 0.0)
     return out
+
+
+def exact_mass(U, types, dx, dy, alpha=None):
+    """CaculateMass of a state summed exactly (math.fsum of the per-cell terms
+    rho*dx*dy of fluid cells, alpha*rho*dx*dy of cut cells, each term rounded as
+    the kernels round it), to check a reduction's own summation error."""
+    import math
+    rho = np.asarray(U)[0::4]
+    t = np.asarray(types)
+    terms = list((rho[t == 0] * dx) * dy)
+    if alpha is not None:
+        a = np.asarray(alpha)
+        terms += list(((a[t == 3] * rho[t == 3]) * dx) * dy)
+    return math.fsum(terms)

@@ -162,7 +162,7 @@ def test_tile_shapes_and_autotune_bitexact(oracle_mod):
     o = O.run_oracle(c, 12, root_mode=1)
 
     def solver():
-        s = ib.Solver(W.mach3_bodies_config(ib, N_x=200, N_y=100))
+        s = ib.Solver(W.mach3_bodies_config(ib, N_x=200, N_y=100, Boundary="IDWBaseGFM"))
         s.geometry("field", (), phi=c.phi)
         s.initialize("uniform", W.MACH3_INFLOW)
         s.prepare(True)
@@ -194,7 +194,7 @@ def test_shapes_levelset_matches_numpy(oracle_mod):
     from ibtfo_amd import workloads as W
     O = oracle_mod
     c = _bodies_case(O, nx=512, ny=256)
-    s = ib.Solver(W.mach3_bodies_config(ib, N_x=512, N_y=256))
+    s = ib.Solver(W.mach3_bodies_config(ib, N_x=512, N_y=256, Boundary="IDWBaseGFM"))
     s.geometry("shapes", W.bodies_shapes())
     XY = s.xycoord()
     phi = XY[2::6]
@@ -311,6 +311,104 @@ def _cut_case(O, name):
     cx = 0.5 if name == "circle_rest" else 0.4
     c.phi = (np.hypot(X - cx, Y - 0.5) - (0.21 if name == "circle_rest" else 0.1)).reshape(-1)
     return c, (40 if name == "circle_rest" else 120)
+
+
+@pytest.mark.parametrize("riemann", ["HLL", "StegerWarming"])
+def test_mask_cutcell_bitexact(oracle_mod, riemann):
+    """The Mach 3 workload as the benchmark runs it, at 256x128: the seeded
+    bodies as a binary mask, the mask level set on the device, cut cells with
+    small-cell merging; cell types and 60 steps == the oracle bit for bit, on the
+    device's phi (which matches the oracle's brute-force search except at
+    distance ties, checked in the mask level-set test)."""
+    import ibtfo_amd as ib
+    from ibtfo_amd import workloads as W
+    O = oracle_mod
+    c = _bodies_case(O, nx=256, ny=128, Riemann=riemann, Boundary="CutCell")
+    img = W.bodies_mask(256, 128)
+    s = ib.Solver(gpu_config(c))
+    s.geometry_mask(img)
+    ncut, ngroups = s.cut_counts()
+    XY = s.xycoord()
+    c.phi = XY[2::6].copy()
+    ref_phi = O.mask_phi(c, img)
+    assert np.abs(c.phi - ref_phi).max() <= 1e-15
+    s.initialize("uniform", W.MACH3_INFLOW)
+    s.prepare(True)
+    s.step(60)
+    assert s.status()[0] == 60
+    o = O.run_oracle(c, 60, root_mode=1)
+    types = XY[5::6]
+    assert nan_equal_bits(types, o["XY"][5::6]) == 0
+    assert ncut == int(np.sum(types == 3)) > 0 and ngroups > 0
+    assert np.isfinite(o["U_OLD"]).all()
+    assert nan_equal_bits(s.state(), o["U_OLD"]) == 0
+    d, _, _ = s.log(60)
+    assert nan_equal_bits(d, o["dt"][:60]) == 0
+
+
+def test_cutcell_strip_heights(oracle_mod):
+    """The post-merge dt/mass partials are recomputed only for the step blocks
+    holding cut cells or merge-group members; that block list follows the strip
+    height: every height (ragged strips included) and a change of height in the
+    middle of a run give the oracle's state, dt and mass."""
+    import ibtfo_amd as ib
+    from ibtfo_amd import workloads as W
+    O = oracle_mod
+    c = _bodies_case(O, nx=200, ny=100, Boundary="CutCell")
+    o = O.run_oracle(c, 20, root_mode=1)
+    for rows in [1, 3, 8, 33, 100, 0]:
+        s = ib.Solver(gpu_config(c))
+        s.geometry("field", (), phi=c.phi)
+        s.initialize("uniform", W.MACH3_INFLOW)
+        s.prepare(True)
+        s.set_rows(rows if rows else 5)
+        s.step(10)
+        if rows == 0:
+            s.set_rows(17)
+        s.step(10)
+        assert s.status()[0] == 20
+        assert nan_equal_bits(s.state(), o["U_OLD"]) == 0, rows
+        d, _, m = s.log(20)
+        assert nan_equal_bits(d, o["dt"][:20]) == 0, rows
+        assert np.allclose(m, o["mass"][:20], rtol=1e-12, atol=0), rows
+        s.close()
+
+
+@pytest.mark.parametrize("boundary", ["IDWBaseGFM", "CutCell"])
+def test_two_level_finish(oracle_mod, boundary):
+    """With 2048 or more step-kernel partials the dt/mass reduction runs in two
+    levels (blocks fold ranges of partials, then one block finishes): 512x256 at
+    one row per strip (9 x 256 partials) and at 16 rows (one level) both give the
+    oracle's state and dt bit for bit. Mass: within 1e-13 of the exactly summed
+    mass of the final state (the device sums in trees), and within the oracle's
+    own sequential-summation bound (cells x 2^-53, measured 2e-12 here) of the
+    oracle's log."""
+    import ibtfo_amd as ib
+    from ibtfo_amd import workloads as W
+    from parity_util import exact_mass
+    O = oracle_mod
+    c = _bodies_case(O, nx=512, ny=256, Boundary=boundary)
+    st = O.setup(c)
+    o = O.run_oracle(c, 12, root_mode=1)
+    alpha = st["cut"][0::8] if st.get("cut") is not None else None
+    m_exact = exact_mass(o["U_OLD"], o["XY"][5::6], 2.0 / 512, 1.0 / 256, alpha)
+    seq_tol = c.M * c.N * 2.0 ** -53
+    for rows in [1, 16]:
+        s = ib.Solver(gpu_config(c))
+        s.geometry("field", (), phi=c.phi)
+        s.initialize("uniform", W.MACH3_INFLOW)
+        s.prepare(True)
+        s.set_rows(rows)
+        r, nbx, nby = s.tile()
+        assert (nbx * nby >= 2048) == (rows == 1)
+        s.step(12)
+        assert s.status()[0] == 12
+        assert nan_equal_bits(s.state(), o["U_OLD"]) == 0, rows
+        d, _, m = s.log(12)
+        assert nan_equal_bits(d, o["dt"][:12]) == 0, rows
+        assert abs(m[-1] - m_exact) <= 1e-13 * m_exact, (rows, m[-1], m_exact)
+        assert np.allclose(m, o["mass"][:12], rtol=seq_tol, atol=0), rows
+        s.close()
 
 
 @pytest.mark.parametrize("name", ["circle_rest", "mach3_cylinder", "bodies"])
