@@ -15,7 +15,9 @@
 //   Scheme_Index  Src/BoundaryCondition/Scheme_Index.H:13-19 -> built with the geometry
 //   Level_Set     Src/Geometry/Level_Set.H:12-18             -> the driver's own
 //                 Level_Set_function hook fills phi on the host; classification,
-//                 normals and ghost records run on the device
+//                 normals and ghost records run on the device. With a `mask_file`
+//                 key in the .inp, the mask level set of Test/PNGGeo/Level_Set.cpp
+//                 (which reads Demo.png) runs on the device from that PNG instead
 // `ndevices` must be 1 and `device` 0: the whole domain lives on one GPU.
 #include <cstdio>
 #include <cstdlib>
@@ -116,6 +118,17 @@ void Level_Set(char* filename, const int N_x, const int N_y, const int ng, doubl
                const int device) {
     one_device(ndevices, device, "Level_Set");
     Bound* b = find(XYCOORD, "Level_Set");
+    char mask[1024];
+    int n = ibtfo_inp_string(filename, "mask_file", mask, (int)sizeof mask);
+    if (n > 0) {
+        // ParamReader drops the blanks and keeps an inline comment: cut at '#'
+        int k = 0;
+        while (mask[k] && mask[k] != '#') k++;
+        mask[k] = 0;
+        check(ibtfo_geometry_png(b->ctx, mask), "Level_Set");
+        check(ibtfo_download_geometry(b->ctx, XYCOORD), "Level_Set");
+        return;
+    }
     if (!Level_Set_function) {
         fprintf(stderr, "ibtfo: Level_Set: no Level_Set_function hook linked\n");
         abort();

@@ -1,0 +1,77 @@
+"""The reference-signature boundary (include/ibtfo_ref.h) driven the way the
+reference's test drivers drive it: tests/drivers/ref_sig_driver.cpp keeps the
+Test/Sod/Sod.cpp call sequence (Psy_coord, Level_Set, Scheme_Index, Initialize,
+Boundary, ComputeDt, then TimeAdvance / Boundary / Global_Boundary / ComputeDt /
+CaculateMass per step) with the OpenACC data directives replaced by the four
+ibtfo_ref_* calls, and reads its mask from a PNG named in the .inp (the PNGGeo
+case). Its results must equal the oracle's: cell types, state and dt bit for
+bit, mass within the oracle's sequential-summation bound."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from parity_util import nan_equal_bits
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "drivers", "ref_sig_driver")
+
+INP = """N_x = {nx}
+N_y = {ny}
+num_ghost_cell = 2
+lo_x = 0.0
+lo_y = 0.0
+hi_x = 2.0
+hi_y = 1.0
+Psy_time = 100.0
+max_iter = {steps}
+CFL_number = 0.5
+gamma = 1.4
+Scheme = MUSCL
+Riemann = {riemann}
+Limiter = VanLeer
+Boundary = {boundary}
+Time = SingleStep
+Edge = inflow
+quirks = 0
+inflow_rho = 1.4
+inflow_u = 3.0
+inflow_v = 0.0
+inflow_p = 1.0
+mask_file = Demo.png   # the PNGGeo driver's image
+"""
+
+
+@pytest.mark.parametrize("boundary,riemann", [("IDWBaseGFM", "HLL"), ("CutCell", "StegerWarming")])
+def test_reference_signature_driver(oracle_mod, tmp_path, boundary, riemann):
+    PIL = pytest.importorskip("PIL.Image")
+    from ibtfo_amd import workloads as W
+    assert os.path.exists(DRIVER), "tests/drivers/ref_sig_driver is not built (run __graft_entry__.build())"
+    nx, ny, steps = 128, 64, 40
+    PIL.fromarray(W.bodies_mask(nx, ny), "RGBA").save(tmp_path / "Demo.png")
+    (tmp_path / "case.inp").write_text(INP.format(nx=nx, ny=ny, steps=steps, boundary=boundary, riemann=riemann))
+    out = tmp_path / "out"
+    out.mkdir()
+    print("ref_sig_driver %s %s" % (boundary, riemann), flush=True)
+    r = subprocess.run([DRIVER, "case.inp", str(out)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    XY = np.fromfile(out / "XY.bin")
+    U = np.fromfile(out / "U.bin")
+    log = np.fromfile(out / "log.bin").reshape(-1, 3)
+    assert len(log) == steps
+    O = oracle_mod
+    c = O.Case(N_x=nx, N_y=ny, lo_x=0.0, lo_y=0.0, hi_x=2.0, hi_y=1.0, Psy_time=100.0, max_iter=steps,
+               CFL_number=0.5, Scheme="MUSCL", Riemann=riemann, Limiter="VanLeer", Boundary=boundary,
+               case="uniform", edge="inflow", quirks=0, extra=dict(rho=1.4, u=3.0, v=0.0, p=1.0))
+    c.phi = XY[2::6].copy()  # the mask level set computed on the device (its own test pins it)
+    assert np.abs(c.phi - O.mask_phi(c, W.bodies_mask(nx, ny))).max() <= 1e-15
+    o = O.run_oracle(c, steps, root_mode=1)
+    assert o["steps"] == steps
+    assert nan_equal_bits(XY[5::6], o["XY"][5::6]) == 0
+    assert nan_equal_bits(U, o["U_OLD"]) == 0
+    assert nan_equal_bits(log[:, 0], o["dt"][:steps]) == 0
+    assert nan_equal_bits(log[:, 1], o["t"][:steps]) == 0
+    assert np.allclose(log[:, 2], o["mass"][:steps], rtol=c.M * c.N * 2.0 ** -53, atol=0)
