@@ -17,7 +17,7 @@ SCHEME = {"Lax": 0, "MUSCL": 1, "MHM": 2, "Godunov": 3}
 RIEMANN = {"HLL": 0, "HLLC": 1, "PVRS": 2, "StegerWarming": 3}
 LIMITER = {"minmod": 0, "VanLeer": 1, "VanAlbada": 2}
 BOUNDARY = {"Regular": 0, "IDWBaseGFM": 1, "LinearBaseGFM": 2, "RiemannBaseGFM": 3, "BCGFM": 4, "HBCGFM": 5,
-            "CutCell": 6}
+            "CutCell": 6, "SimpleBound": 7}
 TIME = {"SingleStep": 0, "RK2": 1}
 EDGE = {"reference": 0, "periodic": 1, "inflow": 2}
 PRECISION = {"FP64": 0, "fp64": 0, "mixed": 1}

@@ -208,6 +208,10 @@ extern "C" int ibtfo_batch_create(const ibtfo_batch_config* bc, ibtfo_batch** ou
     }
     if (bc->scheme == IBTFO_LAX) { set_error("the batch engine runs the reconstructing schemes only"); return IBTFO_E_ARG; }
     if (bc->boundary == IBTFO_CUTCELL) { set_error("the batch engine has no cut-cell step sequence"); return IBTFO_E_ARG; }
+    if (bc->boundary == IBTFO_SIMPLE_BOUND) {
+        set_error("the batch engine runs one ghost-fill pass per step; SimpleBound may need several");
+        return IBTFO_E_ARG;
+    }
     ibtfo_batch* b = new ibtfo_batch();
     b->bc = *bc;
     if (b->bc.rows == 0) b->bc.rows = 16;

@@ -88,11 +88,15 @@ static int validate(const ibtfo_config* c) {
     if (c->N_x < 4 || c->N_y < 2 * c->num_ghost_cell) { set_error("grid too small"); return IBTFO_E_ARG; }
     if ((long long)c->N_x * c->N_y > (1ll << 31)) { set_error("grid too large"); return IBTFO_E_ARG; }
     if (c->scheme < 0 || c->scheme > 3 || c->riemann < 0 || c->riemann > 3 || c->limiter < 0 || c->limiter > 2 ||
-        c->boundary < 0 || c->boundary > 6 || c->edge < 0 || c->edge > 2) {
+        c->boundary < 0 || c->boundary > 7 || c->edge < 0 || c->edge > 2) {
         set_error("slot value out of range");
         return IBTFO_E_ARG;
     }
     if (c->time != IBTFO_SINGLESTEP && c->time != IBTFO_RK2) { set_error("unknown Time slot"); return IBTFO_E_ARG; }
+    if (c->boundary == IBTFO_SIMPLE_BOUND && c->scheme == IBTFO_LAX && c->edge == IBTFO_EDGE_REFERENCE) {
+        set_error("SimpleBound with Lax and the reference edge hook is not supported");
+        return IBTFO_E_ARG;
+    }
     if (c->boundary == IBTFO_CUTCELL && (c->quirks || c->time != IBTFO_SINGLESTEP || c->scheme == IBTFO_LAX)) {
         set_error("the cut-cell boundary needs quirks = 0, Time = SingleStep and a reconstructing scheme");
         return IBTFO_E_ARG;
@@ -129,7 +133,7 @@ void ibtfo::free_ctx(ibtfo_ctx* x) {
     for (void* p : {(void*)x->state[0], (void*)x->state[1], (void*)x->type, (void*)x->phi, (void*)x->gnx,
                     (void*)x->gny, (void*)x->ghosts, (void*)x->gvals, (void*)x->dcount, (void*)x->ctl,
                     (void*)x->mass_part, (void*)x->corner, (void*)x->cuts, (void*)x->cut_index, (void*)x->groups,
-                    (void*)x->rblocks, (void*)x->fold})
+                    (void*)x->rblocks, (void*)x->fold, (void*)x->unew0})
         dfree(x, p);
     if (x->own_stream) (void)hipStreamDestroy(x->own_stream);
     delete x;
@@ -226,6 +230,7 @@ extern "C" int ibtfo_sch_width(const ibtfo_ctx* x) {
         case IBTFO_IDW_GFM: case IBTFO_CUTCELL: return 8;
         case IBTFO_LINEAR_GFM: case IBTFO_RIEMANN_GFM: return 4;
         case IBTFO_BCGFM: case IBTFO_HBCGFM: return 16;
+        case IBTFO_SIMPLE_BOUND: return 2;
         default: return 1;
     }
 }
@@ -353,6 +358,56 @@ static void cut_blocks(ibtfo_ctx* x) {
     x->rb_rows = a.rows;
 }
 
+// SimpleBound fills its ghosts in the reference's sequential order (i outer, j
+// inner, LS_SimpleBound.cpp:32-36) in place, so a donor that is itself a ghost
+// filled earlier is read with its new value. Such dependencies are found here,
+// once per geometry: each ghost gets the index of an earlier-filled ghost donor
+// (x and y) and a level one above that donor's; the fill then runs level by level.
+static int simple_bound_levels(ibtfo_ctx* x) {
+    const Layout& L = x->L;
+    x->nlevels = 1;
+    if (x->cfg.boundary != IBTFO_SIMPLE_BOUND || x->nghost == 0) return IBTFO_OK;
+    std::vector<Ghost> gh(x->nghost);
+    HIPCHK(scopy(x, gh.data(), x->ghosts, x->nghost * sizeof(Ghost), hipMemcpyDeviceToHost));
+    auto key = [&](long long cell) {  // position in the reference's loop order
+        const long long r = cell - L.off;
+        return (r % L.pitch) * (long long)L.N + r / L.pitch;
+    };
+    std::vector<int> order(x->nghost);
+    std::map<long long, int> at;
+    for (int q = 0; q < x->nghost; q++) { order[q] = q; at[gh[q].cell] = q; }
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return key(gh[a].cell) < key(gh[b].cell); });
+    int top = 0;
+    for (int q : order) {
+        Ghost& g = gh[q];
+        g.pick = 0;
+        g.idxe = g.idye = -1;
+        auto dep = [&](bool use, long long d, int& slot) {
+            if (!use) return;
+            auto it = at.find(d);
+            if (it == at.end() || key(d) >= key(g.cell)) return;  // not a ghost, or filled later
+            slot = it->second;
+            g.pick = std::max(g.pick, gh[it->second].pick + 1);
+        };
+        dep(g.idx != 0, g.d1, g.idxe);
+        dep(g.idy != 0, g.d1e, g.idye);
+        top = std::max(top, g.pick);
+    }
+    x->nlevels = top + 1;
+    HIPCHK(scopy(x, x->ghosts, gh.data(), x->nghost * sizeof(Ghost), hipMemcpyHostToDevice));
+    return IBTFO_OK;
+}
+
+// ghost fill phase 1, every dependency level in order
+static hipError_t launch_fill_levels(ibtfo_ctx* x, FillArgs f) {
+    for (int l = 0; l < std::max(1, x->nlevels); l++) {
+        f.level = l;
+        hipError_t e = launch_fill(f, x->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 static int geometry_finish(ibtfo_ctx* x, GeomArgs& g, double* dphi) {
     hipError_t ge = geom_run(g, x->stream);
     if (ge != hipSuccess) { dfree(x, dphi); HIPCHK(ge); }
@@ -383,6 +438,10 @@ static int geometry_finish(ibtfo_ctx* x, GeomArgs& g, double* dphi) {
             HIPCHK(geom_ghosts(g, x->stream));
         }
         x->nghost = (int)cnt;
+    }
+    {
+        int rc = simple_bound_levels(x);
+        if (rc) { dfree(x, dphi); return rc; }
     }
     HIPCHK(hipStreamSynchronize(x->stream));
     dfree(x, dphi);
@@ -569,6 +628,10 @@ extern "C" int ibtfo_download_scheme_index(ibtfo_ctx* x, double* S) {
         long long rel = g.cell - L.off;
         int j = (int)(rel / L.pitch), i = (int)(rel % L.pitch);
         double* r = S + ((size_t)L.M * j + i) * W;
+        if (W == 2) {  // SimpleBound: {IDX, IDY}
+            r[0] = g.idx; r[1] = g.idy;
+            continue;
+        }
         r[0] = g.xm; r[1] = g.ym; r[2] = g.idx; r[3] = g.idy;
         if (W >= 8) for (int q = 0; q < 4; q++) r[4 + q] = g.a[q];
         if (W == 16) {
@@ -592,7 +655,7 @@ static void corner_from_aos(ibtfo_ctx* x, const double* U) {
         }
 }
 
-static int upload_aos(ibtfo_ctx* x, const double* U, int buf) {
+static int upload_aos_to(ibtfo_ctx* x, const double* U, double* dst) {
     const Layout& L = x->L;
     std::vector<double> planes(4 * L.plane, 0.0);
     for (int j = 0; j < L.N; j++)
@@ -601,9 +664,11 @@ static int upload_aos(ibtfo_ctx* x, const double* U, int buf) {
             long long o = L.at(i, j);
             for (int k = 0; k < 4; k++) planes[o + k * L.plane] = q[k];
         }
-    HIPCHK(scopy(x, x->state[buf], planes.data(), 4 * L.plane * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(scopy(x, dst, planes.data(), 4 * L.plane * sizeof(double), hipMemcpyHostToDevice));
     return IBTFO_OK;
 }
+
+static int upload_aos(ibtfo_ctx* x, const double* U, int buf) { return upload_aos_to(x, U, x->state[buf]); }
 
 static int read_ctl_head(ibtfo_ctx* x, Ctl& c) {
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -621,6 +686,13 @@ extern "C" int ibtfo_upload_state(ibtfo_ctx* x, const double* U_OLD, const doubl
     HIPCHK(hipStreamSynchronize(x->stream));
     int rc = upload_aos(x, U_OLD, 0);
     if (rc) return rc;
+    if (x->cfg.boundary == IBTFO_SIMPLE_BOUND) {
+        if (!x->unew0 && dalloc(x, &x->unew0, 4 * x->L.plane * sizeof(double)) != hipSuccess) {
+            set_error("ibtfo_upload_state: out of device memory");
+            return IBTFO_E_NOMEM;
+        }
+        if ((rc = upload_aos_to(x, U_NEW ? U_NEW : U_OLD, x->unew0))) return rc;
+    }
     corner_from_aos(x, U_NEW ? U_NEW : U_OLD);
     HIPCHK(scopy(x, x->corner, x->corner_host.data(), x->corner_host.size() * sizeof(double), hipMemcpyHostToDevice));
     Ctl c;
@@ -703,7 +775,10 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     a.max_iter = c.max_iter;
     a.quirks = c.quirks;
     a.edge_fused = (c.edge == IBTFO_EDGE_REFERENCE && c.scheme != IBTFO_LAX) ? 1 : 0;
-    a.corner_from_input = c.boundary != IBTFO_REGULAR;
+    // the reference's U_NEW ring is refreshed by Boundary's copy back (GFM family),
+    // else it keeps the uploaded values (Regular, SimpleBound)
+    a.corner_from_input = c.boundary != IBTFO_REGULAR && c.boundary != IBTFO_SIMPLE_BOUND;
+    a.unew_nf = c.boundary == IBTFO_SIMPLE_BOUND ? x->unew0 : nullptr;
     step_grid(L, x->rows_override, &a.nbx, &a.nby, &a.rows);
     a.cut = x->cfg.boundary == IBTFO_CUTCELL ? x->cuts : nullptr;
     a.cut_index = x->cut_index;
@@ -721,6 +796,8 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     memset(&e, 0, sizeof e);
     e.buf[0] = x->state[0]; e.buf[1] = x->state[1];
     e.L = L; e.ctl = x->ctl; e.kind = c.edge; e.gam = c.gamma; e.require_ran = 1;
+    e.unew_nf = a.unew_nf;
+    e.type = x->type;
     double r = c.edge_state[0], u = c.edge_state[1], v = c.edge_state[2], p = c.edge_state[3];
     e.inflow[0] = r; e.inflow[1] = r * u; e.inflow[2] = r * v;
     e.inflow[3] = p / (c.gamma - 1) + 0.5 * r * (u * u + v * v);
@@ -756,7 +833,7 @@ void ibtfo::build_args(ibtfo_ctx* x) {
 // ghost fill phase 1, the edge ring, ghost fill phase 2.
 static int enqueue_boundaries(ibtfo_ctx* x) {
     const ibtfo_config& c = x->cfg;
-    if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_fill(x->fa, x->stream));
+    if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_fill_levels(x, x->fa));
     if (c.edge == IBTFO_EDGE_REFERENCE && c.scheme == IBTFO_LAX)
         HIPCHK(launch_edge_ref(x->ea, x->corner, x->sa.corner_from_input, x->stream));
     if (c.edge != IBTFO_EDGE_REFERENCE) HIPCHK(launch_edge(x->ea, x->stream));
@@ -831,7 +908,7 @@ extern "C" int ibtfo_boundary(ibtfo_ctx* x) {
     FillArgs f = x->fa;
     f.require_ran = 0;
     f.ring_prev = 0;  // U_NEW = U_OLD here, ring included
-    HIPCHK(launch_fill(f, x->stream));
+    HIPCHK(launch_fill_levels(x, f));
     HIPCHK(launch_scatter(f, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     return IBTFO_OK;

@@ -17,10 +17,12 @@ struct ibtfo_ctx {
     hipStream_t stream;
     hipStream_t own_stream;
     double* state[2];
+    double* unew0;  // SimpleBound: the uploaded U_NEW (its non-fluid cells stay as uploaded)
     uint8_t* type;
     double *phi, *gnx, *gny;
     ibtfo::Ghost* ghosts;
     int nghost;
+    int nlevels;  // ghost-fill passes per Boundary (SimpleBound dependency levels; else 1)
     double* gvals;
     unsigned int* dcount;
     ibtfo::Ctl* ctl;

@@ -147,6 +147,35 @@ __global__ void k_geom_ghosts(GeomArgs g) {
     double x = xc(g, i), y = yc(g, j);
     const int bc = g.bc;
     double dxg = xc(g, 1) - xc(g, 0), dyg = yc(g, 2) - yc(g, 1);
+    if (bc == IBTFO_SIMPLE_BOUND) {
+        // SimpleBound/Scheme_Index.cpp:12-66: the nearest fluid cell within ng
+        // along -x (else +x) and -y (else +y); k cells away -> donor offset 1 - 2k
+        // (2k - 1). Dependencies on earlier-filled ghosts are resolved on the host.
+        auto t = [&](int ii, int jj) { return g.type[L.at(ii, jj)]; };
+        int IDX = 0, IDY = 0;
+        for (int k = 0; k < ng + 1; k++)
+            if (t(i - k, j) < 1) { IDX = 1 - 2 * k; break; }
+        if (IDX == 0)
+            for (int k = 0; k < ng + 1; k++)
+                if (t(i + k, j) < 1) { IDX = 2 * k - 1; break; }
+        for (int k = 0; k < ng + 1; k++)
+            if (t(i, j - k) < 1) { IDY = 1 - 2 * k; break; }
+        if (IDY == 0)
+            for (int k = 0; k < ng + 1; k++)
+                if (t(i, j + k) < 1) { IDY = 2 * k - 1; break; }
+        auto ring_cell = [&](int ii, int jj) { return ii < ng || ii >= L.nx + ng || jj < ng || jj >= L.ny + ng; };
+        gh.idx = IDX; gh.idy = IDY;
+        gh.d1 = L.at(i + IDX, j);
+        gh.d1e = L.at(i, j + IDY);
+        gh.idxe = gh.idye = -1;
+        gh.pick = 0;
+        gh.ring = (ring_cell(i + IDX, j) ? 1 : 0) | (ring_cell(i, j + IDY) ? 16 : 0);
+        gh.xm = gh.ym = gh.xe = gh.ye = 0.0;
+        for (int q = 0; q < 4; q++) { gh.a[q] = 0.0; gh.ae[q] = 0.0; }
+        unsigned int slot = atomicAdd(g.count, 1u);
+        if (slot < (unsigned int)g.cap) g.ghosts[slot] = gh;
+        return;
+    }
     if (bc == 1 || bc == 2 || bc == 3) {
         gh.xm = x + 2 * gh.nx * ph;
         gh.ym = y + 2 * gh.ny * ph;

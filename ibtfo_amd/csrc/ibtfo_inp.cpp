@@ -167,9 +167,9 @@ extern "C" int ibtfo_config_from_inp(const char* path, ibtfo_config* c) {
     // OIDWBaseGFM is what Test/InclineSod/Makefile:4 and Test/Vortex/Makefile:4 name; no such
     // directory exists and IDWBaseGFM is the evident intent.
     static const char* bn[] = {"Regular", "IDWBaseGFM", "OIDWBaseGFM", "LinearBaseGFM", "RiemannBaseGFM", "BCGFM", "HBCGFM",
-                               "CutCell"};
+                               "CutCell", "SimpleBound"};
     static const int bv[] = {IBTFO_REGULAR, IBTFO_IDW_GFM, IBTFO_IDW_GFM, IBTFO_LINEAR_GFM, IBTFO_RIEMANN_GFM,
-                             IBTFO_BCGFM, IBTFO_HBCGFM, IBTFO_CUTCELL};
+                             IBTFO_BCGFM, IBTFO_HBCGFM, IBTFO_CUTCELL, IBTFO_SIMPLE_BOUND};
     static const char* tn[] = {"SingleStep", "RK2"};
     static const int tv[] = {IBTFO_SINGLESTEP, IBTFO_RK2};
     static const char* en[] = {"reference", "periodic", "inflow"};
@@ -180,7 +180,7 @@ extern "C" int ibtfo_config_from_inp(const char* path, ibtfo_config* c) {
     if ((rc = lookup(str, "Scheme", sn, sv, 4, &c->scheme))) return rc;
     if ((rc = lookup(str, "Riemann", rn, rv, 4, &c->riemann))) return rc;
     if ((rc = lookup(str, "Limiter", ln, lv, 3, &c->limiter))) return rc;
-    if ((rc = lookup(str, "Boundary", bn, bv, 8, &c->boundary))) return rc;
+    if ((rc = lookup(str, "Boundary", bn, bv, 9, &c->boundary))) return rc;
     if ((rc = lookup(str, "Time", tn, tv, 2, &c->time))) return rc;
     if ((rc = lookup(str, "Edge", en, ev, 3, &c->edge))) return rc;
     if ((rc = lookup(str, "Precision", pn, pv, 2, &c->precision))) return rc;

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "../../include/ibtfo.h"
 #include "ibtfo_math.h"
 
 namespace ibtfo {
@@ -69,6 +70,9 @@ struct StepArgs {
     int max_iter;
     int quirks;
     int edge_fused;         // write the reference Global_Boundary ring inside the step
+    // SimpleBound: the reference's U_NEW of non-fluid cells, which its Boundary
+    // never refreshes (no copy back), as Global_Boundary reads it; else null
+    const double* unew_nf;
     int corner_from_input;  // corners from the input buffer's ring (GFM) or `corner` (Regular)
     int rows;               // rows per block strip
     int nbx, nby;           // grid
@@ -99,7 +103,9 @@ struct Ghost {
     double a[4];        // image-point weights (IDW/BCGFM); Linear/RiemannBaseGFM: a[0] = chosen donor
     double ae[4];       // extra-point weights
     double xm, ym, xe, ye;  // image (mirror) and extra points
-    int idx, idy, idxe, idye, pick;
+    int idx, idy, idxe, idye, pick;  // SimpleBound: idx/idy = donor offsets IDX/IDY, idxe/idye = ghost
+                                     // index of an x/y donor that is a ghost filled earlier (else -1),
+                                     // pick = dependency level
     int ring;           // bit q (image) / 4+q (extra): donor q lies in the domain ring
 };
 
@@ -118,6 +124,7 @@ struct FillArgs {
     int require_ran;   // skip unless the preceding step executed
     int ring_prev;     // read ring donors from the previous state (the ring before Global_Boundary)
     int quirks;
+    int level;         // SimpleBound: fill only the ghosts of this dependency level (Ghost::pick)
 };
 
 struct EdgeArgs {
@@ -128,6 +135,10 @@ struct EdgeArgs {
     double gam;
     double inflow[4];  // conserved inflow state
     int require_ran;
+    // SimpleBound: non-fluid source cells are read from the U_NEW its Boundary
+    // leaves unrefreshed (as the edge policies read U_NEW); else null
+    const double* unew_nf;
+    const uint8_t* type;
 };
 
 struct ReduceArgs {

@@ -561,28 +561,33 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
 #pragma unroll
                 for (int k = 0; k < 4; k++) Uo[o + k * PL] = un[k];
                 if (a.edge_fused) {
-                    // Test/Sod/Global_Boudary.cpp:24-59 (rows) and :62-75 (columns)
-                    if (j < 2 * ng) Uo[L.at(c, 2 * ng - 1 - j) + 2 * PL] = -un[2];
+                    // Test/Sod/Global_Boudary.cpp:24-59 (rows) and :62-75 (columns), which
+                    // read U_NEW: the updated state, or for a non-fluid cell under
+                    // SimpleBound the value its U_NEW kept
+                    double e[4] = {un[0], un[1], un[2], un[3]};
+                    if (a.unew_nf && !fluid)
+                        for (int k = 0; k < 4; k++) e[k] = a.unew_nf[o + k * PL];
+                    if (j < 2 * ng) Uo[L.at(c, 2 * ng - 1 - j) + 2 * PL] = -e[2];
                     if (j == 2 * ng - 1)
                         for (int r = 0; r < ng; r++) {
                             long long q = L.at(c, r);
-                            Uo[q] = un[0]; Uo[q + PL] = un[1]; Uo[q + 3 * PL] = un[3];
+                            Uo[q] = e[0]; Uo[q + PL] = e[1]; Uo[q + 3 * PL] = e[3];
                         }
-                    if (j >= ny) Uo[L.at(c, 2 * (ny + ng) - 1 - j) + 2 * PL] = -un[2];
+                    if (j >= ny) Uo[L.at(c, 2 * (ny + ng) - 1 - j) + 2 * PL] = -e[2];
                     if (j == ny + ng - 1)
                         for (int r = 0; r < ng; r++) {
                             long long q = L.at(c, ny + ng + r);
-                            Uo[q] = un[0]; Uo[q + PL] = un[1]; Uo[q + 3 * PL] = un[3];
+                            Uo[q] = e[0]; Uo[q + PL] = e[1]; Uo[q + 3 * PL] = e[3];
                         }
                     if (c == ng)
                         for (int r = 0; r < ng; r++) {
                             long long q = L.at(r, j);
-                            for (int k = 0; k < 4; k++) Uo[q + k * PL] = un[k];
+                            for (int k = 0; k < 4; k++) Uo[q + k * PL] = e[k];
                         }
                     if (c == nx + ng - 1)
                         for (int r = 0; r < ng; r++) {
                             long long q = L.at(nx + ng + r, j);
-                            for (int k = 0; k < 4; k++) Uo[q + k * PL] = un[k];
+                            for (int k = 0; k < 4; k++) Uo[q + k * PL] = e[k];
                         }
                 }
             }
@@ -772,6 +777,7 @@ __device__ __forceinline__ void fill_one(const FillArgs& a, const int gi) {
     if (a.require_ran && !a.ctl->ran) return;
     if (gi >= a.nghost) return;
     const Ghost gh = a.ghosts[gi];
+    if (a.bc == IBTFO_SIMPLE_BOUND && gh.pick != a.level) return;
     const int cur = a.use_cur ? a.ctl->cur : a.fixed;
     const double* U = a.buf[cur];
     // Boundary runs before Global_Boundary in the reference, so a donor in the
@@ -812,6 +818,36 @@ __device__ __forceinline__ void fill_one(const FillArgs& a, const int gi) {
         u_n = -u_n;
         uw = u_n * n_x + u_t * n_y;
         vw = u_n * n_y - u_t * n_x;
+    } else if (a.bc == IBTFO_SIMPLE_BOUND) {
+        // SimpleBound/LS_SimpleBound.cpp:38-105: reflect the x donor's u and/or the y
+        // donor's v (the mean of both reflections when both exist). A donor that is
+        // a ghost filled earlier in the reference's order is read from its new
+        // value (an earlier dependency level), any other from the state.
+        const int IDX = gh.idx, IDY = gh.idy;
+        if (IDX == 0 && IDY == 0) {  // no fluid within reach: the ghost keeps its value
+            double* o = a.out + 4 * (long long)gi;
+            for (int k = 0; k < 4; k++) o[k] = U[gh.cell + k * PL];
+            return;
+        }
+        auto donor = [&](long long d, int earlier, bool ring, double& r, double& u, double& v, double& p) {
+            if (earlier >= 0) {
+                const double* q = a.out + 4 * (long long)earlier;
+                double qq[4] = {q[0], q[1], q[2], q[3]};
+                primitives(gm1, qq, r, u, v, p);
+            } else {
+                prim(d, ring, r, u, v, p);
+            }
+        };
+        double rx = 0, ux = 0, vx = 0, px = 0, ry = 0, uy = 0, vy = 0, py = 0;
+        if (IDX != 0) donor(gh.d1, gh.idxe, gh.ring & 1, rx, ux, vx, px);
+        if (IDY != 0) donor(gh.d1e, gh.idye, (gh.ring >> 4) & 1, ry, uy, vy, py);
+        if (IDX != 0 && IDY != 0) {
+            rw = 0.5 * (rx + ry); uw = 0.5 * (-ux + uy); vw = 0.5 * (vx - vy); pw = 0.5 * (px + py);
+        } else if (IDX != 0) {
+            rw = rx; uw = -ux; vw = vx; pw = px;
+        } else {
+            rw = ry; uw = uy; vw = -vy; pw = py;
+        }
     } else if (a.bc == 2 || a.bc == 3) {  // Linear/RiemannBaseGFM: one donor
         int pick = (int)gh.a[0];
         if (pick >= 0) {
@@ -900,7 +936,8 @@ __device__ __forceinline__ void edge_one(const EdgeArgs& a, const int q) {
         int si = i < ng ? i + nx : (i >= nx + ng ? i - nx : i);
         int sj = j < ng ? j + ny : (j >= ny + ng ? j - ny : j);
         long long s = L.at(si, sj);
-        for (int k = 0; k < 4; k++) v[k] = U[s + k * PL];
+        const double* S = (a.unew_nf && a.type[s] != 0) ? a.unew_nf : U;
+        for (int k = 0; k < 4; k++) v[k] = S[s + k * PL];
     } else if (a.kind == 2) {  // inflow left, zero gradient elsewhere
         if (i < ng) {
             for (int k = 0; k < 4; k++) v[k] = a.inflow[k];
@@ -908,7 +945,8 @@ __device__ __forceinline__ void edge_one(const EdgeArgs& a, const int q) {
             int si = i >= nx + ng ? nx + ng - 1 : i;
             int sj = j < ng ? ng : (j >= ny + ng ? ny + ng - 1 : j);
             long long s = L.at(si, sj);
-            for (int k = 0; k < 4; k++) v[k] = U[s + k * PL];
+            const double* S = (a.unew_nf && a.type[s] != 0) ? a.unew_nf : U;
+            for (int k = 0; k < 4; k++) v[k] = S[s + k * PL];
         }
     } else {
         return;  // the reference policy is written by the step kernel

@@ -47,8 +47,10 @@ enum ibtfo_limiter { IBTFO_MINMOD = 0, IBTFO_VANLEER = 1, IBTFO_VANALBADA = 2 };
 enum ibtfo_boundary {
     IBTFO_REGULAR = 0, IBTFO_IDW_GFM = 1, IBTFO_LINEAR_GFM = 2, IBTFO_RIEMANN_GFM = 3,
     IBTFO_BCGFM = 4, IBTFO_HBCGFM = 5,
-    IBTFO_CUTCELL = 6 /* cut cells with small-cell merging (no reference counterpart; corrected mode,
-                         single-step, not in batches) */
+    IBTFO_CUTCELL = 6, /* cut cells with small-cell merging (no reference counterpart; corrected mode,
+                          single-step, not in batches) */
+    IBTFO_SIMPLE_BOUND = 7 /* SimpleBound/Scheme_Index.cpp:5-66 + LS_SimpleBound.cpp:6-110 (single-device
+                              signatures in the reference; SCHEME_IDX = 2 doubles {IDX, IDY}; not in batches) */
 };
 enum ibtfo_time { IBTFO_SINGLESTEP = 0, IBTFO_RK2 = 1 };
 /* Global_Boundary policies: the reference hook (Test/Sod/Global_Boudary.cpp:24-75),

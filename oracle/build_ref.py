@@ -6,6 +6,8 @@ The reference selects Scheme / Riemann / Limiter / Boundary at link time
 use, the reference sources where they lie under /root/reference together with
 oracle/ref_harness.cpp, oracle/ref_limiter.cpp and the chosen problem hooks, into
 oracle/_ref/libref_<case>_<scheme>_<riemann>_<limiter>_<boundary>.so.
+SimpleBound's sources still define the single-device signatures; for it
+oracle/ref_simplebound.cpp adds the current signatures, forwarding to them.
 
 Compiler: g++ -O2 -fopenacc (OpenACC host fallback, sequential) with
 -ffp-contract=off, so the fixture arithmetic is plain IEEE double in source order.
@@ -37,7 +39,8 @@ CASE_HOOKS = {
     "vortex": [TEST + "/Vortex/Initialize.cpp", TEST + "/Vortex/Level_Set_function.cpp",
                TEST + "/Vortex/Global_Boudary.cpp"],
 }
-NUM_SCH = {"Regular": 1, "IDWBaseGFM": 8, "LinearBaseGFM": 4, "RiemannBaseGFM": 4, "BCGFM": 16, "HBCGFM": 16}
+NUM_SCH = {"Regular": 1, "IDWBaseGFM": 8, "LinearBaseGFM": 4, "RiemannBaseGFM": 4, "BCGFM": 16, "HBCGFM": 16,
+           "SimpleBound": 2}
 
 SCHEMES = ["MUSCL", "MHM", "Godunov", "Lax"]
 RIEMANNS = ["HLL", "HLLC", "PVRS", "StegerWarming"]
@@ -59,6 +62,8 @@ def combos():
     out.append(("incline", "MHM", "HLL", "minmod", "IDWBaseGFM"))
     out.append(("incline", "MUSCL", "StegerWarming", "minmod", "IDWBaseGFM"))
     out.append(("vortex", "MUSCL", "HLL", "minmod", "IDWBaseGFM"))
+    out.append(("incline", "MUSCL", "HLL", "minmod", "SimpleBound"))
+    out.append(("vortex", "MUSCL", "HLL", "minmod", "SimpleBound"))
     return out
 
 
@@ -81,6 +86,8 @@ def sources(c):
         srcs += sorted(os.path.join(rdir, f) for f in os.listdir(rdir) if f.endswith(".cpp"))
     srcs += CASE_HOOKS[case]
     srcs += [os.path.join(HERE, "ref_harness.cpp"), os.path.join(HERE, "ref_limiter.cpp")]
+    if bound == "SimpleBound":  # its sources keep the single-device signatures
+        srcs.append(os.path.join(HERE, "ref_simplebound.cpp"))
     return srcs
 
 

@@ -39,7 +39,8 @@ namespace {
 enum { SCH_LAX = 0, SCH_MUSCL = 1, SCH_MHM = 2, SCH_GODUNOV = 3 };
 enum { RS_HLL = 0, RS_HLLC = 1, RS_PVRS = 2, RS_SW = 3 };
 enum { LIM_MINMOD = 0, LIM_VANLEER = 1, LIM_VANALBADA = 2 };
-enum { BC_REGULAR = 0, BC_IDW = 1, BC_LINEAR = 2, BC_RIEMANN = 3, BC_BCGFM = 4, BC_HBCGFM = 5, BC_CUT = 6 };
+enum { BC_REGULAR = 0, BC_IDW = 1, BC_LINEAR = 2, BC_RIEMANN = 3, BC_BCGFM = 4, BC_HBCGFM = 5, BC_CUT = 6,
+       BC_SIMPLE = 7 };
 enum { EDGE_REFERENCE = 0, EDGE_PERIODIC = 1, EDGE_INFLOW = 2 };
 
 // Sound speed. The reference computes std::pow(x, 0.5) (HLL/Riemann_solver.cpp:144,
@@ -953,6 +954,7 @@ int or_sch_width(int bc) {
         case BC_IDW: case BC_CUT: return 8;
         case BC_LINEAR: case BC_RIEMANN: return 4;
         case BC_BCGFM: case BC_HBCGFM: return 16;
+        case BC_SIMPLE: return 2;
         default: return 1;
     }
 }
@@ -963,6 +965,29 @@ void or_scheme_index(int bc, int nx, int ny, int ng, const double* XY, double* S
     Grid g(nx, ny, ng);
     int W = or_sch_width(bc);
     if (bc == BC_REGULAR) return;
+    if (bc == BC_SIMPLE) {
+        // SimpleBound/Scheme_Index.cpp:12-66: per ghost (type > 1), the nearest
+        // fluid cell (type < 1) within ng cells along -x (else +x) and along -y
+        // (else +y); k steps away gives the mirrored donor offset 1 - 2k (2k - 1)
+        auto t = [&](int i, int j) { return XY[g.c(i, j) * 6 + 5]; };
+        for (int j = ng; j < g.N - ng; j++)
+            for (int i = ng; i < nx + ng; i++) {
+                double* s = S + g.c(i, j) * W;
+                s[0] = 0; s[1] = 0;
+                if (!(t(i, j) > 1)) continue;
+                for (int k = 0; k < ng + 1; k++)
+                    if (t(i - k, j) < 1) { s[0] = 1 - 2 * k; break; }
+                if (s[0] == 0)
+                    for (int k = 0; k < ng + 1; k++)
+                        if (t(i + k, j) < 1) { s[0] = 2 * k - 1; break; }
+                for (int k = 0; k < ng + 1; k++)
+                    if (t(i, j - k) < 1) { s[1] = 1 - 2 * k; break; }
+                if (s[1] == 0)
+                    for (int k = 0; k < ng + 1; k++)
+                        if (t(i, j + k) < 1) { s[1] = 2 * k - 1; break; }
+            }
+        return;
+    }
     for (int j = ng; j < g.N - ng; j++)
         for (int i = ng; i < nx + ng; i++) {
             const double* p = XY + g.c(i, j) * 6;
@@ -1007,6 +1032,37 @@ void or_boundary(int bc, int nx, int ny, int ng, double gam, double* U_OLD, doub
         for (int j = ng; j < g.N - ng; j++)
             for (int i = ng; i < nx + ng; i++)
                 for (int k = 0; k < 4; k++) U_OLD[g.c(i, j) * 4 + k] = U_NEW[g.c(i, j) * 4 + k];
+        return;
+    }
+    if (bc == BC_SIMPLE) {
+        // SimpleBound/LS_SimpleBound.cpp:15-110: fluid cells U_OLD <- U_NEW, then each
+        // ghost from its x and/or y donor of U_OLD with the normal velocity
+        // component negated (the mean of both reflections when both exist), in
+        // the reference's sequential order (i outer, j inner; a ghost donor read
+        // after its own fill holds the new value)
+        for (int j = ng; j < g.N - ng; j++)
+            for (int i = ng; i < nx + ng; i++)
+                if (XY[g.c(i, j) * 6 + 5] == 0)
+                    for (int k = 0; k < 4; k++) U_OLD[g.c(i, j) * 4 + k] = U_NEW[g.c(i, j) * 4 + k];
+        for (int i = ng; i < nx + ng; i++)
+            for (int j = ng; j < g.N - ng; j++) {
+                if (!(XY[g.c(i, j) * 6 + 5] > 1)) continue;
+                const double* s = S + g.c(i, j) * W;
+                const int IDX = (int)s[0], IDY = (int)s[1];
+                double rx = 0, ux = 0, vx = 0, px = 0, ry = 0, uy = 0, vy = 0, py = 0, rw, uw, vw, pw;
+                if (IDX != 0) cons_to_prim(gam, U_OLD + g.c(i + IDX, j) * 4, rx, ux, vx, px);
+                if (IDY != 0) cons_to_prim(gam, U_OLD + g.c(i, j + IDY) * 4, ry, uy, vy, py);
+                if (IDX != 0 && IDY != 0) {
+                    rw = 0.5 * (rx + ry); uw = 0.5 * (-ux + uy); vw = 0.5 * (vx - vy); pw = 0.5 * (px + py);
+                } else if (IDX != 0) {
+                    rw = rx; uw = -ux; vw = vx; pw = px;
+                } else if (IDY != 0) {
+                    rw = ry; uw = uy; vw = -vy; pw = py;
+                } else {
+                    continue;
+                }
+                prim_to_cons(gam, rw, uw, vw, pw, U_OLD + g.c(i, j) * 4);
+            }
         return;
     }
     for (int j = ng; j < g.N - ng; j++)

@@ -23,7 +23,7 @@ SCHEMES = {"Lax": 0, "MUSCL": 1, "MHM": 2, "Godunov": 3}
 RIEMANNS = {"HLL": 0, "HLLC": 1, "PVRS": 2, "StegerWarming": 3}
 LIMITERS = {"minmod": 0, "VanLeer": 1, "VanAlbada": 2}
 BOUNDS = {"Regular": 0, "IDWBaseGFM": 1, "LinearBaseGFM": 2, "RiemannBaseGFM": 3, "BCGFM": 4, "HBCGFM": 5,
-          "CutCell": 6}
+          "CutCell": 6, "SimpleBound": 7}
 EDGES = {"reference": 0, "periodic": 1, "inflow": 2}
 TIMES = {"SingleStep": 0, "RK2": 1}
 # level-set hooks and initialisers (see or_level_set_hook / or_initialize)

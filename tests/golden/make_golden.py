@@ -43,6 +43,11 @@ def run_cases():
                                            Riemann="StegerWarming", case="incline"), 10))
     cases.append(("vortex_IDW", O.Case(N_x=28, N_y=28, hi_y=1.0, Boundary="IDWBaseGFM", case="vortex",
                                        extra=dict(fa=1.0, f0=0.0, f1=1.0)), 10))
+    # SimpleBound (its sources keep the single-device signatures; oracle/ref_simplebound.cpp adapts them)
+    cases.append(("incline_SimpleBound", O.Case(N_x=64, N_y=64, hi_y=1.0, Boundary="SimpleBound", case="incline",
+                                                Psy_time=0.3), 10))
+    cases.append(("vortex_SimpleBound", O.Case(N_x=28, N_y=28, hi_y=1.0, Boundary="SimpleBound", case="vortex",
+                                               extra=dict(fa=1.0, f0=0.0, f1=1.0)), 10))
     return cases
 
 

@@ -61,7 +61,8 @@ def test_defaults_are_reference_driver_defaults():
 
 def test_bad_config_rejected_before_any_device_call():
     for kw in [dict(num_ghost_cell=1), dict(N_x=2), dict(scheme=9), dict(gamma=1.0), dict(Time="RK2", Scheme="Lax"),
-               dict(Precision="mixed", quirks=1), dict(Precision="mixed", quirks=0, Scheme="Lax"), dict(precision=7)]:
+               dict(Precision="mixed", quirks=1), dict(Precision="mixed", quirks=0, Scheme="Lax"), dict(precision=7),
+               dict(boundary=8)]:
         cfg = ib.config(**kw)
         h = C.c_void_p()
         rc = ib.lib().ibtfo_create(C.byref(cfg), C.byref(h))

@@ -60,7 +60,7 @@ def test_sod_400x100_full(oracle_mod):
     assert abs(g["mass_log"][-1] - p["mass"][-1]) <= 1e-12 * abs(p["mass"][-1])
 
 
-@pytest.mark.parametrize("bound", ["IDWBaseGFM", "LinearBaseGFM", "RiemannBaseGFM", "BCGFM", "HBCGFM"])
+@pytest.mark.parametrize("bound", ["IDWBaseGFM", "LinearBaseGFM", "RiemannBaseGFM", "BCGFM", "HBCGFM", "SimpleBound"])
 def test_incline_gfm_bitexact(oracle_mod, bound):
     O = oracle_mod
     c = _case(O, N_x=96, N_y=96, hi_y=1.0, Boundary=bound, case="incline", Psy_time=0.3)
@@ -80,6 +80,59 @@ def test_incline_gfm_bitexact(oracle_mod, bound):
     assert g["steps"] == o["steps"]
     assert nan_equal_bits(g["U_OLD"], o["U_OLD"]) == 0
     assert W == O.lib().or_sch_width(O.BOUNDS[bound])
+
+
+def _simple_levels(st, M, N, ng=2):
+    """SimpleBound ghosts by dependency level: a ghost whose x or y donor is a
+    ghost filled earlier in the reference's order (i outer, j inner) sits one
+    level above that donor."""
+    t = st["XY"][5::6].reshape(N, M)
+    S = st["S"].reshape(N, M, 2)
+    ghosts = [(i, j) for i in range(ng, M - ng) for j in range(ng, N - ng) if t[j, i] == 2]
+    pos = {g: k for k, g in enumerate(ghosts)}
+    lev = {}
+    for g in ghosts:
+        i, j = g
+        L = 0
+        for d in [(i + int(S[j, i, 0]), j) if S[j, i, 0] != 0 else None,
+                  (i, j + int(S[j, i, 1])) if S[j, i, 1] != 0 else None]:
+            if d is not None and d in pos and pos[d] < pos[g]:
+                L = max(L, lev[d] + 1)
+        lev[g] = L
+    return np.bincount(list(lev.values()))
+
+
+@pytest.mark.parametrize("geom", ["vortex", "slits"])
+def test_simplebound_bitexact(oracle_mod, geom):
+    """SimpleBound on the annulus and on a geometry with one-cell slits, where a
+    ghost's donor across the slit is a ghost filled earlier in the reference's
+    sequential order (so it must be read with its new value): cell types, the
+    {IDX, IDY} records and 30 steps == the oracle bit for bit."""
+    O = oracle_mod
+    if geom == "vortex":
+        c = _case(O, N_x=64, N_y=64, hi_y=1.0, Boundary="SimpleBound", case="vortex",
+                  extra=dict(fa=1.0, f0=0.0, f1=1.0))
+    else:
+        n = 48
+        c = _case(O, N_x=n, N_y=n, hi_y=1.0, Boundary="SimpleBound", case="uniform", edge="periodic", quirks=0,
+                  extra=dict(rho=1.0, u=0.5, v=0.2, p=1.0), Psy_time=10.0)
+        h = 1.0 / n
+        x = -2 * h + h / 2 + np.arange(c.M) * h
+        X, Y = np.meshgrid(x, x[: c.N])
+        phi = np.where(X < 0.3, 0.3 - X, -1.0)
+        phi = np.where((np.abs(X - (0.6 + h / 2)) < 0.5 * h) & (Y > 0.2) & (Y < 0.8), 0.25 * h, phi)
+        phi = np.where((np.abs(X - 0.8) < 1.6 * h) & (np.abs(Y - 0.5) < 0.5 * h), 0.25 * h, phi)
+        c.phi = phi.reshape(-1)
+        levels = _simple_levels(O.setup(c), c.M, c.N)
+        assert len(levels) >= 2 and levels[1] > 0, levels
+    g = gpu_run(c, 30)
+    o = O.run_oracle(c, 30, root_mode=1)
+    assert g["steps"] == o["steps"] == 30
+    assert nan_equal_bits(g["XY"][5::6], o["XY"][5::6]) == 0
+    assert nan_equal_bits(g["S"], o["S"]) == 0
+    assert np.isfinite(o["U_OLD"]).all()
+    assert nan_equal_bits(g["U_OLD"], o["U_OLD"]) == 0
+    assert nan_equal_bits(g["dt_log"], o["dt"]) == 0
 
 
 def test_incline_mhm_sw(oracle_mod):

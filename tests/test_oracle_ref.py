@@ -35,6 +35,15 @@ def test_incline_gfm(oracle_mod, bound):
     _cmp(O, c, 40)
 
 
+@pytest.mark.parametrize("case,n", [("incline", 64), ("incline", 100), ("vortex", 48)])
+def test_simplebound(oracle_mod, case, n):
+    """SimpleBound (axis-direction donors, in-place sequential fill) against the
+    reference's SimpleBound sources built through the signature adapter."""
+    O = oracle_mod
+    kw = dict(Psy_time=0.3) if case == "incline" else dict(extra=dict(fa=1.0, f0=0.0, f1=1.0))
+    _cmp(O, O.Case(N_x=n, N_y=n, hi_y=1.0, Boundary="SimpleBound", case=case, **kw), 40)
+
+
 def test_sod_400x100_full_run(oracle_mod):
     """The 400x100 Sod tube (MHM + minmod + HLLC) with the reference's row-split Sod hook replaced by the
     x-split hook (the same hook compiled into both sides)."""
