@@ -63,7 +63,7 @@ def lib():
                      "ibtfo_download_scheme_index", "ibtfo_sch_width", "ibtfo_num_ghosts", "ibtfo_dims",
                      "ibtfo_time_steps", "ibtfo_device_state", "ibtfo_set_stream", "ibtfo_set_rows",
                      "ibtfo_tile", "ibtfo_autotune", "ibtfo_geometry_mask", "ibtfo_geometry_png",
-                     "ibtfo_png_decode", "ibtfo_cut_counts"]:
+                     "ibtfo_png_decode", "ibtfo_cut_counts", "ibtfo_write_vtk", "ibtfo_vtk_write_host"]:
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -99,6 +99,15 @@ def config(**kw):
         else:
             setattr(c, k, v)
     return c
+
+
+
+def write_vtk_host(cfg, iter_, now_t, U_OLD, XYCOORD, directory):
+    """WriteData's files for host arrays in the reference's AoS layout (no GPU needed)."""
+    U = np.ascontiguousarray(U_OLD, dtype=np.float64)
+    XY = np.ascontiguousarray(XYCOORD, dtype=np.float64)
+    _check(lib().ibtfo_vtk_write_host(C.byref(cfg), int(iter_), C.c_double(now_t), ptr(U), ptr(XY),
+                                      str(directory).encode()), "ibtfo_vtk_write_host")
 
 
 def png_decode(path):
@@ -205,6 +214,11 @@ class Solver:
 
     def num_ghosts(self):
         return lib().ibtfo_num_ghosts(self.h)
+
+    def write_vtk(self, directory, iter_, now_t):
+        """WriteData's files (dat_0_<iter>.vtk, result.visit appended) for the current state."""
+        _check(lib().ibtfo_write_vtk(self.h, str(directory).encode(), int(iter_), C.c_double(now_t)),
+               "ibtfo_write_vtk")
 
     def cut_counts(self):
         """(cut cells, small-cell merge groups) of the cut-cell boundary."""

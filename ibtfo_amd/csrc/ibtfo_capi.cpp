@@ -235,6 +235,17 @@ extern "C" int ibtfo_sch_width(const ibtfo_ctx* x) {
     }
 }
 extern "C" int ibtfo_num_ghosts(const ibtfo_ctx* x) { return x ? x->nghost : -1; }
+extern "C" int ibtfo_write_vtk(ibtfo_ctx* x, const char* dir, int iter, double now_t) {
+    if (!x) { set_error("null context"); return IBTFO_E_ARG; }
+    if (!x->have_geometry || !x->have_state) { set_error("ibtfo_write_vtk needs geometry and state"); return IBTFO_E_STATE; }
+    const size_t cells = (size_t)x->L.M * x->L.N;
+    std::vector<double> U(cells * 4), XY(cells * 6);
+    int rc = ibtfo_download_state(x, U.data());
+    if (!rc) rc = ibtfo_download_geometry(x, XY.data());
+    if (!rc) rc = write_vtk(&x->cfg, iter, now_t, U.data(), XY.data(), dir);
+    return rc;
+}
+
 extern "C" int ibtfo_cut_counts(const ibtfo_ctx* x, int* cut_cells, int* merge_groups) {
     if (!x) { set_error("null context"); return IBTFO_E_ARG; }
     if (cut_cells) *cut_cells = x->ncut;

@@ -4,6 +4,8 @@
 #include <map>
 #include <string>
 
+struct ibtfo_config;  // include/ibtfo.h
+
 namespace ibtfo {
 
 void set_error(const char* fmt, ...);
@@ -15,6 +17,10 @@ bool inp_strings(const char* path, std::map<std::string, std::string>& out);
 // x/y/type per cell come from the geometry.
 int host_initialize(int kind, const double* prm, int nprm, int nx, int ny, int ng, double gam, const double* x,
                     const double* y, const unsigned char* type, double* U);
+
+// WriteData's files for host arrays in the reference's AoS layout (ibtfo_vtk.cpp)
+int write_vtk(const ::ibtfo_config* c, int iter, double now_t, const double* U, const double* XY,
+              const char* dir);
 
 // First-fit sub-allocator over one device allocation: the batch engine's HBM
 // budget. Admission and retirement of solves allocate from it without any

@@ -12,6 +12,8 @@
 //   Global_Boundary Src/BoundaryCondition/Boundary.H:22-29  -> applied by the step
 //   ComputeDt     Src/Base/Common/ComputeDt.H:12-23         -> dt kept by the step
 //   CaculateMass  Src/Utils/CaculateMass.H:14-23            -> mass kept by the step
+//   WriteData     Src/Utils/WriteData.H:14-27               -> state copied to the
+//                 host U_OLD (the reference's update self), then the same files
 //   Scheme_Index  Src/BoundaryCondition/Scheme_Index.H:13-19 -> built with the geometry
 //   Level_Set     Src/Geometry/Level_Set.H:12-18             -> the driver's own
 //                 Level_Set_function hook fills phi on the host; classification,
@@ -198,4 +200,12 @@ void CaculateMass(const double, const double, const int, const int, const int, d
     double m = 0;
     check(ibtfo_mass(b->ctx, &m), "CaculateMass");
     MASS_DEVICE[device] = m;
+}
+
+void WriteData(const double, const double, const double, const double, const int, const int, const int, int iter,
+               double now_t, const double, double* U_OLD, double* XYCOORD, const int ndevices, const int device) {
+    one_device(ndevices, device, "WriteData");
+    Bound* b = find(U_OLD, "WriteData");
+    if (b->stepped) check(ibtfo_download_state(b->ctx, b->U_OLD), "WriteData");
+    check(ibtfo_vtk_write_host(&b->cfg, iter, now_t, U_OLD, XYCOORD, nullptr), "WriteData");
 }

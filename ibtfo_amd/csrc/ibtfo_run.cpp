@@ -3,10 +3,15 @@
 // MI355X library: read the .inp, geometry, Initialize, [Boundary], ComputeDt,
 // then the step loop, printing the reference's per-step line.
 //
-//   ibtfo_run case.inp [--chunk K] [--quiet]
+//   ibtfo_run case.inp [--chunk K] [--quiet] [--vtk DIR]
+//
+// With --vtk, WriteData's files go to DIR at the reference driver's cadence
+// (Test/Sod/Sod.cpp:135-150, :186): every plot_per steps (plot_int = 0), or
+// whenever now_t passes the next of plot_int equal time marks, and at the end.
 //
 // Problem hooks come from .inp keys (the reference picks them by which test
 // directory is compiled): Case = sod | sodx | incline | vortex | uniform | shu.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -44,15 +49,18 @@ static std::string str(const char* inp, const char* key, const char* def) {
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        fprintf(stderr, "usage: %s case.inp [--chunk K] [--quiet]\n", argv[0]);
+        fprintf(stderr, "usage: %s case.inp [--chunk K] [--quiet] [--vtk DIR]\n", argv[0]);
         return 2;
     }
     const char* inp = argv[1];
     int chunk = 64, quiet = 0;
+    const char* vtk = nullptr;
     for (int a = 2; a < argc; a++) {
         if (!strcmp(argv[a], "--chunk") && a + 1 < argc) chunk = atoi(argv[++a]);
         else if (!strcmp(argv[a], "--quiet")) quiet = 1;
+        else if (!strcmp(argv[a], "--vtk") && a + 1 < argc) vtk = argv[++a];
     }
+    if (chunk < 1) chunk = 1;
     ibtfo_config cfg;
     CK(ibtfo_config_from_inp(inp, &cfg));
     printf(" ==== parameters are read ====\n");
@@ -98,10 +106,30 @@ int main(int argc, char** argv) {
 
     int iter = 0;
     double now_t = 0, dt = 0;
+    const int plot_per = vtk ? std::max(1, (int)num(inp, "plot_per", 100)) : 0;
+    const int plot_int = vtk ? (int)num(inp, "plot_int", 0) : 0;
+    int output_int = 0;
     std::vector<double> ldt(chunk), lt(chunk), lm(chunk);
     while (true) {
         int before = iter;
-        CK(ibtfo_step_graph(ctx, chunk, chunk));
+        int n_try = chunk;
+        if (vtk) {
+            if (plot_int == 0) {
+                if (iter % plot_per == 0) {
+                    printf(" ==== Writing Data Wait ====\n");
+                    CK(ibtfo_write_vtk(ctx, vtk, iter, now_t));
+                }
+                n_try = std::min(chunk, plot_per - iter % plot_per);
+            } else {
+                if (now_t > output_int * cfg.Psy_time / plot_int || iter == 0) {
+                    printf(" ==== Writing Data Wait ====\n");
+                    CK(ibtfo_write_vtk(ctx, vtk, iter, now_t));
+                    output_int = output_int + 1 * (iter > 0);
+                }
+                n_try = 1;  // the next mark depends on each step's dt
+            }
+        }
+        CK(ibtfo_step_graph(ctx, n_try, chunk));
         int rc = ibtfo_status(ctx, &iter, &now_t, &dt);
         if (rc && rc != IBTFO_E_NONFINITE) CK(rc);
         int n = iter - before;
@@ -111,8 +139,9 @@ int main(int argc, char** argv) {
                 printf("iter : %d.   dt :%g.   time :%g.  Mass loss :%g\n", before + q + 1, ldt[q], lt[q],
                        std::fabs(lm[q] - mass0));
         }
-        if (n < chunk) break;  // the device stopped the loop
+        if (n < n_try) break;  // the device stopped the loop
     }
+    if (vtk) CK(ibtfo_write_vtk(ctx, vtk, iter, now_t));
     printf(" ==== done: %d steps, t = %.9g ====\n", iter, now_t);
     CK(ibtfo_destroy(ctx));
     return 0;

@@ -154,6 +154,18 @@ int ibtfo_num_ghosts(const ibtfo_ctx* ctx);
  * counterpart: the reference has no cut-cell method. */
 int ibtfo_cut_counts(const ibtfo_ctx* ctx, int* cut_cells, int* merge_groups);
 
+/* WriteData (Src/Utils/WriteData.cpp:5-247): the reference's output files for
+ * the current state, byte for byte -- dat_0_<iter>.vtk (legacy-ASCII VTK
+ * STRUCTURED_POINTS: density, x-vel, y-vel, pressure, Energy, Mach_Number,
+ * velocity of the interior cells as "%f" lines; -1 / -100 on non-fluid cells)
+ * and result.visit, appended to ("!NBLOCKS 1" first when iter == 0). Files go
+ * to `dir` (NULL or "" = the working directory). ibtfo_write_vtk downloads the
+ * state and geometry of a solve; ibtfo_vtk_write_host formats host arrays in
+ * the reference's AoS layout (no device needed). */
+int ibtfo_write_vtk(ibtfo_ctx* ctx, const char* dir, int iter, double now_t);
+int ibtfo_vtk_write_host(const ibtfo_config* cfg, int iter, double now_t, const double* U_OLD,
+                         const double* XYCOORD, const char* dir);
+
 /* Up to nsteps steps of the reference loop body (Test/Sod/Sod.cpp:154-177):
  * TimeAdvance(dt); Boundary; Global_Boundary; dt = ComputeDt; mass; now_t += dt;
  * iter++ -- stopping, on the device, when now_t >= Psy_time or iter >= max_iter.

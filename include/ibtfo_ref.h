@@ -55,6 +55,9 @@ void ComputeDt(const double Psy_L, const double Psy_H, const int N_x, const int 
 void CaculateMass(const double Psy_L, const double Psy_H, const int N_x, const int N_y, const int num_ghost_cell,
                   double* U_OLD, double* XYCOORD, double* MASS_DEVICE, const int ndevices,
                   const int device);                                     /* Src/Utils/CaculateMass.H:14-23 */
+void WriteData(const double lo_x, const double lo_y, const double Psy_L, const double Psy_H, const int N_x,
+               const int N_y, const int num_ghost_cell, int iter, double now_t, const double gamma, double* U_OLD,
+               double* XYCOORD, const int ndevices, const int device);    /* Src/Utils/WriteData.H:14-27 */
 #endif
 
 #endif /* IBTFO_REF_H_ */

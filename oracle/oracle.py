@@ -264,6 +264,31 @@ def write_inp(case, path=None):
     return path
 
 
+def ref_write_vtk(case, U, XY, iter_, now_t, directory):
+    """The reference's WriteData (oracle/_ref/libref_writedata.so) on host arrays,
+    writing dat_0_<iter>.vtk and appending to result.visit in `directory`."""
+    L = ref_lib_writedata()
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    XY = np.ascontiguousarray(XY, dtype=np.float64)
+    here = os.getcwd()
+    os.chdir(directory)
+    try:
+        L.ref_write_data(C.c_double(case.lo_x), C.c_double(case.lo_y), C.c_double(case.L), C.c_double(case.H),
+                         case.N_x, case.N_y, case.num_ghost_cell, int(iter_), C.c_double(now_t),
+                         C.c_double(case.gamma), ptr(U), ptr(XY))
+    finally:
+        os.chdir(here)
+
+
+def ref_lib_writedata():
+    name = "libref_writedata.so"
+    if name not in _ref_cache:
+        L = C.CDLL(os.path.join(REF_DIR, name), mode=os.RTLD_LOCAL | os.RTLD_NOW)
+        L.ref_write_data.restype = None
+        _ref_cache[name] = L
+    return _ref_cache[name]
+
+
 def run_ref(case, nsteps):
     """Run the reference translation units on the same case (edge policy and
     hooks are the reference's own)."""

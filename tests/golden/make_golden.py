@@ -99,7 +99,34 @@ def main():
     np.savez_compressed(os.path.join(HERE, "runs.npz"), **arrays)
     with open(os.path.join(HERE, "runs.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
+
+    # WriteData (Src/Utils/WriteData.cpp): the files the reference writes for one
+    # state, at iterations 0 and 10 (result.visit gets both entries)
+    c, U, XY, t = vtk_case()
+    vdir = os.path.join(HERE, "vtk")
+    os.makedirs(vdir, exist_ok=True)
+    for f in os.listdir(vdir):
+        os.remove(os.path.join(vdir, f))
+    O.ref_write_vtk(c, U, XY, 0, 0.0, vdir)
+    O.ref_write_vtk(c, U, XY, 10, t, vdir)
+    np.savez_compressed(os.path.join(HERE, "vtk_state.npz"), U=U, XY=XY, t=np.array([t]))
+    print("golden vtk", sorted(os.listdir(vdir)), flush=True)
     return 0
+
+
+def vtk_case():
+    """An inclined-tube state after 10 steps (28 x 20 cells) with a near-vacuum
+    density, a non-finite value and a negative pressure planted in fluid cells,
+    so the writer's clamp and special-value formatting are covered."""
+    c = O.Case(N_x=28, N_y=20, hi_y=1.0, Boundary="IDWBaseGFM", case="incline", Psy_time=0.3)
+    r = O.run_ref(c, 10)
+    U, XY = r["U_OLD"].copy(), r["XY"].copy()
+    fluid = np.flatnonzero(XY[5::6] == 0)
+    U[fluid[3] * 4] = 1e-9
+    U[fluid[10] * 4 + 1] = np.nan
+    U[fluid[20] * 4 + 3] = -5.0
+    U[fluid[30] * 4 + 2] = 1.25e7
+    return c, U, XY, float(r["t"][r["steps"] - 1])
 
 
 if __name__ == "__main__":

@@ -75,3 +75,31 @@ def test_reference_signature_driver(oracle_mod, tmp_path, boundary, riemann):
     assert nan_equal_bits(log[:, 0], o["dt"][:steps]) == 0
     assert nan_equal_bits(log[:, 1], o["t"][:steps]) == 0
     assert np.allclose(log[:, 2], o["mass"][:steps], rtol=c.M * c.N * 2.0 ** -53, atol=0)
+
+
+def test_ibtfo_run_writes_reference_vtk(oracle_mod, tmp_path):
+    """ibtfo_run --vtk: WriteData's files at the reference driver's cadence
+    (every plot_per steps from iteration 0, and after the loop), the last one
+    byte-identical to the reference format of the oracle's state."""
+    import ibtfo_amd as ib
+    exe = os.path.join(ROOT, "ibtfo_amd", "ibtfo_run")
+    assert os.path.exists(exe), "ibtfo_amd/ibtfo_run is not built"
+    O = oracle_mod
+    c = O.Case(N_x=64, N_y=48, hi_x=1.0, hi_y=1.0, Boundary="IDWBaseGFM", case="incline", Psy_time=0.3,
+               max_iter=23)
+    inp = O.write_inp(c, str(tmp_path / "case.inp"))
+    with open(inp, "a") as f:
+        f.write("Case = incline\nplot_per = 10\n")
+    vdir = tmp_path / "vtk"
+    vdir.mkdir()
+    print("ibtfo_run --vtk", flush=True)
+    r = subprocess.run([exe, inp, "--quiet", "--chunk", "8", "--vtk", str(vdir)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert (vdir / "result.visit").read_text() == "!NBLOCKS 1\ndat_0_0.vtk\ndat_0_10.vtk\ndat_0_20.vtk\ndat_0_23.vtk\n"
+    o = O.run_oracle(c, 23, root_mode=1)
+    ref = tmp_path / "ref"
+    ref.mkdir()
+    cfg = ib.config(N_x=64, N_y=48, hi_x=1.0, hi_y=1.0, gamma=1.4)
+    ib.write_vtk_host(cfg, 23, float(o["t"][-1]), o["U_OLD"], o["XY"], ref)
+    assert (vdir / "dat_0_23.vtk").read_bytes() == (ref / "dat_0_23.vtk").read_bytes()
