@@ -86,9 +86,17 @@ def gpu_leg(args, precision="FP64", keep_state=False, riemann=None):
     if args.graph_chunk > 0:  # capture the step sequence once; the timed loop replays it
         s.step(args.graph_chunk, graph_chunk=args.graph_chunk)
         s.status()
-    total_ms, kern_ms = s.time_steps(args.steps, kernel=True)
-    it, t, dt, rc = s.status()
+    # K timed steps in 5 equal slices (HIP events around each on the solve's
+    # stream): value = all K steps, plus the slices' median
     cells = args.nx * args.ny
+    nsl = 5 if args.steps >= 5 else 1
+    slices = [args.steps // nsl] * nsl
+    slices[-1] += args.steps - sum(slices)
+    slice_ms = [s.time_steps(n, kernel=False)[0] for n in slices]
+    total_ms = sum(slice_ms)
+    slice_rates = sorted(cells * n / (m / 1e3) for n, m in zip(slices, slice_ms))
+    _, kern_ms = s.time_steps(min(args.steps, args.kernel_steps), kernel=True)
+    it, t, dt, rc = s.status()
     ms_step = total_ms / args.steps
     value = cells * args.steps / (total_ms / 1e3)
     achieved = BYTES_PER_CELL * cells / (kern_ms / 1e3) / 1e9
@@ -100,7 +108,8 @@ def gpu_leg(args, precision="FP64", keep_state=False, riemann=None):
         if tr.get("workload") == workload_name(args):
             traffic = tr.get("bytes_per_launch")
     out = dict(value=value, ms_per_step=ms_step, kernel_ms=kern_ms, achieved=achieved, traffic=traffic,
-               iter=it, t=t, dt=dt, status=rc, nghost=nghost, ncut=ncut, ngroups=ngroups, rows=rows, grid=[nbx, nby],
+               iter=it, t=t, dt=dt, status=rc, nghost=nghost, ncut=ncut, ngroups=ngroups, rows=rows,
+               slice_rates=slice_rates, median=slice_rates[len(slice_rates) // 2], grid=[nbx, nby],
                tuned=tuned is not None)
     if keep_state:
         out["rho"] = s.state()[0::4].copy()
@@ -139,6 +148,119 @@ def cpu_leg(args):
     return dict(value=nx * ny * r["steps"] / el, cores=threads, steps=r["steps"], seconds=el,
                 sample="%dx%d cells, %d steps of the same workload (oracle, OpenMP %d threads)" % (
                     nx, ny, r["steps"], threads))
+
+
+def copy_bandwidth():
+    """Device-to-device copy rate on this GPU (GB/s, bytes read + written): a
+    1 GiB float64 copy, 10 repetitions, HIP events."""
+    import torch
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 10 * 2 * n * 8 / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
+
+
+def config_legs(args):
+    """The reference's other configurations, the GPU beside the CPU oracle: the
+    400x100 Sod tube (MHM + minmod + HLLC, to t = 0.2), the inclined Sod channel
+    (800x400, MUSCL + minmod + HLL, IDW ghost fluid, to t = 0.2) and the
+    isentropic vortex (1024^2, periodic, MUSCL + van Leer + HLL with Heun's RK2,
+    to t = 10, FP64 and mixed, L1 density error against the exact solution).
+    GPU: HIP events around the whole run (graph-replayed steps); CPU: the oracle
+    (OpenMP) on the whole run, or its first `cpu_steps` steps where stated."""
+    import ibtfo_amd as ib
+    from ibtfo_amd import workloads as W
+    from oracle import oracle as O
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    O.lib().or_set_num_threads(threads)
+    out = []
+
+    def gpu_run(cfg, ls, init, apply_boundary, chunk=100):
+        s = ib.Solver(cfg)
+        s.geometry(*ls)
+        s.initialize(*init)
+        s.prepare(apply_boundary)
+        s.step(10 ** 7, graph_chunk=chunk)  # untimed: how many steps the run takes
+        n = s.status()[0]
+        s.close()
+        s = ib.Solver(cfg)
+        s.geometry(*ls)
+        s.initialize(*init)
+        s.prepare(apply_boundary)
+        s.step(0, graph_chunk=chunk)  # capture only
+        ms, _ = s.time_steps(n, kernel=False)
+        it, t, dt, rc = s.status()
+        return s, n, ms, t, rc
+
+    def cpu_run(case, nsteps):
+        st = O.setup(case)
+        t0 = time.perf_counter()
+        r = O.run_oracle(case, nsteps, state=st)
+        return r["steps"], time.perf_counter() - t0
+
+    # 400x100 Sod tube
+    cfg = ib.config(N_x=400, N_y=100, hi_x=1.0, hi_y=0.25, Psy_time=0.2, max_iter=10 ** 9, CFL_number=0.5,
+                    Scheme="MHM", Riemann="HLLC", Limiter="minmod", Boundary="Regular")
+    s, n, ms, t, rc = gpu_run(cfg, ("none", ()), ("sod_x", (0.5, 1, 0, 0, 1, 0.125, 0, 0, 0.1)), False)
+    s.close()
+    c = O.Case(N_x=400, N_y=100, hi_y=0.25, Scheme="MHM", Riemann="HLLC", Limiter="minmod", case="sodx",
+               Psy_time=0.2)
+    cn, cs = cpu_run(c, 10 ** 6)
+    out.append(dict(workload="Sod tube 400x100 on [0,1]x[0,0.25], MHM + minmod + HLLC, Regular, to t = 0.2",
+                    steps=n, gpu_value=400 * 100 * n / (ms / 1e3), gpu_ms=ms, status=rc,
+                    cpu_value=400 * 100 * cn / cs, cpu_steps=cn, cpu_cores=threads))
+    # inclined Sod channel, 800x400, IDW
+    cfg = ib.config(N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Psy_time=0.2, max_iter=10 ** 9, CFL_number=0.5,
+                    Scheme="MUSCL", Riemann="HLL", Limiter="minmod", Boundary="IDWBaseGFM")
+    s, n, ms, t, rc = gpu_run(cfg, ("channel", (30.0, 0.4)),
+                              ("incline", (30.0, 1, 0, 0, 1, 0.125, 0, 0, 0.1)), True)
+    s.close()
+    c = O.Case(N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Boundary="IDWBaseGFM", case="incline", Psy_time=0.2)
+    cn, cs = cpu_run(c, args.cpu_incline_steps)
+    out.append(dict(workload="inclined Sod channel 800x400 on [0,2]x[0,1] (30 deg, AxisD 0.4), MUSCL + minmod "
+                             "+ HLL, IDW ghost fluid, to t = 0.2",
+                    steps=n, gpu_value=800 * 400 * n / (ms / 1e3), gpu_ms=ms, status=rc,
+                    cpu_value=800 * 400 * cn / cs, cpu_steps=cn, cpu_cores=threads,
+                    cpu_note="first %d steps of the run" % cn))
+    # isentropic vortex 1024^2, FP64 and mixed
+    nv = 1024
+    h = 10.0 / nv
+    xc = (np.arange(nv) + 0.5) * h
+    X, Y = np.meshgrid(xc, xc)
+    vort = {}
+    for prec in ["FP64", "mixed"]:
+        cfg = ib.config(N_x=nv, N_y=nv, hi_x=10.0, hi_y=10.0, Psy_time=10.0, max_iter=10 ** 9, CFL_number=0.5,
+                        Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer", Boundary="Regular", Time="RK2",
+                        Edge="periodic", quirks=0, precision=ib.PRECISION[prec])
+        s, n, ms, t, rc = gpu_run(cfg, ("none", ()), ("shu", (5.0, 5.0, 5.0, 1.0, 1.0, 1.0, 1.0)), False)
+        rho = s.state().reshape(nv + 4, nv + 4, 4)[2:-2, 2:-2, 0]
+        s.close()
+        vort[prec] = dict(steps=n, ms=ms, t=t, rc=rc,
+                          l1=float(np.mean(np.abs(rho - W.shu_exact(X, Y, t)))), rho=rho)
+    c = O.Case(N_x=nv, N_y=nv, hi_x=10.0, hi_y=10.0, Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer",
+               Time="RK2", case="shu", edge="periodic", quirks=0, Psy_time=10.0)
+    cn, cs = cpu_run(c, args.cpu_vortex_steps)
+    f, m = vort["FP64"], vort["mixed"]
+    out.append(dict(workload="isentropic vortex 1024x1024 on [0,10]^2 (strength 5, free stream (1,1,1,1)), "
+                             "periodic, MUSCL + VanLeer + HLL, RK2 (Heun), to t = 10",
+                    steps=f["steps"], gpu_value=nv * nv * f["steps"] / (f["ms"] / 1e3), gpu_ms=f["ms"],
+                    status=f["rc"], mixed_value=nv * nv * m["steps"] / (m["ms"] / 1e3),
+                    l1_rho_exact_fp64=f["l1"], l1_rho_exact_mixed=m["l1"], l1_ratio=m["l1"] / f["l1"],
+                    l1_rho_mixed_vs_fp64=float(np.mean(np.abs(m["rho"] - f["rho"]))),
+                    cpu_value=nv * nv * cn / cs, cpu_steps=cn, cpu_cores=threads,
+                    cpu_note="first %d steps of the run" % cn))
+    return out
 
 
 def _plain_solve(it):
@@ -366,6 +488,10 @@ def main():
     ap.add_argument("--geometry", default="mask", choices=["mask", "shapes"],
                     help="bodies as a binary mask (level set on the device) or as exact signed distances")
     ap.add_argument("--no-sw", action="store_true", help="skip the Steger-Warming leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other reference configurations")
+    ap.add_argument("--kernel-steps", type=int, default=500, help="steps of the per-launch kernel timing pass")
+    ap.add_argument("--cpu-vortex-steps", type=int, default=24)
+    ap.add_argument("--cpu-incline-steps", type=int, default=120)
     ap.add_argument("--rows", type=int, default=0, help="fixed rows per strip (skips the autotuner)")
     ap.add_argument("--no-tune", action="store_true", help="default tile shape, no autotuning")
     ap.add_argument("--tune-steps", type=int, default=20)
@@ -418,6 +544,8 @@ def main():
               "roofline_frac": w["achieved"] / HBM_PEAK_GBS, "tile_rows": w["rows"], "steps_taken": w["iter"],
               "status": w["status"]}
     serving = None if args.no_serve else serving_leg(args)
+    configs = None if args.no_configs else config_legs(args)
+    copy_gbs = copy_bandwidth()
     cpu = None
     if not args.no_cpu:
         c = cpu_leg(args)
@@ -444,11 +572,15 @@ def main():
         "roofline": {"bound": "hbm", "achieved": g["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": g["achieved"] / HBM_PEAK_GBS, "traffic": g["traffic"],
                      "kernel": "k_step<MUSCL,HLL,VanLeer>", "kernel_ms": g["kernel_ms"],
-                     "bytes_per_cell": BYTES_PER_CELL},
+                     "bytes_per_cell": BYTES_PER_CELL, "copy_gbs_measured": copy_gbs,
+                     "frac_of_copy": g["achieved"] / copy_gbs},
+        "timing": {"slices": len(g["slice_rates"]), "slice_values_sorted": g["slice_rates"],
+                   "median_slice_value": g["median"]},
         "cpu_baseline": cpu,
         "steger_warming": sw,
         "mixed_precision": mixed,
         "serving": serving,
+        "configs": configs,
     }
     print(json.dumps(line))
     return 0

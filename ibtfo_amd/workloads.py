@@ -77,6 +77,18 @@ def bodies_mask(N_x, N_y, ng=2, L=2.0, H=1.0, seed=20261202):
     return img
 
 
+def shu_exact(x, y, t, gam=1.4, beta=5.0, L=10.0):
+    """Density of the isentropic vortex (strength beta, centred at (5, 5) at t = 0)
+    advected by the free stream (1, 1) on the periodic [0, L]^2."""
+    xc = (5.0 + t) % L
+    yc = (5.0 + t) % L
+    dxv = (x - xc + L / 2) % L - L / 2
+    dyv = (y - yc + L / 2) % L - L / 2
+    r2 = dxv * dxv + dyv * dyv
+    T = 1 - (gam - 1) * beta * beta / (8 * gam * math.pi ** 2) * np.exp(1 - r2)
+    return T ** (1 / (gam - 1))
+
+
 # Mach 3 inflow (rho, u, v, p) of the 4096x2048 immersed-body workload
 MACH3_INFLOW = (1.4, 3.0, 0.0, 1.0)
 
