@@ -33,6 +33,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+def _say(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 BYTES_PER_CELL = 65.0
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = "traffic_step_mach3_mask_cutcell.json"
@@ -217,6 +221,7 @@ def config_legs(args):
     c = O.Case(N_x=400, N_y=100, hi_y=0.25, Scheme="MHM", Riemann="HLLC", Limiter="minmod", case="sodx",
                Psy_time=0.2)
     cn, cs = cpu_run(c, 10 ** 6)
+    _say("Sod tube measured")
     out.append(dict(workload="Sod tube 400x100 on [0,1]x[0,0.25], MHM + minmod + HLLC, Regular, to t = 0.2",
                     steps=n, gpu_value=400 * 100 * n / (ms / 1e3), gpu_ms=ms, status=rc,
                     cpu_value=400 * 100 * cn / cs, cpu_steps=cn, cpu_cores=threads))
@@ -228,6 +233,7 @@ def config_legs(args):
     s.close()
     c = O.Case(N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Boundary="IDWBaseGFM", case="incline", Psy_time=0.2)
     cn, cs = cpu_run(c, args.cpu_incline_steps)
+    _say("inclined channel measured")
     out.append(dict(workload="inclined Sod channel 800x400 on [0,2]x[0,1] (30 deg, AxisD 0.4), MUSCL + minmod "
                              "+ HLL, IDW ghost fluid, to t = 0.2",
                     steps=n, gpu_value=800 * 400 * n / (ms / 1e3), gpu_ms=ms, status=rc,
@@ -248,6 +254,7 @@ def config_legs(args):
         s.close()
         vort[prec] = dict(steps=n, ms=ms, t=t, rc=rc,
                           l1=float(np.mean(np.abs(rho - W.shu_exact(X, Y, t)))), rho=rho)
+    _say("isentropic vortex on the GPU measured")
     c = O.Case(N_x=nv, N_y=nv, hi_x=10.0, hi_y=10.0, Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer",
                Time="RK2", case="shu", edge="periodic", quirks=0, Psy_time=10.0)
     cn, cs = cpu_run(c, args.cpu_vortex_steps)
@@ -312,6 +319,7 @@ def serving_leg(args):
                         Precision="mixed", quirks=0, hbm_budget=args.serve_budget << 30, max_slots=args.serve_slots,
                         max_step_blocks=args.serve_step_blocks, max_ghost_blocks=4096, max_ring_blocks=8192,
                         rows=args.serve_rows, chunk=args.serve_chunk)
+    _say("plain per-solve path measured")
     # saturation: every request queued at t = 0
     b = S.Batch(bc)
     b.submit(**items32[0])
@@ -323,6 +331,7 @@ def serving_leg(args):
     b.close()
     sat_rate = len(items32) / sat["wall"]
     sat_cu = cells.sum() * args.serve_steps / sat["wall"]
+    _say("saturated batch measured")
     # Poisson arrivals at 0.8x the plain path's solves/s
     lam = 0.8 * plain_rate
     rng = np.random.default_rng(4321)
@@ -331,6 +340,7 @@ def serving_leg(args):
     pois = S.serve(b, items32, arrivals=arr)
     b.close()
     lat = pois["latency"] * 1e3
+    _say("Poisson-arrival batch measured")
     # L1 density gate on a sample: mixed (batched) and FP64 (plain) against a 2x-refined FP64 solve
     gate = []
     for i in range(min(args.gate_sample, len(stream))):
@@ -523,13 +533,20 @@ def main():
         return 2
     if args.pmc_probe:
         return pmc_probe(args)
+    t_start = time.perf_counter()
+
+    def progress(what):
+        print("bench: %s done (%.0f s)" % (what, time.perf_counter() - t_start), file=sys.stderr, flush=True)
+
     g = gpu_leg(args, keep_state=not args.no_mixed)
+    progress("FP64 leg")
     mixed = None
     if not args.no_mixed:
         m = gpu_leg(args, precision="mixed", keep_state=True)
         fluid = g["types"] == 0
         l1 = float(np.mean(np.abs(m["rho"][fluid] - g["rho"][fluid])))
         scale = float(np.mean(np.abs(g["rho"][fluid])))
+        progress("mixed-precision leg")
         mixed = {"form": "FP32 limiter slopes and Riemann fluxes; FP64 state, face-state primitives, update, "
                          "ghost fill and reductions",
                  "value": m["value"], "unit": "cell-updates/s", "ms_per_step": m["ms_per_step"],
@@ -543,12 +560,16 @@ def main():
               "kernel": "k_step<MUSCL,StegerWarming,VanLeer>", "kernel_ms": w["kernel_ms"],
               "roofline_frac": w["achieved"] / HBM_PEAK_GBS, "tile_rows": w["rows"], "steps_taken": w["iter"],
               "status": w["status"]}
+        progress("Steger-Warming leg")
     serving = None if args.no_serve else serving_leg(args)
+    progress("request-stream leg")
     configs = None if args.no_configs else config_legs(args)
+    progress("other configurations")
     copy_gbs = copy_bandwidth()
     cpu = None
     if not args.no_cpu:
         c = cpu_leg(args)
+        progress("CPU baseline")
         cpu = dict(value=c["value"], unit="cell-updates/s", cores=c["cores"], kind="port", sample=c["sample"])
     line = {
         "metric": "cell-updates/s (frac. HBM roofline); batched p50/p99 solve latency at fixed L1",

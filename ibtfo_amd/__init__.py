@@ -19,10 +19,10 @@ LIMITER = {"minmod": 0, "VanLeer": 1, "VanAlbada": 2}
 BOUNDARY = {"Regular": 0, "IDWBaseGFM": 1, "LinearBaseGFM": 2, "RiemannBaseGFM": 3, "BCGFM": 4, "HBCGFM": 5,
             "CutCell": 6, "SimpleBound": 7}
 TIME = {"SingleStep": 0, "RK2": 1}
-EDGE = {"reference": 0, "periodic": 1, "inflow": 2}
+EDGE = {"reference": 0, "periodic": 1, "inflow": 2, "pnggeo": 3}
 PRECISION = {"FP64": 0, "fp64": 0, "mixed": 1}
 LEVELSET = {"none": 0, "channel": 1, "annulus": 2, "field": 3, "ellipse": 4, "shapes": 5}
-INIT = {"sod_rows": 0, "incline": 1, "vortex": 2, "sod_x": 3, "uniform": 4, "shu": 5}
+INIT = {"sod_rows": 0, "incline": 1, "vortex": 2, "sod_x": 3, "uniform": 4, "shu": 5, "shock": 6}
 
 OK, E_ARG, E_HIP, E_NOMEM, E_IO, E_STATE, E_NONFINITE = range(7)
 

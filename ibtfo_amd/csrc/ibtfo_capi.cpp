@@ -88,7 +88,7 @@ static int validate(const ibtfo_config* c) {
     if (c->N_x < 4 || c->N_y < 2 * c->num_ghost_cell) { set_error("grid too small"); return IBTFO_E_ARG; }
     if ((long long)c->N_x * c->N_y > (1ll << 31)) { set_error("grid too large"); return IBTFO_E_ARG; }
     if (c->scheme < 0 || c->scheme > 3 || c->riemann < 0 || c->riemann > 3 || c->limiter < 0 || c->limiter > 2 ||
-        c->boundary < 0 || c->boundary > 7 || c->edge < 0 || c->edge > 2) {
+        c->boundary < 0 || c->boundary > 7 || c->edge < 0 || c->edge > 3) {
         set_error("slot value out of range");
         return IBTFO_E_ARG;
     }
@@ -695,7 +695,10 @@ static int write_ctl_head(ibtfo_ctx* x, const Ctl& c) {
 extern "C" int ibtfo_upload_state(ibtfo_ctx* x, const double* U_OLD, const double* U_NEW) {
     if (!x || !U_OLD) return IBTFO_E_ARG;
     HIPCHK(hipStreamSynchronize(x->stream));
+    // both buffers: ring cells that no edge policy writes (the PNGGeo hook's
+    // corners) keep the uploaded U_OLD in either
     int rc = upload_aos(x, U_OLD, 0);
+    if (!rc) rc = upload_aos(x, U_OLD, 1);
     if (rc) return rc;
     if (x->cfg.boundary == IBTFO_SIMPLE_BOUND) {
         if (!x->unew0 && dalloc(x, &x->unew0, 4 * x->L.plane * sizeof(double)) != hipSuccess) {

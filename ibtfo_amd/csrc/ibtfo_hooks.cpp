@@ -18,8 +18,8 @@ static inline void to_cons(double gam, double rho, double u, double v, double p,
 
 int host_initialize(int kind, const double* prm, int nprm, int nx, int ny, int ng, double gam, const double* xs,
                     const double* ys, const unsigned char* type, double* U) {
-    static const int need[] = {8, 9, 2, 9, 4, 7};
-    if (kind < 0 || kind > 5) { set_error("unknown initialiser %d", kind); return IBTFO_E_ARG; }
+    static const int need[] = {8, 9, 2, 9, 4, 7, 6};
+    if (kind < 0 || kind > 6) { set_error("unknown initialiser %d", kind); return IBTFO_E_ARG; }
     if (!prm || nprm < need[kind]) { set_error("initialiser %d needs %d parameters", kind, need[kind]); return IBTFO_E_ARG; }
     const int M = nx + 2 * ng, N = ny + 2 * ng;
     const double pi = M_PI;
@@ -85,6 +85,19 @@ int host_initialize(int kind, const double* prm, int nprm, int nx, int ny, int n
                 case IBTFO_INIT_UNIFORM:
                     to_cons(gam, prm[0], prm[1], prm[2], prm[3], u);
                     break;
+                case IBTFO_INIT_SHOCK: {  // Test/PNGGeo/Initialize.cpp:29-55, in its operation order
+                    const double Ms = prm[1], rho_1 = prm[2], u_1 = prm[3], v_1 = prm[4], p_1 = prm[5];
+                    if (x > prm[0]) {
+                        to_cons(gam, rho_1, u_1, v_1, p_1, u);
+                    } else {
+                        double rho_2 = rho_1 * (gam + 1) * Ms * Ms / (2 + (gam - 1) * Ms * Ms);
+                        double p_2 = p_1 * ((2 * gam) * Ms * Ms / (gam + 1) - (gam - 1) / (gam + 1));
+                        double u_2 = Ms * (1 - rho_1 / rho_2);
+                        u[0] = rho_2; u[1] = rho_2 * u_2; u[2] = 0;
+                        u[3] = p_2 / (gam - 1) + 0.5 * rho_2 * (u_2 * u_2);
+                    }
+                    break;
+                }
                 default: {  // isentropic vortex: T = 1 - (g-1) b^2/(8 g pi^2) e^(1-r^2), rho = T^(1/(g-1)), p = rho T
                     double dxv = x - prm[0], dyv = y - prm[1], r2 = dxv * dxv + dyv * dyv, beta = prm[2];
                     double T = 1 - (gam - 1) * beta * beta / (8 * gam * pi * pi) * exp(1 - r2);

@@ -172,8 +172,8 @@ extern "C" int ibtfo_config_from_inp(const char* path, ibtfo_config* c) {
                              IBTFO_BCGFM, IBTFO_HBCGFM, IBTFO_CUTCELL, IBTFO_SIMPLE_BOUND};
     static const char* tn[] = {"SingleStep", "RK2"};
     static const int tv[] = {IBTFO_SINGLESTEP, IBTFO_RK2};
-    static const char* en[] = {"reference", "periodic", "inflow"};
-    static const int ev[] = {IBTFO_EDGE_REFERENCE, IBTFO_EDGE_PERIODIC, IBTFO_EDGE_INFLOW};
+    static const char* en[] = {"reference", "periodic", "inflow", "pnggeo"};
+    static const int ev[] = {IBTFO_EDGE_REFERENCE, IBTFO_EDGE_PERIODIC, IBTFO_EDGE_INFLOW, IBTFO_EDGE_PNGGEO};
     static const char* pn[] = {"FP64", "mixed"};
     static const int pv[] = {IBTFO_FP64, IBTFO_MIXED};
     int rc;
@@ -182,9 +182,18 @@ extern "C" int ibtfo_config_from_inp(const char* path, ibtfo_config* c) {
     if ((rc = lookup(str, "Limiter", ln, lv, 3, &c->limiter))) return rc;
     if ((rc = lookup(str, "Boundary", bn, bv, 9, &c->boundary))) return rc;
     if ((rc = lookup(str, "Time", tn, tv, 2, &c->time))) return rc;
-    if ((rc = lookup(str, "Edge", en, ev, 3, &c->edge))) return rc;
+    if ((rc = lookup(str, "Edge", en, ev, 4, &c->edge))) return rc;
     if ((rc = lookup(str, "Precision", pn, pv, 2, &c->precision))) return rc;
     c->quirks = int(get("quirks", c->quirks));
+    if (c->edge == IBTFO_EDGE_PNGGEO) {
+        // Test/PNGGeo/Global_Boudary.cpp:61-66: the post-shock state of a Mach 1.3
+        // shock into rho = 1, p = 0.71429, in the hook's operation order
+        const double gamma = c->gamma, rho1 = 1.0, p1 = 0.71429, Ms = 1.3;
+        const double rho2 = rho1 * (gamma + 1) * Ms * Ms / (2 + (gamma - 1) * Ms * Ms);
+        const double p2 = p1 * ((2 * gamma) * Ms * Ms / (gamma + 1) - (gamma - 1) / (gamma + 1));
+        const double u2 = Ms * (1 - rho1 / rho2);
+        c->edge_state[0] = rho2; c->edge_state[1] = u2; c->edge_state[2] = 0.0; c->edge_state[3] = p2;
+    }
     c->edge_state[0] = get("inflow_rho", c->edge_state[0]);
     c->edge_state[1] = get("inflow_u", c->edge_state[1]);
     c->edge_state[2] = get("inflow_v", c->edge_state[2]);

@@ -10,7 +10,9 @@
 // whenever now_t passes the next of plot_int equal time marks, and at the end.
 //
 // Problem hooks come from .inp keys (the reference picks them by which test
-// directory is compiled): Case = sod | sodx | incline | vortex | uniform | shu.
+// directory is compiled): Case = sod | sodx | incline | vortex | uniform | shu |
+// pnggeo (Test/PNGGeo: the mask PNG named by mask_file, default Demo.png, and
+// the Mach 1.3 shock initialiser; its edge hook is Edge = pnggeo).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -89,6 +91,10 @@ int main(int argc, char** argv) {
     } else if (cs == "uniform") {
         init = IBTFO_INIT_UNIFORM;
         ip = {num(inp, "rho", 1), num(inp, "u", 0), num(inp, "v", 0), num(inp, "p", 1)};
+    } else if (cs == "pnggeo") {
+        init = IBTFO_INIT_SHOCK;
+        ip = {0.1, 1.3};
+        ip.insert(ip.end(), L.begin(), L.end());
     } else if (cs == "shu") {
         init = IBTFO_INIT_SHU_VORTEX;
         ip = {num(inp, "x0", 5), num(inp, "y0", 5), num(inp, "beta", 5), 1.0, num(inp, "u", 1), num(inp, "v", 1), 1.0};
@@ -96,7 +102,12 @@ int main(int argc, char** argv) {
         fprintf(stderr, "unknown Case '%s'\n", cs.c_str());
         return 2;
     }
-    CK(ibtfo_geometry(ctx, ls, lp.data(), (int)lp.size(), nullptr));
+    if (cs == "pnggeo") {
+        std::string png = str(inp, "mask_file", "Demo.png");
+        CK(ibtfo_geometry_png(ctx, png.c_str()));
+    } else {
+        CK(ibtfo_geometry(ctx, ls, lp.data(), (int)lp.size(), nullptr));
+    }
     printf(" ====  Geometry Initialize complete ====\n");
     CK(ibtfo_initialize(ctx, init, ip.data(), (int)ip.size()));
     CK(ibtfo_prepare(ctx, cfg.boundary != IBTFO_REGULAR));

@@ -938,6 +938,27 @@ __device__ __forceinline__ void edge_one(const EdgeArgs& a, const int q) {
         long long s = L.at(si, sj);
         const double* S = (a.unew_nf && a.type[s] != 0) ? a.unew_nf : U;
         for (int k = 0; k < 4; k++) v[k] = S[s + k * PL];
+    } else if (a.kind == 3) {
+        // Test/PNGGeo/Global_Boudary.cpp:14-79 (U_NEW = the state before the ghost
+        // fill's phase 2): rows as the Sod hook over the interior columns, the
+        // right columns zero-gradient, the left columns held at the inflow state;
+        // the ring corners are not written
+        const bool row = j < ng || j >= ny + ng;
+        if (row && (i < ng || i >= nx + ng)) return;
+        auto src = [&](long long s, int k) {
+            const double* S = (a.unew_nf && a.type[s] != 0) ? a.unew_nf : U;
+            return S[s + k * PL];
+        };
+        if (row) {
+            const int sr = j < ng ? 2 * ng - 1 : ny + ng - 1;
+            const int rr = j < ng ? 2 * ng - j - 1 : 2 * (ny + ng) - 1 - j;
+            for (int k = 0; k < 4; k++) v[k] = src(L.at(i, sr), k);
+            v[2] = -src(L.at(i, rr), 2);
+        } else if (i < ng) {
+            for (int k = 0; k < 4; k++) v[k] = a.inflow[k];
+        } else {
+            for (int k = 0; k < 4; k++) v[k] = src(L.at(nx + ng - 1, j), k);
+        }
     } else if (a.kind == 2) {  // inflow left, zero gradient elsewhere
         if (i < ng) {
             for (int k = 0; k < 4; k++) v[k] = a.inflow[k];

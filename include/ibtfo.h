@@ -55,7 +55,11 @@ enum ibtfo_boundary {
 enum ibtfo_time { IBTFO_SINGLESTEP = 0, IBTFO_RK2 = 1 };
 /* Global_Boundary policies: the reference hook (Test/Sod/Global_Boudary.cpp:24-75),
  * periodic in x and y, supersonic inflow at x = lo_x with zero-gradient elsewhere. */
-enum ibtfo_edge { IBTFO_EDGE_REFERENCE = 0, IBTFO_EDGE_PERIODIC = 1, IBTFO_EDGE_INFLOW = 2 };
+enum ibtfo_edge { IBTFO_EDGE_REFERENCE = 0, IBTFO_EDGE_PERIODIC = 1, IBTFO_EDGE_INFLOW = 2,
+                  IBTFO_EDGE_PNGGEO = 3 /* Test/PNGGeo/Global_Boudary.cpp:14-79: rows as the Sod hook,
+                                           right zero-gradient, left held at edge_state (the .inp
+                                           default is the hook's Mach 1.3 post-shock state), ring
+                                           corners untouched */ };
 enum ibtfo_precision { IBTFO_FP64 = 0, IBTFO_MIXED = 1 };
 
 /* level-set hooks (Level_Set_function user hook) */
@@ -77,7 +81,10 @@ enum ibtfo_init_kind {
     IBTFO_INIT_VORTEX = 2,    /* Test/Vortex/Initialize.cpp:15-98, prm {fa, fw} */
     IBTFO_INIT_SOD_X = 3,     /* Sod split at x = prm[0], prm[1..8] = L, R states */
     IBTFO_INIT_UNIFORM = 4,   /* prm {rho,u,v,p} */
-    IBTFO_INIT_SHU_VORTEX = 5 /* isentropic vortex, prm {x0, y0, beta, rho_inf, u_inf, v_inf, p_inf} */
+    IBTFO_INIT_SHU_VORTEX = 5, /* isentropic vortex, prm {x0, y0, beta, rho_inf, u_inf, v_inf, p_inf} */
+    IBTFO_INIT_SHOCK = 6       /* Test/PNGGeo/Initialize.cpp:14-59: prm {x_s, Ms, rho, u, v, p}; that state
+                                  where x > x_s, its Mach-Ms normal-shock image behind (the hook's
+                                  x_s = 0.1, Ms = 1.3) */
 };
 
 typedef struct ibtfo_config {

@@ -41,7 +41,7 @@ enum { RS_HLL = 0, RS_HLLC = 1, RS_PVRS = 2, RS_SW = 3 };
 enum { LIM_MINMOD = 0, LIM_VANLEER = 1, LIM_VANALBADA = 2 };
 enum { BC_REGULAR = 0, BC_IDW = 1, BC_LINEAR = 2, BC_RIEMANN = 3, BC_BCGFM = 4, BC_HBCGFM = 5, BC_CUT = 6,
        BC_SIMPLE = 7 };
-enum { EDGE_REFERENCE = 0, EDGE_PERIODIC = 1, EDGE_INFLOW = 2 };
+enum { EDGE_REFERENCE = 0, EDGE_PERIODIC = 1, EDGE_INFLOW = 2, EDGE_PNGGEO = 3 };
 
 // Sound speed. The reference computes std::pow(x, 0.5) (HLL/Riemann_solver.cpp:144,
 // ComputeDt.cpp:60); glibc's pow is not always the correctly rounded root. Mode 1
@@ -1197,6 +1197,32 @@ void or_global_boundary(int kind, const double* prm, int nx, int ny, int ng, dou
                     O(i, j, k) = Nw(ng, j, k);
                     O(nx + ng + i, j, k) = Nw(nx + ng - 1, j, k);
                 }
+    } else if (kind == EDGE_PNGGEO) {
+        // Test/PNGGeo/Global_Boudary.cpp:14-79: rows as the Sod hook over the
+        // interior columns, right columns zero-gradient from U_NEW, left columns
+        // the fixed post-shock state prm = (rho, u, v, p); ring corners untouched
+        for (int i = ng; i < nx + ng; i++)
+            for (int j = 0; j < ng; j++)
+                for (int k = 0; k < 4; k++) {
+                    O(i, j, k) = Nw(i, 2 * ng - 1, k);
+                    O(i, ny + ng + j, k) = Nw(i, ny + ng - 1, k);
+                }
+        for (int i = ng; i < nx + ng; i++)
+            for (int j = 0; j < ng; j++) {
+                O(i, j, 2) = -Nw(i, 2 * ng - j - 1, 2);
+                O(i, ny + ng + j, 2) = -Nw(i, ny + ng - 1 - j, 2);
+            }
+        for (int j = ng; j < ny + ng; j++)
+            for (int i = 0; i < ng; i++)
+                for (int k = 0; k < 4; k++) {
+                    O(i, j, k) = Nw(ng, j, k);
+                    O(nx + ng + i, j, k) = Nw(nx + ng - 1, j, k);
+                }
+        double Uin[4];
+        prim_to_cons(gam, prm[0], prm[1], prm[2], prm[3], Uin);
+        for (int j = ng; j < ny + ng; j++)
+            for (int i = 0; i < ng; i++)
+                for (int k = 0; k < 4; k++) O(i, j, k) = Uin[k];
     } else if (kind == EDGE_PERIODIC) {
         for (int j = 0; j < g.N; j++)
             for (int i = 0; i < g.M; i++) {
@@ -1224,7 +1250,10 @@ void or_global_boundary(int kind, const double* prm, int nx, int ny, int ng, dou
 // 2: Test/Vortex/Initialize.cpp:15-98 (prm = {fa, fw});
 // 3: Sod split at x = prm[0] with prm[1..8] = left/right primitive states (no reference hook);
 // 4: uniform state prm = {rho,u,v,p} (no reference hook);
-// 5: isentropic vortex, prm = {x0, y0, beta, rho_inf, u_inf, v_inf, p_inf} (no reference hook).
+// 5: isentropic vortex, prm = {x0, y0, beta, rho_inf, u_inf, v_inf, p_inf} (no reference hook);
+// 6: Test/PNGGeo/Initialize.cpp:14-59, a shock of Mach prm[1] at x = prm[0]: the
+//    state prm[2..5] = (rho, u, v, p) where x > prm[0], its normal-shock image
+//    behind (x <= prm[0]).
 // prm for kind 0: {rho_L,u_L,v_L,p_L, rho_R,u_R,v_R,p_R}.
 void or_initialize(int kind, const double* prm, int nx, int ny, int ng, double gam, double* U_OLD,
                    double* U_NEW, const double* XY) {
@@ -1275,6 +1304,18 @@ void or_initialize(int kind, const double* prm, int nx, int ny, int ng, double g
                 prim_to_cons(gam, s[0], s[1], s[2], s[3], U);
             } else if (kind == 4) {
                 prim_to_cons(gam, prm[0], prm[1], prm[2], prm[3], U);
+            } else if (kind == 6) {
+                const double rho_1 = prm[2], u_1 = prm[3], v_1 = prm[4], p_1 = prm[5], Ms = prm[1];
+                if (p[0] > prm[0]) {
+                    U[0] = rho_1; U[1] = rho_1 * u_1; U[2] = rho_1 * v_1;
+                    U[3] = p_1 / (gam - 1) + 0.5 * rho_1 * (u_1 * u_1 + v_1 * v_1);
+                } else {
+                    double rho_2 = rho_1 * (gam + 1) * Ms * Ms / (2 + (gam - 1) * Ms * Ms);
+                    double p_2 = p_1 * ((2 * gam) * Ms * Ms / (gam + 1) - (gam - 1) / (gam + 1));
+                    double u_2 = Ms * (1 - rho_1 / rho_2);
+                    U[0] = rho_2; U[1] = rho_2 * u_2; U[2] = 0;
+                    U[3] = p_2 / (gam - 1) + 0.5 * rho_2 * (u_2 * u_2);
+                }
             } else {
                 double x0 = prm[0], y0 = prm[1], beta = prm[2];
                 double dxv = p[0] - x0, dyv = p[1] - y0, r2 = dxv * dxv + dyv * dyv;

@@ -24,11 +24,11 @@ RIEMANNS = {"HLL": 0, "HLLC": 1, "PVRS": 2, "StegerWarming": 3}
 LIMITERS = {"minmod": 0, "VanLeer": 1, "VanAlbada": 2}
 BOUNDS = {"Regular": 0, "IDWBaseGFM": 1, "LinearBaseGFM": 2, "RiemannBaseGFM": 3, "BCGFM": 4, "HBCGFM": 5,
           "CutCell": 6, "SimpleBound": 7}
-EDGES = {"reference": 0, "periodic": 1, "inflow": 2}
+EDGES = {"reference": 0, "periodic": 1, "inflow": 2, "pnggeo": 3}
 TIMES = {"SingleStep": 0, "RK2": 1}
 # level-set hooks and initialisers (see or_level_set_hook / or_initialize)
-LS = {"sod": 0, "sodx": 0, "incline": 1, "vortex": 2, "phi": 3, "none": 0}
-INIT = {"sod": 0, "incline": 1, "vortex": 2, "sodx": 3, "uniform": 4, "shu": 5}
+LS = {"sod": 0, "sodx": 0, "incline": 1, "vortex": 2, "phi": 3, "none": 0, "pngshock": 1}
+INIT = {"sod": 0, "incline": 1, "vortex": 2, "sodx": 3, "uniform": 4, "shu": 5, "pngshock": 6}
 
 _D = C.POINTER(C.c_double)
 _lib = None
@@ -109,7 +109,7 @@ class Case:
 
     def ls_prm(self):
         e = self.extra
-        if self.case == "incline":
+        if self.case in ("incline", "pngshock"):
             return [e.get("angle", 30.0), e.get("AxisD", 0.4)]
         if self.case == "vortex":
             return [e.get("f0", 0.0), e.get("f1", 1.0)]
@@ -132,15 +132,28 @@ class Case:
         if self.case == "shu":
             return [e.get("x0", 5.0), e.get("y0", 5.0), e.get("beta", 5.0), 1.0, e.get("u", 1.0),
                     e.get("v", 1.0), 1.0]
+        if self.case == "pngshock":  # Test/PNGGeo/Initialize.cpp: shock at x = 0.1, Mach 1.3
+            return [0.1, 1.3] + left
         raise ValueError(self.case)
 
     def edge_prm(self):
         e = self.extra
+        if self.edge == "pnggeo" and "rho" not in e:
+            return pnggeo_inflow(self.gamma)
         return [e.get("rho", 1.0), e.get("u", 0.0), e.get("v", 0.0), e.get("p", 1.0)]
 
     def init_boundary(self):
         # Sod.cpp applies no Boundary before the loop; InclineSod.cpp:81 and Vortex.cpp do
         return 0 if self.Boundary == "Regular" else 1
+
+
+def pnggeo_inflow(gamma, rho1=1.0, p1=0.71429, Ms=1.3):
+    """The post-shock state (rho, u, v, p) that Test/PNGGeo/Global_Boudary.cpp:59-70
+    holds on the left edge (its constants as defaults), in its operation order."""
+    rho2 = rho1 * (gamma + 1) * Ms * Ms / (2 + (gamma - 1) * Ms * Ms)
+    p2 = p1 * ((2 * gamma) * Ms * Ms / (gamma + 1) - (gamma - 1) / (gamma + 1))
+    u2 = Ms * (1 - rho1 / rho2)
+    return [rho2, u2, 0.0, p2]
 
 
 def _num(v):

@@ -48,13 +48,17 @@ def run_cases():
                                                 Psy_time=0.3), 10))
     cases.append(("vortex_SimpleBound", O.Case(N_x=28, N_y=28, hi_y=1.0, Boundary="SimpleBound", case="vortex",
                                                extra=dict(fa=1.0, f0=0.0, f1=1.0)), 10))
+    # Test/PNGGeo's shock initialiser and edge hook (adapted signatures) over InclineSod's channel
+    cases.append(("pngshock_SW_IDW", O.Case(N_x=40, N_y=40, hi_y=1.0, Boundary="IDWBaseGFM", Riemann="StegerWarming",
+                                            case="pngshock", edge="pnggeo", Psy_time=0.3, CFL_number=0.1,
+                                            extra=dict(rho_L=1.0, u_L=0.0, v_L=0.0, p_L=0.71429)), 12))
     return cases
 
 
 def case_meta(c):
     return dict(N_x=c.N_x, N_y=c.N_y, hi_x=c.hi_x, hi_y=c.hi_y, Psy_time=c.Psy_time, CFL_number=c.CFL_number,
                 Scheme=c.Scheme, Riemann=c.Riemann, Limiter=c.Limiter, Boundary=c.Boundary, case=c.case,
-                extra=c.extra)
+                edge=c.edge, extra=c.extra)
 
 
 def main():

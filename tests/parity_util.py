@@ -29,14 +29,14 @@ def gpu_hooks(case):
     e = case.extra
     if case.phi is not None:
         ls = ("field", ())
-    elif case.case == "incline":
+    elif case.case in ("incline", "pngshock"):
         ls = ("channel", case.ls_prm())
     elif case.case == "vortex":
         ls = ("annulus", case.ls_prm())
     else:
         ls = ("none", ())
     init = {"sod": "sod_rows", "sodx": "sod_x", "incline": "incline", "vortex": "vortex", "uniform": "uniform",
-            "shu": "shu"}[case.case]
+            "shu": "shu", "pngshock": "shock"}[case.case]
     return ls, (init, case.init_prm())
 
 

@@ -24,7 +24,7 @@ def runs():
 def _case(O, m):
     return O.Case(N_x=m["N_x"], N_y=m["N_y"], hi_x=m["hi_x"], hi_y=m["hi_y"], Psy_time=m["Psy_time"],
                   CFL_number=m["CFL_number"], Scheme=m["Scheme"], Riemann=m["Riemann"], Limiter=m["Limiter"],
-                  Boundary=m["Boundary"], case=m["case"], extra=m["extra"])
+                  Boundary=m["Boundary"], case=m["case"], edge=m.get("edge", "reference"), extra=m["extra"])
 
 
 def test_fixture_set_is_complete(runs):

@@ -242,3 +242,31 @@ def test_rk2_free_stream_preserved_bitwise(oracle_mod):
     U0 = st["U_OLD"].copy()
     r = O.run_oracle(c, 10, state=st)
     assert r["steps"] == 10 and np.array_equal(r["U_OLD"], U0)
+
+
+def test_pnggeo_shock_initialiser_is_a_normal_shock(oracle_mod):
+    """Test/PNGGeo/Initialize.cpp's post-shock state against the textbook
+    normal-shock relations (Toro 3.1.3) written independently: density ratio
+    (g+1) M^2 / ((g-1) M^2 + 2), pressure ratio 1 + 2g (M^2 - 1) / (g+1), and mass
+    flux continuity across a shock moving at speed Ms into still gas; the hook
+    takes the shock speed as Ms itself, exact when the upstream sound speed is 1
+    (p = 0.71429 = 1/g, the PNGGeo input). Ahead of x = 0.1 the state is the
+    upstream one."""
+    O = oracle_mod
+    g, M = 1.4, 1.3
+    c = O.Case(N_x=40, N_y=8, hi_y=0.2, case="pngshock", edge="pnggeo", Psy_time=0.0,
+               extra=dict(rho_L=1.0, u_L=0.0, v_L=0.0, p_L=0.71429))
+    st = O.setup(c)
+    U = st["U_OLD"].reshape(c.N, c.M, 4)
+    x = st["XY"][0::6].reshape(c.N, c.M)
+    behind, ahead = U[x <= 0.1], U[x > 0.1]
+    assert np.all(ahead[:, 0] == 1.0) and np.all(ahead[:, 1] == 0.0) and np.all(ahead[:, 3] == 0.71429 / (g - 1))
+    r2, m2, E2 = behind[0, 0], behind[0, 1], behind[0, 3]
+    u2 = m2 / r2
+    p2 = (g - 1) * (E2 - 0.5 * r2 * u2 * u2)
+    assert abs(r2 - (g + 1) * M * M / ((g - 1) * M * M + 2)) <= 1e-15 * r2
+    assert abs(p2 / 0.71429 - (1 + 2 * g * (M * M - 1) / (g + 1))) <= 1e-14
+    assert abs(1.0 * (0.0 - M) - r2 * (u2 - M)) <= 1e-15
+    assert np.all(behind == behind[0]) and np.all(behind[:, 2] == 0.0)
+    # the edge hook's inflow state is the same shock's (its constants are these)
+    assert np.allclose(O.pnggeo_inflow(g), [r2, u2, 0.0, p2], rtol=1e-15, atol=0)

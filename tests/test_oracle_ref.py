@@ -44,6 +44,16 @@ def test_simplebound(oracle_mod, case, n):
     _cmp(O, O.Case(N_x=n, N_y=n, hi_y=1.0, Boundary="SimpleBound", case=case, **kw), 40)
 
 
+@pytest.mark.parametrize("rs", ["StegerWarming", "HLL"])
+def test_pnggeo_hooks(oracle_mod, rs):
+    """Test/PNGGeo's Mach 1.3 shock initialiser and its edge hook (single-device
+    signatures, adapted), over InclineSod's channel, IDW ghost fluid."""
+    O = oracle_mod
+    c = O.Case(N_x=64, N_y=64, hi_y=1.0, Boundary="IDWBaseGFM", Riemann=rs, case="pngshock", edge="pnggeo",
+               Psy_time=0.3, CFL_number=0.1, extra=dict(rho_L=1.0, u_L=0.0, v_L=0.0, p_L=0.71429))
+    _cmp(O, c, 40)
+
+
 def test_sod_400x100_full_run(oracle_mod):
     """The 400x100 Sod tube (MHM + minmod + HLLC) with the reference's row-split Sod hook replaced by the
     x-split hook (the same hook compiled into both sides)."""
