@@ -195,8 +195,7 @@ def config_legs(args):
         s.geometry(*ls)
         s.initialize(*init)
         s.prepare(apply_boundary)
-        s.step(10 ** 7, graph_chunk=chunk)  # untimed: how many steps the run takes
-        n = s.status()[0]
+        n = s.run(graph_chunk=chunk)  # untimed: how many steps the run takes
         s.close()
         s = ib.Solver(cfg)
         s.geometry(*ls)

@@ -183,6 +183,20 @@ class Solver:
         else:
             _check(lib().ibtfo_step(self.h, int(n)), "ibtfo_step")
 
+    def run(self, max_steps=None, graph_chunk=100, block=2000):
+        """Steps in blocks until the device stops the run (Psy_time or max_iter)
+        or `max_steps` more steps were taken; returns the steps taken. Blocks keep
+        the host from queueing steps the device would skip."""
+        start = done = self.status()[0]
+        while max_steps is None or done - start < max_steps:
+            n = block if max_steps is None else min(block, max_steps - (done - start))
+            self.step(n, graph_chunk=graph_chunk if n >= graph_chunk else 0)
+            it = self.status()[0]
+            if it < done + n:
+                return it - start
+            done = it
+        return done - start
+
     def status(self):
         it, t, dt = C.c_int(), C.c_double(), C.c_double()
         rc = lib().ibtfo_status(self.h, C.byref(it), C.byref(t), C.byref(dt))

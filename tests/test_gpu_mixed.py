@@ -85,7 +85,7 @@ def _vortex_run(prec, n, t_end):
     s.geometry("none", ())
     s.initialize("shu", (5.0, 5.0, 5.0, 1.0, 1.0, 1.0, 1.0))
     s.prepare(False)
-    s.step(10 ** 6, graph_chunk=100)
+    s.run(graph_chunk=100)
     it, t, dt, rc = s.status()
     U = s.state().reshape(n + 4, n + 4, 4)[2:-2, 2:-2, 0]
     s.close()
