@@ -847,6 +847,18 @@ void ibtfo::build_args(ibtfo_ctx* x) {
 // ghost fill phase 1, the edge ring, ghost fill phase 2.
 static int enqueue_boundaries(ibtfo_ctx* x) {
     const ibtfo_config& c = x->cfg;
+    if (c.boundary != IBTFO_REGULAR && c.edge != IBTFO_EDGE_REFERENCE) {
+        // fill (first level) and ring in one launch; further SimpleBound levels after
+        FillArgs f = x->fa;
+        f.level = 0;
+        HIPCHK(launch_fill_edge(f, x->ea, x->stream));
+        for (int l = 1; l < x->nlevels; l++) {
+            f.level = l;
+            HIPCHK(launch_fill(f, x->stream));
+        }
+        HIPCHK(launch_scatter(x->fa, x->stream));
+        return IBTFO_OK;
+    }
     if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_fill_levels(x, x->fa));
     if (c.edge == IBTFO_EDGE_REFERENCE && c.scheme == IBTFO_LAX)
         HIPCHK(launch_edge_ref(x->ea, x->corner, x->sa.corner_from_input, x->stream));

@@ -36,7 +36,7 @@ struct Ctl {
     int cur;        // which state buffer holds the current state
     int ran;        // 1 if the last launched step executed
     int status;     // 0 ok; 1 non-finite value met in the dt reduction
-    int pad;
+    unsigned int folded;  // fold blocks done in the running finish (0 between launches)
     double mass_now;                      // CaculateMass of the state seen by the last reduction
     double log_dt[LOGCAP], log_t[LOGCAP], log_mass[LOGCAP];
 };
@@ -195,6 +195,7 @@ hipError_t launch_step(int scheme, int rs, int lim, int prec, const StepArgs& a,
 hipError_t launch_fill(const FillArgs& a, hipStream_t s);
 hipError_t launch_scatter(const FillArgs& a, hipStream_t s);
 hipError_t launch_edge(const EdgeArgs& a, hipStream_t s);
+hipError_t launch_fill_edge(const FillArgs& f, const EdgeArgs& e, hipStream_t s);  // fill phase 1 + ring
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
 hipError_t launch_reduce_partials(const ReduceArgs& a, hipStream_t s);  // without the finish
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s);

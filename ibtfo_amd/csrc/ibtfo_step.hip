@@ -101,7 +101,7 @@ __device__ __forceinline__ void block_partials(double* part, int bid, double ama
 // the block partials, then the host loop's bookkeeping of Test/Sod/Sod.cpp:154-177:
 // dt for the next step, now_t += dt, iter++, stop test, log entry. `advance` = 0
 // for the stand-alone ComputeDt (no step taken; `reset` starts the run clock).
-__device__ __forceinline__ void finish_body(const FinishArgs& f, const int advance) {
+template <int NT> __device__ __forceinline__ void finish_body(const FinishArgs& f, const int advance) {
     Ctl* ctl = f.ctl;
     if (advance && ctl->done) return;  // the step kernel did not run
     if (advance == 2) {
@@ -111,25 +111,25 @@ __device__ __forceinline__ void finish_body(const FinishArgs& f, const int advan
         }
         return;
     }
-    __shared__ double sA[FINISH_THREADS], sU[FINISH_THREADS], sV[FINISH_THREADS], sM[FINISH_THREADS];
-    __shared__ int sB[FINISH_THREADS];
+    __shared__ double sA[NT], sU[NT], sV[NT], sM[NT];
+    __shared__ int sB[NT];
     const int t = threadIdx.x;
     double A = 0.0, Um = 0.0, Vm = 0.0, m = 0.0;
     int bad = 0;
     // batches of FB records per thread, all loads issued before any is used, so
     // one memory round trip covers FB records (the loop is latency bound)
     constexpr int FB = 8;
-    for (int q0 = t; q0 < f.nblocks; q0 += FB * FINISH_THREADS) {
+    for (int q0 = t; q0 < f.nblocks; q0 += FB * NT) {
         double ra[FB], ru[FB], rv[FB], rm[FB], rb[FB];
 #pragma unroll
         for (int b = 0; b < FB; b++) {
-            const int q = q0 + b * FINISH_THREADS;
+            const int q = q0 + b * NT;
             const double* r = f.part + PART_STRIDE * (long long)(q < f.nblocks ? q : q0);
             ra[b] = r[0]; ru[b] = r[1]; rv[b] = r[2]; rm[b] = r[3]; rb[b] = r[4];
         }
 #pragma unroll
         for (int b = 0; b < FB; b++) {
-            if (q0 + b * FINISH_THREADS >= f.nblocks) break;
+            if (q0 + b * NT >= f.nblocks) break;
             A = (A < ra[b]) ? ra[b] : A;
             Um = (Um < ru[b]) ? ru[b] : Um;
             Vm = (Vm < rv[b]) ? rv[b] : Vm;
@@ -139,7 +139,7 @@ __device__ __forceinline__ void finish_body(const FinishArgs& f, const int advan
     }
     sA[t] = A; sU[t] = Um; sV[t] = Vm; sM[t] = m; sB[t] = bad;
     __syncthreads();
-    for (int w = FINISH_THREADS / 2; w > 0; w >>= 1) {
+    for (int w = NT / 2; w > 0; w >>= 1) {
         if (t < w) {
             sA[t] = (sA[t] < sA[t + w]) ? sA[t + w] : sA[t];
             sU[t] = (sU[t] < sU[t + w]) ? sU[t + w] : sU[t];
@@ -181,7 +181,9 @@ __device__ __forceinline__ void finish_body(const FinishArgs& f, const int advan
         }
     }
 }
-__global__ __launch_bounds__(FINISH_THREADS) void k_finish(FinishArgs f) { finish_body(f, f.advance); }
+__global__ __launch_bounds__(FINISH_THREADS) void k_finish(FinishArgs f) {
+    finish_body<FINISH_THREADS>(f, f.advance);
+}
 
 // Partials of a 256-thread block: wave reductions, then the four waves in order.
 __device__ __forceinline__ void block4_partials(double* part, int bid, double amax, double umax, double vmax,
@@ -213,10 +215,13 @@ __device__ __forceinline__ void block4_partials(double* part, int bid, double am
     }
 }
 
-// First level of the two-level finish: block g folds partials
-// [g * fold_per, (g + 1) * fold_per) into fold record g.
+// Two-level finish in one launch: block g folds partials [g * fold_per,
+// (g + 1) * fold_per) into fold record g; the last block to finish (counted in
+// Ctl::folded, with device-scope fences around the count) reduces the fold
+// records and runs the clock. Only the records cross blocks, so the fences
+// write back and invalidate little.
 __global__ __launch_bounds__(FOLD_THREADS) void k_fold(FinishArgs f) {
-    if (f.ctl->done) return;  // the step kernel did not run; k_finish returns too
+    if (f.ctl->done) return;  // the step kernel did not run (every block sees the same flag)
     const int q0 = blockIdx.x * f.fold_per, q1 = min(q0 + f.fold_per, f.nblocks);
     double A = 0.0, U = 0.0, V = 0.0, m = 0.0;
     int bad = 0;
@@ -229,13 +234,26 @@ __global__ __launch_bounds__(FOLD_THREADS) void k_fold(FinishArgs f) {
         bad |= r[4] != 0.0;
     }
     block4_partials(f.fold, blockIdx.x, A, U, V, m, bad);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        __threadfence();  // this block's record before its count
+        last = atomicAdd(&f.ctl->folded, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // every record after the count
+    if (threadIdx.x == 0) f.ctl->folded = 0;  // no other block touches it any more
+    FinishArgs g = f;
+    g.part = f.fold;
+    g.nblocks = gridDim.x;
+    finish_body<FOLD_THREADS>(g, 1);
 }
 
 // Batched finish: block s finishes solve s (slots with a null control block are idle).
 __global__ __launch_bounds__(FINISH_THREADS) void k_finish_batch(const FinishArgs* __restrict__ f, int nslots,
                                                                  int advance) {
     if ((int)blockIdx.x >= nslots || f[blockIdx.x].ctl == nullptr) return;
-    finish_body(f[blockIdx.x], advance);
+    finish_body<FINISH_THREADS>(f[blockIdx.x], advance);
 }
 
 // Primitive view of an FP64 conserved state in the compute type T. FP64:
@@ -976,6 +994,13 @@ __device__ __forceinline__ void edge_one(const EdgeArgs& a, const int q) {
     for (int k = 0; k < 4; k++) U[o + k * PL] = v[k];
 }
 __global__ void k_edge(EdgeArgs a) { edge_one(a, blockIdx.x * blockDim.x + threadIdx.x); }
+// Ghost fill phase 1 and the edge ring in one launch (independent work: the fill
+// reads ring donors from the previous state): blocks below nfb fill, the rest
+// write the ring.
+__global__ __launch_bounds__(FILL_THREADS) void k_fill_edge(FillArgs f, EdgeArgs e, int nfb) {
+    if ((int)blockIdx.x < nfb) fill_one(f, blockIdx.x * FILL_THREADS + threadIdx.x);
+    else edge_one(e, (blockIdx.x - nfb) * FILL_THREADS + threadIdx.x);
+}
 // Batched edge ring: block b covers ring cells map[b].y .. +EDGE_THREADS-1 of solve map[b].x.
 __global__ void k_edge_batch(const EdgeArgs* __restrict__ slots, const int2* __restrict__ map) {
     const int2 m = map[blockIdx.x];
@@ -1121,6 +1146,14 @@ hipError_t launch_scatter(const FillArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_scatter, dim3((a.nghost + FILL_THREADS - 1) / FILL_THREADS), dim3(FILL_THREADS), 0, s, a);
     return hipGetLastError();
 }
+hipError_t launch_fill_edge(const FillArgs& f, const EdgeArgs& e, hipStream_t s) {
+    if (f.nghost == 0) return launch_edge(e, s);
+    const int nfb = (f.nghost + FILL_THREADS - 1) / FILL_THREADS;
+    const int nring = e.L.M * e.L.N - e.L.nx * e.L.ny;
+    const int neb = (nring + FILL_THREADS - 1) / FILL_THREADS;
+    hipLaunchKernelGGL(k_fill_edge, dim3(nfb + neb), dim3(FILL_THREADS), 0, s, f, e, nfb);
+    return hipGetLastError();
+}
 hipError_t launch_edge(const EdgeArgs& a, hipStream_t s) {
     int nring = a.L.M * a.L.N - a.L.nx * a.L.ny;
     hipLaunchKernelGGL(k_edge, dim3((nring + EDGE_THREADS - 1) / EDGE_THREADS), dim3(EDGE_THREADS), 0, s, a);
@@ -1157,11 +1190,6 @@ hipError_t launch_finish(const FinishArgs& f, hipStream_t s) {
         g.nfold = std::min(FOLD_MAX, (f.nblocks + FOLD_THREADS - 1) / FOLD_THREADS);
         g.fold_per = (f.nblocks + g.nfold - 1) / g.nfold;
         hipLaunchKernelGGL(k_fold, dim3(g.nfold), dim3(FOLD_THREADS), 0, s, g);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        g.part = f.fold;
-        g.nblocks = g.nfold;
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, g);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, f);
