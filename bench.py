@@ -149,9 +149,17 @@ def cpu_leg(args):
     t0 = time.perf_counter()
     r = O.run_oracle(case, steps, state=st)
     el = time.perf_counter() - t0
+    # one thread, the first steps of the same sample
+    O.lib().or_set_num_threads(1)
+    st1 = O.setup(case)
+    t1 = time.perf_counter()
+    r1 = O.run_oracle(case, args.cpu_steps_1t, state=st1)
+    el1 = time.perf_counter() - t1
+    O.lib().or_set_num_threads(threads)
     return dict(value=nx * ny * r["steps"] / el, cores=threads, steps=r["steps"], seconds=el,
                 sample="%dx%d cells, %d steps of the same workload (oracle, OpenMP %d threads)" % (
-                    nx, ny, r["steps"], threads))
+                    nx, ny, r["steps"], threads),
+                value_1t=nx * ny * r1["steps"] / el1, steps_1t=r1["steps"])
 
 
 def copy_bandwidth():
@@ -516,6 +524,7 @@ def main():
     ap.add_argument("--graph-chunk", type=int, default=100, help="steps per captured HIP graph (0: plain launches)")
     ap.add_argument("--cpu-steps", type=int, default=600)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps-1t", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-stream leg")
@@ -578,7 +587,9 @@ def main():
     if not args.no_cpu:
         c = cpu_leg(args)
         progress("CPU baseline")
-        cpu = dict(value=c["value"], unit="cell-updates/s", cores=c["cores"], kind="port", sample=c["sample"])
+        cpu = dict(value=c["value"], unit="cell-updates/s", cores=c["cores"], kind="port", sample=c["sample"],
+                   single_thread={"value": c["value_1t"], "cores": 1, "sample": "the first %d steps of the same "
+                                  "sample on one thread" % c["steps_1t"]})
     line = {
         "metric": "cell-updates/s (frac. HBM roofline); batched p50/p99 solve latency at fixed L1",
         "value": g["value"],
