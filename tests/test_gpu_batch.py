@@ -144,3 +144,7 @@ def test_refused_requests():
     with pytest.raises(ib.IbtfoError):
         b.submit(cfg=wrong, levelset=items[0]["levelset"], init=items[0]["init"])
     b.close()
+    # batch selections the engine does not run: refused at creation, with a message
+    for bad in ["CutCell", "SimpleBound"]:
+        with pytest.raises(ib.IbtfoError, match="batch engine"):
+            S.Batch(_bc("FP64", Boundary=bad))

@@ -43,6 +43,20 @@ def test_sodx_bitexact(oracle_mod, scheme, riemann, limiter):
         assert max(rel_linf(g["U_OLD"], p["U_OLD"])) <= REL_TOL
 
 
+@pytest.mark.parametrize("scheme,quirks", [("MUSCL", 1), ("MHM", 0), ("Godunov", 1)])
+def test_vanalbada_bitexact(oracle_mod, scheme, quirks):
+    """The VanAlbada limiter as its evident intent (the reference's macro does not
+    compile): GPU == oracle bit for bit on an x-Sod, in both modes."""
+    O = oracle_mod
+    c = _case(O, N_x=96, N_y=24, hi_y=0.25, Scheme=scheme, Riemann="HLL", Limiter="VanAlbada", case="sodx",
+              quirks=quirks)
+    g = gpu_run(c, 25)
+    o = O.run_oracle(c, 25, root_mode=1)
+    assert g["steps"] == o["steps"] == 25
+    assert nan_equal_bits(g["U_OLD"], o["U_OLD"]) == 0
+    assert nan_equal_bits(g["dt_log"], o["dt"]) == 0
+
+
 def test_sod_400x100_full(oracle_mod):
     """400x100 Sod, MHM + minmod + HLLC(=HLL), CFL 0.5, t_end 0.2."""
     O = oracle_mod
