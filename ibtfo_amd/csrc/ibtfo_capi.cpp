@@ -551,19 +551,27 @@ extern "C" int ibtfo_geometry_mask(ibtfo_ctx* x, const unsigned char* rgba, int 
     m.dxp = (Psy_L / c.N_x) * (c.N_x + 2 * c.num_ghost_cell) / width;
     m.dyp = (Psy_H / c.N_y) * (c.N_y + 2 * c.num_ghost_cell) / height;
     const size_t WH = (size_t)width * height, WN = (size_t)width * L.N, NW1 = (size_t)L.N * (width + 1);
+    m.nseg = (height + MASK_SEG - 1) / MASK_SEG;
+    const size_t SW = (size_t)m.nseg * width;
+    // the row envelope runs in LDS up to its limit; wider masks take global scratch
+    const bool lds_rows = width < 65536 && mask_lds_bytes(width) <= MASK_LDS_MAX &&
+                          hipFuncSetAttribute((const void*)mask_rows_kernel(), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)mask_lds_bytes(width)) == hipSuccess;
     unsigned char *drgba = nullptr, *dedge = nullptr;
     double *dv = nullptr, *dg1 = nullptr, *dz = nullptr, *dphi = nullptr;
-    int *dr1 = nullptr, *dr2 = nullptr, *dvb = nullptr;
+    int *dr1 = nullptr, *dr2 = nullptr, *dvb = nullptr, *dsf = nullptr, *dsl = nullptr, *dbe = nullptr, *daf = nullptr;
     auto release = [&]() {
         for (void* p : {(void*)drgba, (void*)dedge, (void*)dv, (void*)dg1, (void*)dz, (void*)dr1, (void*)dr2,
-                        (void*)dvb})
+                        (void*)dvb, (void*)dsf, (void*)dsl, (void*)dbe, (void*)daf})
             dfree(x, p);
     };
     if (dalloc(x, &drgba, WH * 4) != hipSuccess || dalloc(x, &dedge, WH) != hipSuccess ||
         dalloc(x, &dv, WH * sizeof(double)) != hipSuccess || dalloc(x, &dg1, WN * sizeof(double)) != hipSuccess ||
         dalloc(x, &dr1, WN * sizeof(int)) != hipSuccess || dalloc(x, &dr2, WN * sizeof(int)) != hipSuccess ||
-        dalloc(x, &dvb, NW1 * sizeof(int)) != hipSuccess ||
-        dalloc(x, &dz, NW1 * sizeof(double)) != hipSuccess ||
+        dalloc(x, &dsf, SW * sizeof(int)) != hipSuccess || dalloc(x, &dsl, SW * sizeof(int)) != hipSuccess ||
+        dalloc(x, &dbe, SW * sizeof(int)) != hipSuccess || dalloc(x, &daf, SW * sizeof(int)) != hipSuccess ||
+        (!lds_rows && (dalloc(x, &dvb, NW1 * sizeof(int)) != hipSuccess ||
+                       dalloc(x, &dz, NW1 * sizeof(double)) != hipSuccess)) ||
         dalloc(x, &dphi, (size_t)L.M * L.N * sizeof(double)) != hipSuccess) {
         release();
         dfree(x, dphi);
@@ -571,6 +579,7 @@ extern "C" int ibtfo_geometry_mask(ibtfo_ctx* x, const unsigned char* rgba, int 
         return IBTFO_E_NOMEM;
     }
     m.rgba = drgba; m.v = dv; m.edge = dedge; m.g1 = dg1; m.r1 = dr1; m.r2 = dr2; m.vbuf = dvb; m.zbuf = dz;
+    m.sfirst = dsf; m.slast = dsl; m.before = dbe; m.after = daf;
     m.phi = dphi;
     hipError_t e = scopy(x, drgba, rgba, WH * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = mask_run(m, x->stream);

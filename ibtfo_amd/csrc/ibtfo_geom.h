@@ -44,16 +44,24 @@ struct MaskArgs {
     int M, N;                   // cells with ghosts
     double Li, Hi, dxc;         // cell centres (Psy_coord)
     double exo, eyo, dxp, dyp;  // edge pixel x: exo + dxp * x, y: eyo + dyp * y
-    double* g1;                 // W x N: squared y distance to the column's nearest edge
-    int* r1;                    // W x N: that edge's pixel row (-1: none)
-    int* r2;                    // W x N: the column's other candidate row (-1: none)
-    int* vbuf;                  // N x (W+1) envelope scratch
+    double* g1;                 // N x W (cell row major): squared y distance to the column's nearest edge
+    int* r1;                    // N x W: that edge's pixel row (-1: none)
+    int* r2;                    // N x W: the column's other candidate row (-1: none)
+    int nseg;                   // 32-row segments per pixel column
+    int *sfirst, *slast;        // nseg x W: first / last edge row of each segment (-1: none)
+    int *before, *after;        // nseg x W: last edge row below / first above each segment (-1: none)
+    int* vbuf;                  // N x (W+1) envelope scratch (masks too wide for the LDS envelope)
     double* zbuf;               // N x (W+1)
     double* phi;                // M x N, reference cell order
 };
+constexpr int MASK_SEG = 32;
+// LDS envelope of one cell row: z (W+1 doubles), g (W doubles), v (W+1 uint16)
+inline size_t mask_lds_bytes(int W) { return (size_t)(W + 1) * 8 + (size_t)W * 8 + (size_t)(W + 1) * 2 + 16; }
+constexpr size_t MASK_LDS_MAX = 160 * 1024;
 
 hipError_t geom_run(GeomArgs& g, hipStream_t s);
-hipError_t mask_run(MaskArgs& m, hipStream_t s);
+hipError_t mask_run(MaskArgs& m, hipStream_t s);  // row envelope in LDS when m.vbuf is null
+const void* mask_rows_kernel();                     // that kernel (its LDS attribute is set by the caller)
 // cut-cell geometry on top of geom_run's phi and normals: records, types, merge
 // targets, ghosts (dense scratch g.dcut), then the compaction passes (counting
 // when ccap / gcap are 0)

@@ -474,58 +474,176 @@ __global__ void k_mask_edge(MaskArgs m) {
 __device__ __forceinline__ double edge_x(const MaskArgs& m, int x) { return m.exo + m.dxp * x; }
 __device__ __forceinline__ double edge_y(const MaskArgs& m, int y) { return m.eyo + m.dyp * y; }
 
-// Per pixel column: for every cell row j, the nearest edge row of the column to
-// the cell-centre height y_j (two-pointer sweep; both sequences increase).
-__global__ void k_edt_cols(MaskArgs m) {
+// Per pixel column: the first and last edge row of each 32-row segment.
+__global__ void k_edt_segs(MaskArgs m) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, sg = blockIdx.y;
+    if (x >= m.W) return;
+    int first = -1, last = -1;
+    const int y1 = min((sg + 1) * MASK_SEG, m.H);
+    for (int y = sg * MASK_SEG; y < y1; y++)
+        if (m.edge[(long long)y * m.W + x]) {
+            if (first < 0) first = y;
+            last = y;
+        }
+    m.sfirst[(long long)sg * m.W + x] = first;
+    m.slast[(long long)sg * m.W + x] = last;
+}
+
+// Per pixel column: the last edge row below each segment and the first above it.
+__global__ void k_edt_segscan(MaskArgs m) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= m.W) return;
-    auto next_edge = [&](int from) {
-        for (int y = from; y < m.H; y++)
-            if (m.edge[(long long)y * m.W + x]) return y;
-        return -1;
-    };
-    int prv = -1, nxt = next_edge(0);
-    for (int j = 0; j < m.N; j++) {
-        const double yq = m.Hi + j * m.dxc;
-        while (nxt >= 0 && edge_y(m, nxt) <= yq) {
-            prv = nxt;
-            nxt = next_edge(nxt + 1);
-        }
-        double best = HUGE_VAL;
-        int arg = -1;
-        if (prv >= 0) {
-            double d = edge_y(m, prv) - yq;
-            best = d * d;
-            arg = prv;
-        }
-        if (nxt >= 0) {
-            double d = edge_y(m, nxt) - yq;
-            if (d * d < best) { best = d * d; arg = nxt; }
-        }
-        m.g1[(long long)x * m.N + j] = best;
-        m.r1[(long long)x * m.N + j] = arg;
-        m.r2[(long long)x * m.N + j] = arg == prv ? nxt : prv;  // the other neighbour (near ties)
+    int b = -1;
+    for (int sg = 0; sg < m.nseg; sg++) {
+        const long long o = (long long)sg * m.W + x;
+        m.before[o] = b;
+        if (m.slast[o] >= 0) b = m.slast[o];
+    }
+    int a = -1;
+    for (int sg = m.nseg - 1; sg >= 0; sg--) {
+        const long long o = (long long)sg * m.W + x;
+        m.after[o] = a;
+        if (m.sfirst[o] >= 0) a = m.sfirst[o];
     }
 }
 
-// Per cell row j: lower envelope of f_x(q) = (q - x_e(x))^2 + g1(x, j) over the
-// pixel columns that hold an edge, evaluated at the cell centres x_i; then
-// phi(i, j) = sign * min(100, distance to the chosen edge pixel) with the
-// sign sampled at pixel (int((i+0.5) W / M), int((j+0.5) H / N)).
-__global__ void k_edt_rows(MaskArgs m) {
+// Per (pixel column x, cell row j): the column's edge rows just below and just
+// above the cell-centre height y_j (the last with edge height <= y_j and the
+// first above it: within the 32-row segment around y_j, else from the segment
+// summaries), the nearer of them and its squared height difference.
+__global__ void k_edt_cols(MaskArgs m) {
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)m.W * m.N) return;
+    const int x = (int)(t % m.W), j = (int)(t / m.W);
+    const double yq = m.Hi + j * m.dxc;
+    // y0: the last pixel row whose edge height is <= yq (-1: none); the heights
+    // increase with the row, so a rounded estimate is corrected by single steps
+    int y0 = (int)floor((yq - m.eyo) / m.dyp);
+    y0 = min(max(y0, -1), m.H - 1);
+    while (y0 >= 0 && edge_y(m, y0) > yq) y0--;
+    while (y0 + 1 < m.H && edge_y(m, y0 + 1) <= yq) y0++;
+    int prv = -1, nxt = -1;
+    if (y0 >= 0) {
+        const int s0 = y0 / MASK_SEG;
+        for (int y = y0; y >= s0 * MASK_SEG; y--)
+            if (m.edge[(long long)y * m.W + x]) { prv = y; break; }
+        if (prv < 0) prv = m.before[(long long)s0 * m.W + x];
+    }
+    if (y0 + 1 < m.H) {
+        const int y1 = y0 + 1, s1 = y1 / MASK_SEG, ye = min((s1 + 1) * MASK_SEG, m.H);
+        for (int y = y1; y < ye; y++)
+            if (m.edge[(long long)y * m.W + x]) { nxt = y; break; }
+        if (nxt < 0) nxt = m.after[(long long)s1 * m.W + x];
+    }
+    double best = HUGE_VAL;
+    int arg = -1;
+    if (prv >= 0) {
+        double d = edge_y(m, prv) - yq;
+        best = d * d;
+        arg = prv;
+    }
+    if (nxt >= 0) {
+        double d = edge_y(m, nxt) - yq;
+        if (d * d < best) { best = d * d; arg = nxt; }
+    }
+    m.g1[t] = best;
+    m.r1[t] = arg;
+    m.r2[t] = arg == prv ? nxt : prv;  // the other neighbour (near ties)
+}
+
+// phi of cell (i, j) from the lower envelope (v[0..k], boundaries z) of cell row
+// j: the envelope's parabola at the cell centre and its neighbours, each with the
+// column's two nearest edge rows; points whose distances tie with the nearest in
+// exact arithmetic but round differently are all evaluated with the reference's
+// expression, whose minimum it keeps. Then the sign sampled at pixel
+// (int((i+0.5) W / M), int((j+0.5) H / N)).
+template <typename V>
+__device__ __forceinline__ double mask_phi_cell(const MaskArgs& m, int i, int j, const V* v, const double* z,
+                                                int k) {
+    const double X = m.Li + i * m.dxc, Y = m.Hi + j * m.dxc;
+    double dmin = 100.0;  // Level_Set.cpp:131 starts every search at 100
+    if (k >= 0) {
+        // q = the last envelope piece whose left boundary lies below X
+        int lo = 0, hi = k;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (z[mid] < X) lo = mid;
+            else hi = mid - 1;
+        }
+        const int q = lo;
+        for (int c = (q > 0 ? q - 1 : q); c <= (q < k ? q + 1 : q); c++) {
+            const int ex = v[c];
+            const double ax = edge_x(m, ex);
+            for (int rr = 0; rr < 2; rr++) {
+                const int ey = rr == 0 ? m.r1[(long long)j * m.W + ex] : m.r2[(long long)j * m.W + ex];
+                if (ey < 0) continue;
+                const double ay = edge_y(m, ey);
+                const double d = ib_sqrt((ax - X) * (ax - X) + (ay - Y) * (ay - Y));
+                if (d < dmin) dmin = d;
+            }
+        }
+    }
+    const int xs = (int)((i + 0.5) * (m.W) / (m.M));
+    const int ys = (int)((j + 0.5) * (m.H) / (m.N));
+    const int pn = m.v[(long long)ys * m.W + xs] >= 0 ? 1 : -1;
+    return pn * dmin;
+}
+
+// Per cell row j (one 64-lane block): the lower envelope of f_x(q) = (q - x_e(x))^2
+// + g1(x, j) over the pixel columns that hold an edge (Felzenszwalb-Huttenlocher),
+// built by lane 0 with its stack in LDS, the row's g1 staged in LDS by all lanes;
+// then every lane evaluates cells i = lane, lane + 64, ...
+__global__ __launch_bounds__(64) void k_edt_rows(MaskArgs m) {
+    extern __shared__ double lds[];
+    const int j = blockIdx.x, lane = threadIdx.x, W = m.W;
+    double* z = lds;                                    // W + 1
+    double* gr = z + (W + 1);                           // W
+    unsigned short* v = (unsigned short*)(gr + W);      // W + 1
+    __shared__ int kk;
+    for (int x = lane; x < W; x += 64) gr[x] = m.g1[(long long)j * W + x];
+    __syncthreads();
+    if (lane == 0) {
+        int k = -1;
+        for (int x = 0; x < W; x++) {
+            const double f = gr[x];
+            if (!(f < HUGE_VAL)) continue;
+            const double p = edge_x(m, x);
+            double s = -HUGE_VAL;
+            while (k >= 0) {
+                const int u = v[k];
+                const double pu = edge_x(m, u), fu = gr[u];
+                s = ((f + p * p) - (fu + pu * pu)) / (2.0 * (p - pu));
+                if (s <= z[k]) k--;
+                else break;
+            }
+            k++;
+            v[k] = (unsigned short)x;
+            z[k] = k == 0 ? -HUGE_VAL : s;
+        }
+        kk = k;
+    }
+    __syncthreads();
+    const int k = kk;
+    for (int i = lane; i < m.M; i += 64) m.phi[(long long)j * m.M + i] = mask_phi_cell(m, i, j, v, z, k);
+}
+
+// The same for masks too wide for the LDS envelope: one thread per cell row,
+// its stack in global scratch.
+__global__ void k_edt_rows_global(MaskArgs m) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m.N) return;
     int* v = m.vbuf + (long long)j * (m.W + 1);
     double* z = m.zbuf + (long long)j * (m.W + 1);
+    const double* g = m.g1 + (long long)j * m.W;
     int k = -1;
     for (int x = 0; x < m.W; x++) {
-        const double f = m.g1[(long long)x * m.N + j];
+        const double f = g[x];
         if (!(f < HUGE_VAL)) continue;
         const double p = edge_x(m, x);
         double s = -HUGE_VAL;
         while (k >= 0) {
             const int u = v[k];
-            const double pu = edge_x(m, u), fu = m.g1[(long long)u * m.N + j];
+            const double pu = edge_x(m, u), fu = g[u];
             s = ((f + p * p) - (fu + pu * pu)) / (2.0 * (p - pu));
             if (s <= z[k]) k--;
             else break;
@@ -534,43 +652,24 @@ __global__ void k_edt_rows(MaskArgs m) {
         v[k] = x;
         z[k] = k == 0 ? -HUGE_VAL : s;
     }
-    const double Y = m.Hi + j * m.dxc;
-    int q = 0;
-    for (int i = 0; i < m.M; i++) {
-        const double X = m.Li + i * m.dxc;
-        double dmin = 100.0;  // Level_Set.cpp:131 starts every search at 100
-        if (k >= 0) {
-            while (q < k && z[q + 1] < X) q++;
-            // the envelope's parabola at X and its neighbours, each with the column's
-            // two nearest edge rows: points whose distances tie with the nearest in
-            // exact arithmetic but round differently are all evaluated with the
-            // reference's expression, whose minimum it keeps
-            for (int c = (q > 0 ? q - 1 : q); c <= (q < k ? q + 1 : q); c++) {
-                const int ex = v[c];
-                const double ax = edge_x(m, ex);
-                for (int rr = 0; rr < 2; rr++) {
-                    const int ey = rr == 0 ? m.r1[(long long)ex * m.N + j] : m.r2[(long long)ex * m.N + j];
-                    if (ey < 0) continue;
-                    const double ay = edge_y(m, ey);
-                    const double d = ib_sqrt((ax - X) * (ax - X) + (ay - Y) * (ay - Y));
-                    if (d < dmin) dmin = d;
-                }
-            }
-        }
-        const int xs = (int)((i + 0.5) * (m.W) / (m.M));
-        const int ys = (int)((j + 0.5) * (m.H) / (m.N));
-        const int pn = m.v[(long long)ys * m.W + xs] >= 0 ? 1 : -1;
-        m.phi[(long long)j * m.M + i] = pn * dmin;
-    }
+    for (int i = 0; i < m.M; i++) m.phi[(long long)j * m.M + i] = mask_phi_cell(m, i, j, v, z, k);
 }
 
 hipError_t mask_run(MaskArgs& m, hipStream_t s) {
     dim3 pg((m.W + 255) / 256, m.H);
     hipLaunchKernelGGL(k_mask_sign, pg, dim3(256), 0, s, m);
     hipLaunchKernelGGL(k_mask_edge, pg, dim3(256), 0, s, m);
-    hipLaunchKernelGGL(k_edt_cols, dim3((m.W + 63) / 64), dim3(64), 0, s, m);
-    hipLaunchKernelGGL(k_edt_rows, dim3((m.N + 63) / 64), dim3(64), 0, s, m);
+    hipLaunchKernelGGL(k_edt_segs, dim3((m.W + 255) / 256, m.nseg), dim3(256), 0, s, m);
+    hipLaunchKernelGGL(k_edt_segscan, dim3((m.W + 255) / 256), dim3(256), 0, s, m);
+    const long long nq = (long long)m.W * m.N;
+    hipLaunchKernelGGL(k_edt_cols, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, m);
+    if (!m.vbuf) {
+        hipLaunchKernelGGL(k_edt_rows, dim3(m.N), dim3(64), mask_lds_bytes(m.W), s, m);
+    } else {
+        hipLaunchKernelGGL(k_edt_rows_global, dim3((m.N + 63) / 64), dim3(64), 0, s, m);
+    }
     return hipGetLastError();
 }
+const void* mask_rows_kernel() { return (const void*)k_edt_rows; }
 
 }  // namespace ibtfo
