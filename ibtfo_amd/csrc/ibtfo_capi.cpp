@@ -133,7 +133,7 @@ void ibtfo::free_ctx(ibtfo_ctx* x) {
     for (void* p : {(void*)x->state[0], (void*)x->state[1], (void*)x->type, (void*)x->phi, (void*)x->gnx,
                     (void*)x->gny, (void*)x->ghosts, (void*)x->gvals, (void*)x->dcount, (void*)x->ctl,
                     (void*)x->mass_part, (void*)x->corner, (void*)x->cuts, (void*)x->cut_index, (void*)x->groups,
-                    (void*)x->rblocks, (void*)x->fold, (void*)x->unew0})
+                    (void*)x->stype, (void*)x->merge_part, (void*)x->fold, (void*)x->unew0})
         dfree(x, p);
     if (x->own_stream) (void)hipStreamDestroy(x->own_stream);
     delete x;
@@ -311,62 +311,27 @@ static int cut_geometry(ibtfo_ctx* x, GeomArgs& g) {
     g.dcut = nullptr;
     if (rc) return rc;
     HIPCHK(e);
-    // cells the merge may change or that carry a volume fraction: every cut cell
-    // and every group member, as (i, j), for the post-merge partials' block list
-    x->touched_ij.clear();
-    dfree(x, x->rblocks); x->rblocks = nullptr;
-    x->nrblocks = x->rb_cap = 0;
-    x->rb_nbx = x->rb_rows = -1;
-    if (x->ncut) {
-        std::vector<uint8_t> tp(L.plane);
-        std::vector<MergeGroup> gr(x->ngroups);
-        HIPCHK(scopy(x, tp.data(), x->type, L.plane, hipMemcpyDeviceToHost));
-        if (x->ngroups) HIPCHK(scopy(x, gr.data(), x->groups, x->ngroups * sizeof(MergeGroup), hipMemcpyDeviceToHost));
-        auto add = [&](long long o) {
-            const long long r = o - L.off;
-            x->touched_ij.push_back((int)(r % L.pitch));
-            x->touched_ij.push_back((int)(r / L.pitch));
-        };
-        for (int j = L.ng; j < L.ny + L.ng; j++)
-            for (int i = L.ng; i < L.nx + L.ng; i++)
-                if (tp[L.at(i, j)] == 3) add(L.at(i, j));
-        for (const MergeGroup& m : gr) {
-            add(m.target);
-            for (int q = 0; q < m.n; q++) add(m.member[q]);
-        }
-        x->rb_cap = (int)(x->touched_ij.size() / 2);
-        if (dalloc(x, &x->rblocks, x->rb_cap * sizeof(int)) != hipSuccess) {
-            set_error("cut-cell geometry: out of device memory");
-            return IBTFO_E_NOMEM;
-        }
+    // the merge groups' cells (targets and members) carry MERGED_BIT in the step
+    // kernel's type plane: it leaves their dt / mass partials to the merge kernel,
+    // which writes one record per block
+    dfree(x, x->stype); x->stype = nullptr;
+    dfree(x, x->merge_part); x->merge_part = nullptr;
+    std::vector<uint8_t> mk(L.plane);
+    std::vector<MergeGroup> gr(x->ngroups);
+    HIPCHK(scopy(x, mk.data(), x->type, L.plane, hipMemcpyDeviceToHost));
+    if (x->ngroups) HIPCHK(scopy(x, gr.data(), x->groups, x->ngroups * sizeof(MergeGroup), hipMemcpyDeviceToHost));
+    for (const MergeGroup& m : gr) {
+        mk[m.target] |= MERGED_BIT;
+        for (int q = 0; q < m.n; q++) mk[m.member[q]] |= MERGED_BIT;
     }
+    if (dalloc(x, &x->stype, L.plane) != hipSuccess ||
+        (x->ngroups && dalloc(x, &x->merge_part, PART_STRIDE * (size_t)merge_blocks(x->ngroups) * sizeof(double)) !=
+                           hipSuccess)) {
+        set_error("cut-cell geometry: out of device memory");
+        return IBTFO_E_NOMEM;
+    }
+    HIPCHK(scopy(x, x->stype, mk.data(), L.plane, hipMemcpyHostToDevice));
     return IBTFO_OK;
-}
-
-// The step blocks holding a touched cell (cut cell or merge-group member) for
-// the current step grid, uploaded when the grid changed. Set-up time only (it
-// synchronises the stream); a failure is reported by the next step.
-static void cut_blocks(ibtfo_ctx* x) {
-    const StepArgs& a = x->sa;
-    if (x->cfg.boundary != IBTFO_CUTCELL || (a.nbx == x->rb_nbx && a.rows == x->rb_rows)) return;
-    std::vector<char> mark((size_t)a.nbx * a.nby, 0);
-    std::vector<int> list;
-    const Layout& L = x->L;
-    for (size_t q = 0; q + 1 < x->touched_ij.size(); q += 2) {
-        const int i = x->touched_ij[q], j = x->touched_ij[q + 1];
-        if (i < L.ng || i >= L.nx + L.ng || j < L.ng || j >= L.ny + L.ng) continue;  // ring cells: no partials
-        const int b = ((j - L.ng) / a.rows) * a.nbx + (i - L.ng) / STRIP_W;
-        if (!mark[b]) { mark[b] = 1; list.push_back(b); }
-    }
-    x->rb_error = 0;
-    if ((int)list.size() > x->rb_cap) { x->rb_error = 1; return; }
-    if (!list.empty() && scopy(x, x->rblocks, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
-        x->rb_error = 1;
-        return;
-    }
-    x->nrblocks = (int)list.size();
-    x->rb_nbx = a.nbx;
-    x->rb_rows = a.rows;
 }
 
 // SimpleBound fills its ghosts in the reference's sequential order (i outer, j
@@ -805,6 +770,7 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     step_grid(L, x->rows_override, &a.nbx, &a.nby, &a.rows);
     a.cut = x->cfg.boundary == IBTFO_CUTCELL ? x->cuts : nullptr;
     a.cut_index = x->cut_index;
+    if (a.cut) a.type = x->stype;
 
     FillArgs& f = x->fa;
     memset(&f, 0, sizeof f);
@@ -831,6 +797,10 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     fn.cfl = c.CFL_number; fn.dx = a.dx; fn.t_end = c.Psy_time; fn.max_iter = c.max_iter;
     fn.advance = 1; fn.reset = 0;
     fn.fold = x->fold;
+    if (c.boundary == IBTFO_CUTCELL && x->ngroups) {  // the merge kernel's records follow the step kernel's
+        fn.part2 = x->merge_part;
+        fn.nblocks2 = merge_blocks(x->ngroups);
+    }
 
     ReduceArgs& q = x->ra;
     memset(&q, 0, sizeof q);
@@ -838,8 +808,6 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     q.gam = c.gamma; q.gm1 = c.gamma - 1; q.dx = a.dx; q.dy = a.dy; q.cfl = c.CFL_number;
     q.t_end = c.Psy_time; q.max_iter = c.max_iter;
     q.nblocks = std::min(x->mass_cap, std::max(1, (L.nx * L.ny + 255) / 256));
-    q.buf[0] = x->state[0]; q.buf[1] = x->state[1];
-    q.from_step = 0;
     q.cut = a.cut;
     q.cut_index = x->cut_index;
 
@@ -849,7 +817,8 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     mg.plane = L.plane;
     mg.ctl = x->ctl; mg.groups = x->groups; mg.ngroups = x->ngroups;
     mg.gm1 = c.gamma - 1; mg.r_gm1 = 1.0 / (c.gamma - 1); mg.quirks = c.quirks;
-    cut_blocks(x);
+    mg.dx = a.dx; mg.dy = a.dy;
+    mg.part = x->merge_part;
 }
 
 // Boundary and Global_Boundary after an update (the state just made current):
@@ -905,21 +874,9 @@ static int enqueue_step(ibtfo_ctx* x, hipEvent_t* ev = nullptr) {
         if (ev) HIPCHK(hipEventRecord(ev[1], x->stream));
     }
     if (c.boundary == IBTFO_CUTCELL) {
-        // small cells merged, then ComputeDt / mass over the merged state
+        // small cells merged (with the merged cells' dt / mass partials), then
+        // ComputeDt / mass over both kernels' partials
         HIPCHK(launch_merge(x->ma, x->stream));
-        if (x->rb_error || x->rb_nbx != x->sa.nbx || x->rb_rows != x->sa.rows) {
-            set_error("cut-cell block list not built for the current step grid");
-            return IBTFO_E_STATE;
-        }
-        ReduceArgs r = x->ra;
-        r.from_step = 1;
-        r.blocks = x->rblocks;
-        r.nblocks = x->nrblocks;
-        r.nbx = x->sa.nbx;
-        r.rows = x->sa.rows;
-        HIPCHK(launch_reduce_partials(r, x->stream));
-        HIPCHK(launch_finish(x->fin, x->stream));
-        return enqueue_boundaries(x);
     }
     HIPCHK(launch_finish(x->fin, x->stream));
     return enqueue_boundaries(x);

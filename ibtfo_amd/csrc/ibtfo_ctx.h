@@ -43,11 +43,8 @@ struct ibtfo_ctx {
     int* cut_index;
     ibtfo::MergeGroup* groups;
     int ncut, ngroups;
-    // step blocks whose partials the merge invalidates (they hold a cut cell or a
-    // group member), for the current step grid; rebuilt when the grid changes
-    std::vector<int> touched_ij;  // (i, j) of every cut cell and group member
-    int* rblocks;
-    int nrblocks, rb_cap, rb_nbx, rb_rows, rb_error;
+    uint8_t* stype;       // the cut-cell step kernel's type plane: type | MERGED_BIT for group cells
+    double* merge_part;   // PART_STRIDE doubles per merge block
     hipGraphExec_t graph;
     int graph_chunk;
     std::vector<double> corner_host;

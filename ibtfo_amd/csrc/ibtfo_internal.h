@@ -61,7 +61,7 @@ struct MergeGroup {
 struct StepArgs {
     double* buf[2];     // state buffers
     Layout L;
-    const uint8_t* type;
+    const uint8_t* type;  // cell types; the cut variant's plane adds MERGED_BIT
     Ctl* ctl;
     double* mass_part;  // per-block partials, PART_STRIDE doubles each
     const double* corner;  // [2 sides][2ng rows][4]: U_NEW ring at columns ng / N_x+ng-1
@@ -91,7 +91,14 @@ struct MergeArgs {
     int ngroups;
     double gm1, r_gm1;
     int quirks;
+    double dx, dy;
+    double* part;  // PART_STRIDE doubles per merge block: dt / mass partials of the groups' cells
 };
+constexpr int MERGE_THREADS = 64;
+// in the cut-cell step kernel's type plane: a merge group's cell (target or
+// member), whose dt / mass partials the merge kernel writes
+constexpr uint8_t MERGED_BIT = 4;
+inline int merge_blocks(int ngroups) { return (ngroups + MERGE_THREADS - 1) / MERGE_THREADS; }
 
 // Ghost-cell records (compacted list of interior cells of type 2), built once.
 struct Ghost {
@@ -142,9 +149,7 @@ struct EdgeArgs {
 };
 
 struct ReduceArgs {
-    const double* U;  // state buffer (when from_step = 0)
-    double* buf[2];   // from_step = 1: the buffer the step just wrote, buf[ctl->cur ^ 1]
-    int from_step;
+    const double* U;  // state buffer
     const CutCell* cut;  // cut cells count with their volume fraction
     const int* cut_index;
     Layout L;
@@ -155,9 +160,6 @@ struct ReduceArgs {
     int max_iter;
     int nblocks;
     int reset;  // 1: set iter = 0, now_t = 0 (start of a run)
-    // from_step = 1: recompute the partials of the listed step blocks only
-    const int* blocks;
-    int nbx, rows;
 };
 
 // the dt / mass reduction over the step kernel's per-block partials
@@ -176,6 +178,9 @@ struct FinishArgs {
     // `fold_per` consecutive partials into fold[] first, then one block finishes
     double* fold;
     int nfold, fold_per;
+    // a second run of partials after part (the merge kernel's), null if none
+    const double* part2 = nullptr;
+    int nblocks2 = 0;
 };
 constexpr int FOLD_THREADS = 256;
 constexpr int FOLD_MAX = 128;      // fold records
@@ -197,7 +202,6 @@ hipError_t launch_scatter(const FillArgs& a, hipStream_t s);
 hipError_t launch_edge(const EdgeArgs& a, hipStream_t s);
 hipError_t launch_fill_edge(const FillArgs& f, const EdgeArgs& e, hipStream_t s);  // fill phase 1 + ring
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
-hipError_t launch_reduce_partials(const ReduceArgs& a, hipStream_t s);  // without the finish
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s);
 hipError_t launch_finish(const FinishArgs& f, hipStream_t s);
 // batched forms: one launch over many solves (maps: block -> {slot, first index})

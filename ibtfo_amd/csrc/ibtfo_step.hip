@@ -101,6 +101,11 @@ __device__ __forceinline__ void block_partials(double* part, int bid, double ama
 // the block partials, then the host loop's bookkeeping of Test/Sod/Sod.cpp:154-177:
 // dt for the next step, now_t += dt, iter++, stop test, log entry. `advance` = 0
 // for the stand-alone ComputeDt (no step taken; `reset` starts the run clock).
+// record q of the partials (part, then part2)
+__device__ __forceinline__ const double* part_rec(const FinishArgs& f, int q) {
+    return q < f.nblocks ? f.part + PART_STRIDE * (long long)q : f.part2 + PART_STRIDE * (long long)(q - f.nblocks);
+}
+
 template <int NT> __device__ __forceinline__ void finish_body(const FinishArgs& f, const int advance) {
     Ctl* ctl = f.ctl;
     if (advance && ctl->done) return;  // the step kernel did not run
@@ -119,17 +124,18 @@ template <int NT> __device__ __forceinline__ void finish_body(const FinishArgs& 
     // batches of FB records per thread, all loads issued before any is used, so
     // one memory round trip covers FB records (the loop is latency bound)
     constexpr int FB = 8;
-    for (int q0 = t; q0 < f.nblocks; q0 += FB * NT) {
+    const int nrec = f.nblocks + (f.part2 ? f.nblocks2 : 0);
+    for (int q0 = t; q0 < nrec; q0 += FB * NT) {
         double ra[FB], ru[FB], rv[FB], rm[FB], rb[FB];
 #pragma unroll
         for (int b = 0; b < FB; b++) {
             const int q = q0 + b * NT;
-            const double* r = f.part + PART_STRIDE * (long long)(q < f.nblocks ? q : q0);
+            const double* r = part_rec(f, q < nrec ? q : q0);
             ra[b] = r[0]; ru[b] = r[1]; rv[b] = r[2]; rm[b] = r[3]; rb[b] = r[4];
         }
 #pragma unroll
         for (int b = 0; b < FB; b++) {
-            if (q0 + b * NT >= f.nblocks) break;
+            if (q0 + b * NT >= nrec) break;
             A = (A < ra[b]) ? ra[b] : A;
             Um = (Um < ru[b]) ? ru[b] : Um;
             Vm = (Vm < rv[b]) ? rv[b] : Vm;
@@ -222,11 +228,12 @@ __device__ __forceinline__ void block4_partials(double* part, int bid, double am
 // write back and invalidate little.
 __global__ __launch_bounds__(FOLD_THREADS) void k_fold(FinishArgs f) {
     if (f.ctl->done) return;  // the step kernel did not run (every block sees the same flag)
-    const int q0 = blockIdx.x * f.fold_per, q1 = min(q0 + f.fold_per, f.nblocks);
+    const int nrec = f.nblocks + (f.part2 ? f.nblocks2 : 0);
+    const int q0 = blockIdx.x * f.fold_per, q1 = min(q0 + f.fold_per, nrec);
     double A = 0.0, U = 0.0, V = 0.0, m = 0.0;
     int bad = 0;
     for (int q = q0 + (int)threadIdx.x; q < q1; q += FOLD_THREADS) {
-        const double* r = f.part + PART_STRIDE * (long long)q;
+        const double* r = part_rec(f, q);
         A = (A < r[0]) ? r[0] : A;
         U = (U < r[1]) ? r[1] : U;
         V = (V < r[2]) ? r[2] : V;
@@ -246,6 +253,8 @@ __global__ __launch_bounds__(FOLD_THREADS) void k_fold(FinishArgs f) {
     FinishArgs g = f;
     g.part = f.fold;
     g.nblocks = gridDim.x;
+    g.part2 = nullptr;
+    g.nblocks2 = 0;
     finish_body<FOLD_THREADS>(g, 1);
 }
 
@@ -526,7 +535,9 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
             if (upd_col) {
                 double un[4];
                 const long long o = L.at(c, j);
-                const bool fluid = a.type[o] == 0;
+                const uint8_t tc = a.type[o];
+                const bool fluid = CUT ? (tc & 3) == 0 : tc == 0;
+                const bool merged = CUT && (tc & MERGED_BIT);
                 if (fluid) {
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
@@ -540,7 +551,13 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
                         }
                     }
                     if (stage == 0) {
-                        floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, amax, umax, vmax, bad);
+                        if (!merged) {
+                            floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, amax, umax, vmax, bad);
+                        } else {  // a merge target: floored here, its partials come from the merge
+                            double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+                            int db = 0;
+                            floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, d0, d1, d2, db);
+                        }
                     } else {
                         // RK2 (Heun): each stage's update takes the floors; the second
                         // stage then averages with U^n, which the output buffer holds
@@ -553,8 +570,8 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
                             dt_terms(a.gam, a.gm1, un, amax, umax, vmax, bad);
                         }
                     }
-                    msum += un[0] * dx * dy;
-                } else if (CUT && a.type[o] == 3) {
+                    if (!merged) msum += un[0] * dx * dy;
+                } else if (CUT && (tc & 3) == 3) {
                     // cut cell: aperture-weighted face fluxes and the wall's pressure
                     // force S p (S closes the cell's faces), over the fluid volume
                     // alpha dx dy; small cells are merged (and floored) afterwards
@@ -569,9 +586,16 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
                         double v = w[0][k] - dt * ((cc.ae * Fk - cc.aw * Fm) * dy) / (cc.alpha * dx * dy);
                         un[k] = v - dt * ((cc.an * Gk - cc.as * Gm) * dx + wall[k]) / (cc.alpha * dx * dy);
                     }
-                    if (cc.alpha >= 0.5) floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, amax, umax, vmax, bad);
-                    // (a block holding a cut cell has its partials recomputed after
-                    // the merge, cut-cell mass included)
+                    // a merge-group cell (small cell or target) takes its floors and
+                    // partials from the merge; the others are floored and counted here
+                    if (!merged) {
+                        floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, amax, umax, vmax, bad);
+                        msum += cc.alpha * un[0] * dx * dy;
+                    } else if (cc.alpha >= 0.5) {
+                        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+                        int db = 0;
+                        floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, d0, d1, d2, db);
+                    }
                 } else {
 #pragma unroll
                     for (int k = 0; k < 4; k++) un[k] = w[0][k];
@@ -693,35 +717,8 @@ __global__ __launch_bounds__(64) void k_step_lax(StepArgs a) {
 
 // ComputeDt + CaculateMass of the current state (ComputeDt.cpp:46-67,
 // CaculateMass.cpp:27-38); with `reset`, also starts the run clock.
-__global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
+__global__ __launch_bounds__(64) void k_reduce(ReduceArgs a) {
     const Layout L = a.L;
-    if (a.from_step) {
-        // after the small-cell merge: the partials of the step blocks holding a
-        // cut cell or a merge-group member, over the merged state (the other
-        // blocks' partials, written by the step kernel, are unchanged by the merge)
-        if (a.ctl->done) return;  // no step was taken
-        const double* Us = a.buf[a.ctl->cur ^ 1];
-        const int bid = a.blocks[blockIdx.x];
-        const int bx = bid % a.nbx, by = bid / a.nbx;
-        const int lane = threadIdx.x & 63;
-        const int i = L.ng + STRIP_W * bx + lane;
-        const int j0 = L.ng + a.rows * by, j1 = min(j0 + a.rows, L.ny + L.ng);
-        double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
-        int bad = 0;
-        if (lane < STRIP_W && i < L.nx + L.ng)
-            for (int j = j0 + (threadIdx.x >> 6); j < j1; j += 4) {
-                const long long o = L.at(i, j);
-                const uint8_t t = a.type[o];
-                double U[4] = {Us[o], Us[o + L.plane], Us[o + 2 * L.plane], Us[o + 3 * L.plane]};
-                if (t == 0 || t == 3) {
-                    dt_terms(a.gam, a.gm1, U, amax, umax, vmax, bad);
-                    if (t == 0) msum += U[0] * a.dx * a.dy;
-                    else msum += a.cut[a.cut_index[o]].alpha * U[0] * a.dx * a.dy;
-                }
-            }
-        block4_partials(a.mass_part, bid, amax, umax, vmax, msum, bad);
-        return;
-    }
     const double* Us = a.U;
     const int nint = L.nx * L.ny;
     double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
@@ -742,49 +739,57 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
 
 // Small-cell merging (cut cells): each group's target and members take the
 // volume-fraction-weighted mean of their updated states; the corrected-mode
-// floors, which the small cells skipped in the step, apply to the mean.
-__global__ void k_merge(MergeArgs a) {
-    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= a.ngroups || a.ctl->done) return;
-    const MergeGroup gr = a.groups[gi];
-    double* U = a.buf[a.ctl->cur ^ 1];
-    const long long PL = a.plane;
-    // every load first (absent members read the target, unused), then the sums
-    // in group order
-    double ut[4], um[4][4];
+// floors, which the small cells skipped in the step, apply to the mean. The
+// groups' cells are left out of the step kernel's dt / mass partials; each
+// 64-thread block writes theirs, over the merged state, as one more partial
+// record (mass: the group's volume times the merged density).
+__global__ __launch_bounds__(MERGE_THREADS) void k_merge(MergeArgs a) {
+    if (a.ctl->done) return;  // no step was taken (every block sees the same flag)
+    const int gi = blockIdx.x * MERGE_THREADS + threadIdx.x;
+    double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
+    int bad = 0;
+    if (gi < a.ngroups) {
+        const MergeGroup gr = a.groups[gi];
+        double* U = a.buf[a.ctl->cur ^ 1];
+        const long long PL = a.plane;
+        // every load first (absent members read the target, unused), then the
+        // sums in group order
+        double ut[4], um[4][4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) ut[k] = U[gr.target + k * PL];
+        for (int k = 0; k < 4; k++) ut[k] = U[gr.target + k * PL];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const long long o = q < gr.n ? gr.member[q] : gr.target;
+        for (int q = 0; q < 4; q++) {
+            const long long o = q < gr.n ? gr.member[q] : gr.target;
 #pragma unroll
-        for (int k = 0; k < 4; k++) um[q][k] = U[o + k * PL];
-    }
-    double asum = gr.at;
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-        if (q < gr.n) asum = asum + gr.am[q];
-    double v[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        double s = gr.at * ut[k];
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-            if (q < gr.n) s = s + gr.am[q] * um[q][k];
-        v[k] = s / asum;
-    }
-    if (!a.quirks) {
-        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-        int db = 0;
-        floor_and_dt(a.gm1 + 1.0, a.gm1, Rcp<double>{a.gm1, a.r_gm1}, v, 0, d0, d1, d2, db);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        U[gr.target + k * PL] = v[k];
+            for (int k = 0; k < 4; k++) um[q][k] = U[o + k * PL];
+        }
+        double asum = gr.at;
 #pragma unroll
         for (int q = 0; q < 4; q++)
-            if (q < gr.n) U[gr.member[q] + k * PL] = v[k];
+            if (q < gr.n) asum = asum + gr.am[q];
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double s = gr.at * ut[k];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (q < gr.n) s = s + gr.am[q] * um[q][k];
+            v[k] = s / asum;
+        }
+        floor_and_dt(a.gm1 + 1.0, a.gm1, Rcp<double>{a.gm1, a.r_gm1}, v, a.quirks, amax, umax, vmax, bad);
+        msum = gr.at * v[0] * a.dx * a.dy;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (q < gr.n) msum = msum + gr.am[q] * v[0] * a.dx * a.dy;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            U[gr.target + k * PL] = v[k];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (q < gr.n) U[gr.member[q] + k * PL] = v[k];
+        }
     }
+    block_partials(a.part, blockIdx.x, amax, umax, vmax, msum, bad);
 }
 
 // Ghost-cell fill, phase 1: values from the donors of U_NEW (the state right
@@ -1164,14 +1169,9 @@ hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_f
     hipLaunchKernelGGL(k_edge_ref, dim3((nring + 255) / 256), dim3(256), 0, s, a, corner, corner_from_input);
     return hipGetLastError();
 }
-hipError_t launch_reduce_partials(const ReduceArgs& a, hipStream_t s) {
-    if (a.nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce, dim3(a.nblocks), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s) {
     if (a.ngroups == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_merge, dim3((a.ngroups + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_merge, dim3(merge_blocks(a.ngroups)), dim3(MERGE_THREADS), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
@@ -1185,10 +1185,11 @@ hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
     return launch_finish(f, s);
 }
 hipError_t launch_finish(const FinishArgs& f, hipStream_t s) {
-    if (f.advance == 1 && f.fold && f.nblocks >= FOLD_MIN_PARTS) {
+    const int nrec = f.nblocks + (f.part2 ? f.nblocks2 : 0);
+    if (f.advance == 1 && f.fold && nrec >= FOLD_MIN_PARTS) {
         FinishArgs g = f;
-        g.nfold = std::min(FOLD_MAX, (f.nblocks + FOLD_THREADS - 1) / FOLD_THREADS);
-        g.fold_per = (f.nblocks + g.nfold - 1) / g.nfold;
+        g.nfold = std::min(FOLD_MAX, (nrec + FOLD_THREADS - 1) / FOLD_THREADS);
+        g.fold_per = (nrec + g.nfold - 1) / g.nfold;
         hipLaunchKernelGGL(k_fold, dim3(g.nfold), dim3(FOLD_THREADS), 0, s, g);
         return hipGetLastError();
     }
