@@ -8,9 +8,9 @@ built on the device (mask -> edge pixels -> signed distance), cut-cell boundary
 with small-cell merging, MUSCL + van Leer limiter, HLL flux (the headline) and
 Steger-Warming flux (second leg), supersonic inflow / zero-gradient edges, FP64.
 A step = the step kernel (reconstruction, fluxes, update, per-block dt/mass
-partials), the small-cell merge kernel, the post-merge dt/mass partials of the
-cut cells, the one-block dt/clock kernel, the two ghost-fill kernels and the
-edge kernel. The state is initialised on the device before the timed region and
+partials), the small-cell merge kernel (with the merged cells' dt/mass
+partials), the dt/clock kernel, and the ghost-fill kernels (the first of them
+also fills the edge ring). The state is initialised on the device before the timed region and
 dt never leaves the device. The step kernel's tile height is picked by the
 library's autotuner; the timed loop replays a captured HIP graph of the step
 sequence.

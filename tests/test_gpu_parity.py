@@ -412,10 +412,10 @@ def test_mask_cutcell_bitexact(oracle_mod, riemann):
 
 
 def test_cutcell_strip_heights(oracle_mod):
-    """The post-merge dt/mass partials are recomputed only for the step blocks
-    holding cut cells or merge-group members; that block list follows the strip
-    height: every height (ragged strips included) and a change of height in the
-    middle of a run give the oracle's state, dt and mass."""
+    """The step kernel leaves the merge groups' cells out of its dt/mass partials
+    and the merge kernel adds theirs after the merge: every strip height (ragged
+    strips included) and a change of height in the middle of a run give the
+    oracle's state, dt and mass."""
     import ibtfo_amd as ib
     from ibtfo_amd import workloads as W
     O = oracle_mod
