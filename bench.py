@@ -6,7 +6,9 @@ Workload: 4096 x 2048 cells on [0,2] x [0,1], Mach 3 inflow (rho, u, v, p) =
 polygons) given as a binary mask image (one pixel per cell), whose level set is
 built on the device (mask -> edge pixels -> signed distance), cut-cell boundary
 with small-cell merging, MUSCL + van Leer limiter, HLL flux (the headline) and
-Steger-Warming flux (second leg), supersonic inflow / zero-gradient edges, FP64.
+Steger-Warming flux (second leg), supersonic inflow / zero-gradient edges, FP64;
+a third leg runs the reference's IDW ghost-fluid boundary in place of the cut
+cells.
 A step = the step kernel (reconstruction, fluxes, update, per-block dt/mass
 partials), the small-cell merge kernel (with the merged cells' dt/mass
 partials), the dt/clock kernel, and the ghost-fill kernels (the first of them
@@ -66,11 +68,11 @@ def workload_geometry(args, s):
         s.geometry("shapes", W.bodies_shapes())
 
 
-def gpu_leg(args, precision="FP64", keep_state=False, riemann=None):
+def gpu_leg(args, precision="FP64", keep_state=False, riemann=None, boundary=None):
     import ibtfo_amd as ib
     from ibtfo_amd import workloads as W
     cfg = W.mach3_bodies_config(ib, N_x=args.nx, N_y=args.ny, Riemann=riemann or args.riemann,
-                                Limiter=args.limiter, Boundary=args.boundary)
+                                Limiter=args.limiter, Boundary=boundary or args.boundary)
     cfg.precision = ib.PRECISION[precision]
     s = ib.Solver(cfg)
     if args.geometry == "mask":
@@ -533,6 +535,7 @@ def main():
     ap.add_argument("--cpu-steps-1t", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-idw", action="store_true", help="skip the IDW ghost-fluid leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-stream leg")
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--serve-steps", type=int, default=500)
@@ -584,6 +587,15 @@ def main():
               "roofline_frac": w["achieved"] / HBM_PEAK_GBS, "tile_rows": w["rows"], "steps_taken": w["iter"],
               "status": w["status"]}
         progress("Steger-Warming leg")
+    idw = None
+    if not args.no_idw and args.boundary != "IDWBaseGFM":
+        w = gpu_leg(args, boundary="IDWBaseGFM")
+        idw = {"value": w["value"], "unit": "cell-updates/s", "ms_per_step": w["ms_per_step"],
+               "boundary": "IDWBaseGFM (the reference's inverse-distance ghost fluid)",
+               "kernel": "k_step<MUSCL,HLL,VanLeer>", "kernel_ms": w["kernel_ms"],
+               "roofline_frac": w["achieved"] / HBM_PEAK_GBS, "ghost_cells": w["nghost"], "tile_rows": w["rows"],
+               "steps_taken": w["iter"], "status": w["status"]}
+        progress("IDW ghost-fluid leg")
     serving = None if args.no_serve else serving_leg(args)
     progress("request-stream leg")
     configs = None if args.no_configs else config_legs(args)
@@ -625,6 +637,7 @@ def main():
                    "median_slice_value": g["median"]},
         "cpu_baseline": cpu,
         "steger_warming": sw,
+        "idw_ghost_fluid": idw,
         "mixed_precision": mixed,
         "serving": serving,
         "configs": configs,
