@@ -378,6 +378,44 @@ __global__ void k_cut_ghosts(GeomArgs g) {
     }
 }
 
+// A face between an active cell and an inactive interior one (zero-area
+// polygons where corners are exactly 0, small cells without a partner) is a wall:
+// its aperture becomes 0 and the wall vector is closed again, so no flux leaves
+// the fluid through it. Each cell writes only its own record; types are read.
+__global__ void k_cut_close(GeomArgs g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const Layout L = g.L;
+    const int ng = L.ng;
+    if (i < ng || i >= L.nx + ng || j < ng || j >= L.ny + ng) return;
+    const uint8_t t = g.type[L.at(i, j)];
+    if (t != 0 && t != 3) return;
+    double* cc = g.dcut + cell_q(L, i, j) * 8;
+    const int ni[4] = {i - 1, i + 1, i, i}, nj[4] = {j, j, j - 1, j + 1};
+    bool closed = false;
+    for (int q = 0; q < 4; q++) {
+        if (ni[q] < ng || ni[q] >= L.nx + ng || nj[q] < ng || nj[q] >= L.ny + ng) continue;
+        const uint8_t tq = g.type[L.at(ni[q], nj[q])];
+        if (tq == 0 || tq == 3 || cc[1 + q] == 0.0) continue;
+        cc[1 + q] = 0.0;
+        closed = true;
+    }
+    if (!closed) return;
+    cc[5] = -((cc[2] - cc[1]) * g.cdy);
+    cc[6] = -((cc[4] - cc[3]) * g.cdx);
+}
+
+// a full cell (type 0, all apertures 1) with a face closed above takes the
+// cut-cell update: type 3 with alpha = 1
+__global__ void k_cut_retype(GeomArgs g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const Layout L = g.L;
+    const int ng = L.ng;
+    if (i < ng || i >= L.nx + ng || j < ng || j >= L.ny + ng) return;
+    const long long o = L.at(i, j);
+    const double* cc = g.dcut + cell_q(L, i, j) * 8;
+    if (g.type[o] == 0 && (cc[1] == 0.0 || cc[2] == 0.0 || cc[3] == 0.0 || cc[4] == 0.0)) g.type[o] = 3;
+}
+
 // compaction: records of the type-3 cells, and merge groups gathered by each
 // target from its W, E, S, N neighbours (counting only when the capacity is 0)
 __global__ void k_cut_compact(GeomArgs g) {
@@ -428,6 +466,8 @@ hipError_t cut_run(GeomArgs& g, hipStream_t s) {
     hipLaunchKernelGGL(k_cut_cells, grd, blk, 0, s, g);
     hipLaunchKernelGGL(k_cut_target, grd, blk, 0, s, g);
     hipLaunchKernelGGL(k_cut_ghosts, grd, blk, 0, s, g);
+    hipLaunchKernelGGL(k_cut_close, grd, blk, 0, s, g);
+    hipLaunchKernelGGL(k_cut_retype, grd, blk, 0, s, g);
     return hipGetLastError();
 }
 hipError_t cut_compact(GeomArgs& g, hipStream_t s) {

@@ -776,7 +776,8 @@ void or_unit_normals(int nx, int ny, int ng, double* XY) {
 // 2 an inactive cell within ng of an active one (filled like an IDW ghost) and
 // the domain ring. A cut cell with alpha < 1/2 is merged with a neighbour with
 // alpha >= 1/2: along the normal's dominant axis first, then the other axis,
-// then the opposite directions; with none it becomes inactive.
+// then the opposite directions; with none it becomes inactive. Finally every
+// face between an active and an inactive interior cell is closed.
 static double cut_frac(double p, double q) {
     if (p >= 0 && q >= 0) return 1.0;
     if (p < 0 && q < 0) return 0.0;
@@ -866,6 +867,34 @@ void or_cut_geometry(int nx, int ny, int ng, double L, double H, double* XY, dou
                         near = true;
             }
             if (near) XY[g.c(i, j) * 6 + 5] = 2;
+        }
+    // a face between an active cell and an inactive interior one (zero-area
+    // polygons where corners are exactly 0, small cells without a partner) is a
+    // wall: aperture 0 and the wall vector closed again; a full cell with such a
+    // face takes the cut-cell update (type 3, alpha = 1), so no flux leaves the
+    // fluid through it
+    for (size_t q = 0; q < t5.size(); q++) t5[q] = XY[q * 6 + 5];
+    for (int j = ng; j < ny + ng; j++)
+        for (int i = ng; i < nx + ng; i++) {
+            const double ti = t5[g.c(i, j)];
+            if (ti != 0 && ti != 3) continue;
+            double* cc = cut + g.c(i, j) * 8;
+            const int ni[4] = {i - 1, i + 1, i, i}, nj[4] = {j, j, j - 1, j + 1};
+            bool closed = false;
+            for (int q = 0; q < 4; q++) {
+                if (!interior(ni[q], nj[q])) continue;
+                const double tq = t5[g.c(ni[q], nj[q])];
+                if (tq == 0 || tq == 3 || cc[1 + q] == 0.0) continue;
+                cc[1 + q] = 0.0;
+                closed = true;
+            }
+            if (!closed) continue;
+            cc[5] = -((cc[2] - cc[1]) * dy);
+            cc[6] = -((cc[4] - cc[3]) * dx);
+            if (ti == 0) {
+                cc[0] = 1.0;
+                XY[g.c(i, j) * 6 + 5] = 3;
+            }
         }
 }
 
@@ -1334,8 +1363,11 @@ void or_initialize(int kind, const double* prm, int nx, int ny, int ng, double g
 // all edge pixels. The edge-magnitude slot is set for pixels 1..W-2 x 1..H-2
 // only; the reference leaves the border pixels' slot uninitialised and scans
 // all pixels, which is taken here as 0 (no edge).
-void or_mask_phi(const unsigned char* rgba, int W, int H, int nx, int ny, int ng, double lo_x, double lo_y,
-                 double hi_x, double hi_y, const double* XY, double* phi) {
+// With cells != null, only the ncell listed cells (reference cell order) are
+// computed, into phi[0 .. ncell): the same search for a sample of a large grid.
+static void mask_phi_impl(const unsigned char* rgba, int W, int H, int nx, int ny, int ng, double lo_x,
+                          double lo_y, double hi_x, double hi_y, const double* XY, double* phi, const long long* cells,
+                          long long ncell) {
     const double Psy_L = hi_x - lo_x, Psy_H = hi_y - lo_y;
     std::vector<double> v((size_t)W * H), g((size_t)W * H, 0.0);
     for (int y = 0; y < H; y++)
@@ -1365,20 +1397,37 @@ void or_mask_phi(const unsigned char* rgba, int W, int H, int nx, int ny, int ng
             }
         }
     const int M = nx + 2 * ng, N = ny + 2 * ng;
-    for (int i = 0; i < M; i++)
-        for (int j = 0; j < N; j++) {
-            int x = (i + 0.5) * (W) / (M);
-            int y = (j + 0.5) * (H) / (N);
-            const double sgn = v[(size_t)y * W + x];
-            const double X = XY[((size_t)j * M + i) * 6 + 0], Y = XY[((size_t)j * M + i) * 6 + 1];
-            double min_distance = 100;
-            for (int e = 0; e < edg_num; e++) {
-                double d = std::sqrt((ex[e] - X) * (ex[e] - X) + (ey[e] - Y) * (ey[e] - Y));
-                if (d < min_distance) min_distance = d;
-            }
-            int pn = sgn >= 0 ? 1 : -1;
-            phi[(size_t)j * M + i] = pn * min_distance;
+    auto at = [&](int i, int j) {
+        int x = (i + 0.5) * (W) / (M);
+        int y = (j + 0.5) * (H) / (N);
+        const double sgn = v[(size_t)y * W + x];
+        const double X = XY[((size_t)j * M + i) * 6 + 0], Y = XY[((size_t)j * M + i) * 6 + 1];
+        double min_distance = 100;
+        for (int e = 0; e < edg_num; e++) {
+            double d = std::sqrt((ex[e] - X) * (ex[e] - X) + (ey[e] - Y) * (ey[e] - Y));
+            if (d < min_distance) min_distance = d;
         }
+        int pn = sgn >= 0 ? 1 : -1;
+        return pn * min_distance;
+    };
+    if (cells) {
+#pragma omp parallel for schedule(static)
+        for (long long q = 0; q < ncell; q++) phi[q] = at((int)(cells[q] % M), (int)(cells[q] / M));
+        return;
+    }
+    for (int i = 0; i < M; i++)
+        for (int j = 0; j < N; j++) phi[(size_t)j * M + i] = at(i, j);
+}
+
+void or_mask_phi(const unsigned char* rgba, int W, int H, int nx, int ny, int ng, double lo_x, double lo_y,
+                 double hi_x, double hi_y, const double* XY, double* phi) {
+    mask_phi_impl(rgba, W, H, nx, ny, ng, lo_x, lo_y, hi_x, hi_y, XY, phi, nullptr, 0);
+}
+
+void or_mask_phi_cells(const unsigned char* rgba, int W, int H, int nx, int ny, int ng, double lo_x, double lo_y,
+                       double hi_x, double hi_y, const double* XY, const long long* cells, long long ncell,
+                       double* phi) {
+    mask_phi_impl(rgba, W, H, nx, ny, ng, lo_x, lo_y, hi_x, hi_y, XY, phi, cells, ncell);
 }
 
 // TimeAdvance. SingleStep (SingleStep/SingleStep.cpp:5-21) is one Advance.

@@ -210,6 +210,24 @@ def mask_phi(case, rgba):
     return phi
 
 
+def mask_phi_cells(case, rgba, cells):
+    """mask_phi at the listed cells only (reference cell order j*M+i); the XY
+    of the grid comes from Psy_coord."""
+    L = lib()
+    g = case
+    XY = np.zeros(g.M * g.N * 6)
+    L.or_psy_coord(C.c_double(g.lo_x), C.c_double(g.lo_y), C.c_double(g.L), C.c_double(g.H),
+                   g.N_x, g.N_y, g.num_ghost_cell, ptr(XY))
+    img = np.ascontiguousarray(rgba, dtype=np.uint8)
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    phi = np.zeros(len(cells))
+    L.or_mask_phi_cells(img.ctypes.data_as(C.c_void_p), img.shape[1], img.shape[0], g.N_x, g.N_y,
+                        g.num_ghost_cell, C.c_double(g.lo_x), C.c_double(g.lo_y), C.c_double(g.hi_x),
+                        C.c_double(g.hi_y), ptr(XY), cells.ctypes.data_as(C.c_void_p), C.c_longlong(len(cells)),
+                        ptr(phi))
+    return phi
+
+
 def run_oracle(case, nsteps, root_mode=0, state=None):
     """Whole run like the reference driver, at most nsteps steps. Returns a dict."""
     L = lib()
