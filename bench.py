@@ -34,6 +34,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU legs' OpenMP threads stay on neighbouring cores (read when the OpenMP
+# runtime starts, with the first load of the oracle library)
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 def _say(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
@@ -165,8 +168,8 @@ def cpu_leg(args):
     el1 = time.perf_counter() - t1
     O.lib().or_set_num_threads(threads)
     return dict(value=nx * ny * r["steps"] / el, cores=threads, steps=r["steps"], seconds=el,
-                sample="%dx%d cells, %d steps of the same workload (oracle, OpenMP %d threads)" % (
-                    nx, ny, r["steps"], threads),
+                sample="%dx%d cells, %d steps of the same workload (oracle, OpenMP %d threads, OMP_PROC_BIND=%s)"
+                % (nx, ny, r["steps"], threads, os.environ.get("OMP_PROC_BIND")),
                 value_1t=nx * ny * r1["steps"] / el1, steps_1t=r1["steps"])
 
 
