@@ -30,15 +30,17 @@ def mask_img():
     return W.bodies_mask(NX, NY)
 
 
-@pytest.mark.parametrize("boundary", ["CutCell", "IDWBaseGFM"])
-def test_fullsize_mask_workload_bitexact(oracle_mod, mask_img, boundary):
-    """The benchmark's workload at 4096x2048 (and with the reference's IDW ghost
-    fluid in place of the cut cells): cell types, and state and dt after 10
-    steps, equal the oracle's bit for bit on the device's level set."""
+@pytest.mark.parametrize("boundary,riemann", [("CutCell", "HLL"), ("IDWBaseGFM", "HLL"), ("CutCell", "StegerWarming")])
+def test_fullsize_mask_workload_bitexact(oracle_mod, mask_img, boundary, riemann):
+    """The benchmark's workload at 4096x2048 (its HLL headline, its
+    Steger-Warming leg, and the reference's IDW ghost fluid in place of the cut
+    cells): cell types, and state and dt after 10 steps, equal the oracle's bit
+    for bit on the device's level set."""
     import ibtfo_amd as ib
     from ibtfo_amd import workloads as W
     O = oracle_mod
     c = _mask_case(O, Boundary=boundary)
+    c.Riemann = riemann
     s = ib.Solver(gpu_config(c))
     s.geometry_mask(mask_img)
     XY = s.xycoord()
