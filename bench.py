@@ -533,9 +533,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default tile shape, no autotuning")
     ap.add_argument("--tune-steps", type=int, default=20)
     ap.add_argument("--graph-chunk", type=int, default=100, help="steps per captured HIP graph (0: plain launches)")
-    ap.add_argument("--cpu-steps", type=int, default=600)
+    ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-steps-1t", type=int, default=60)
+    ap.add_argument("--cpu-steps-1t", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-idw", action="store_true", help="skip the IDW ghost-fluid leg")
