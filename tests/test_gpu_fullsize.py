@@ -164,3 +164,28 @@ def test_fullsize_mask_rest_and_mass(oracle_mod, mask_img):
     s.close()
     assert abs(m1 - m0) <= 1e-12 * m0, (m0, m1)
     assert abs(ml[-1] - m1) <= 1e-12 * m1, (ml[-1], m1)
+
+
+@pytest.mark.parametrize("riemann", ["HLL", "PVRS"])
+def test_fullsize_x_mirror_symmetry(oracle_mod, riemann):
+    """A state mirror-symmetric about x = 1/2 at 4096x2048 (periodic, MUSCL +
+    minmod, corrected mode) stays mirror-symmetric bit for bit for 50 steps on
+    the device: strips on either side of the centre line, and the left and
+    right halves of each strip, do the same arithmetic on mirrored data."""
+    import ibtfo_amd as ib
+    from mirror_util import mirror_defects, mirror_state
+    O = oracle_mod
+    c = O.Case(N_x=NX, N_y=NY, hi_x=1.0, hi_y=NY / NX, Scheme="MUSCL", Riemann=riemann, Limiter="minmod",
+               case="uniform", quirks=0, edge="periodic", Psy_time=1e30, max_iter=10 ** 9, CFL_number=0.5)
+    U0 = mirror_state(NX, NY).reshape(-1)
+    assert mirror_defects(U0, c.M, c.N) == 0
+    s = ib.Solver(gpu_config(c))
+    s.geometry("none", ())
+    s.upload(U0, U0)
+    s.prepare(True)
+    s.step(50)
+    assert s.status()[0] == 50
+    U = s.state()
+    s.close()
+    assert np.isfinite(U).all() and np.abs(U - U0).max() > 1e-3
+    assert mirror_defects(U, c.M, c.N) == 0
