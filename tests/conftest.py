@@ -16,4 +16,8 @@ def pytest_configure(config):
 def oracle_mod():
     from oracle import oracle as O
     O.lib()
+    if os.environ.get("PYTEST_XDIST_WORKER"):
+        # several workers share the cores: one OpenMP thread each (spinning
+        # thread teams oversubscribed many times over slow the suite 40x)
+        O.lib().or_set_num_threads(1)
     return O
