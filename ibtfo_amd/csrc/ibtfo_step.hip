@@ -484,9 +484,10 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
 
     for (int j = j0 - 1; j < j1; j++) {
         const int b = j & 1;
-        // prefetch the next window row and next halo value
+        // prefetch the next window row and next halo value (none after the last
+        // row: they would only be discarded)
         double nw[4], nex[4] = {0, 0, 0, 0};
-        {
+        if (j + 1 < j1) {
             long long o = L.at(cl, min(j + 3, L.N - 1));
 #pragma unroll
             for (int k = 0; k < 4; k++) nw[k] = Ui[o + k * PL];
