@@ -211,7 +211,9 @@ def config_legs(args):
     isentropic vortex (1024^2, periodic, MUSCL + van Leer + HLL with Heun's RK2,
     to t = 10, FP64 and mixed, L1 density error against the exact solution).
     GPU: HIP events around the whole run (graph-replayed steps); CPU: the oracle
-    (OpenMP) on the whole run, or its first `cpu_steps` steps where stated."""
+    (OpenMP) on the whole run, or its first `cpu_steps` steps where stated, and,
+    for the two configurations the reference itself can run, its own
+    translation units built here (`cpu_reference`)."""
     import ibtfo_amd as ib
     from ibtfo_amd import workloads as W
     from oracle import oracle as O
@@ -241,6 +243,27 @@ def config_legs(args):
         r = O.run_oracle(case, nsteps, state=st)
         return r["steps"], time.perf_counter() - t0
 
+    def ref_run(case, nsteps, cells, note):
+        """The reference's own translation units (oracle/_ref, built from its
+        sources with g++ -fopenacc: OpenACC's host fallback, one thread), with
+        their console output sent to stderr; None where they are not built."""
+        if not O.ref_available(case):
+            return None
+        import ctypes as C
+        libc = C.CDLL(None)
+        sys.stdout.flush()
+        saved = os.dup(1)
+        os.dup2(2, 1)
+        try:
+            t0 = time.perf_counter()
+            r = O.run_ref(case, nsteps)
+            el = time.perf_counter() - t0
+        finally:
+            libc.fflush(None)
+            os.dup2(saved, 1)
+            os.close(saved)
+        return dict(value=cells * r["steps"] / el, steps=r["steps"], cores=1, kind="reference", note=note)
+
     # 400x100 Sod tube
     cfg = ib.config(N_x=400, N_y=100, hi_x=1.0, hi_y=0.25, Psy_time=0.2, max_iter=10 ** 9, CFL_number=0.5,
                     Scheme="MHM", Riemann="HLLC", Limiter="minmod", Boundary="Regular")
@@ -249,10 +272,12 @@ def config_legs(args):
     c = O.Case(N_x=400, N_y=100, hi_y=0.25, Scheme="MHM", Riemann="HLLC", Limiter="minmod", case="sodx",
                Psy_time=0.2)
     cn, cs = cpu_run(c, 10 ** 6)
+    c.Riemann = "HLL"  # the reference's HLLC directory holds HLL's code; its build is the HLL one
+    ref = ref_run(c, 10 ** 6, 400 * 100, "the whole run; the x-split Sod hook compiled into the reference's loop")
     _say("Sod tube measured")
     out.append(dict(workload="Sod tube 400x100 on [0,1]x[0,0.25], MHM + minmod + HLLC, Regular, to t = 0.2",
                     steps=n, gpu_value=400 * 100 * n / (ms / 1e3), gpu_ms=ms, status=rc,
-                    cpu_value=400 * 100 * cn / cs, cpu_steps=cn, cpu_cores=threads))
+                    cpu_value=400 * 100 * cn / cs, cpu_steps=cn, cpu_cores=threads, cpu_reference=ref))
     # inclined Sod channel, 800x400, IDW
     cfg = ib.config(N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Psy_time=0.2, max_iter=10 ** 9, CFL_number=0.5,
                     Scheme="MUSCL", Riemann="HLL", Limiter="minmod", Boundary="IDWBaseGFM")
@@ -261,12 +286,13 @@ def config_legs(args):
     s.close()
     c = O.Case(N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Boundary="IDWBaseGFM", case="incline", Psy_time=0.2)
     cn, cs = cpu_run(c, args.cpu_incline_steps)
+    ref = ref_run(c, args.ref_incline_steps, 800 * 400, "the first %d steps of the run" % args.ref_incline_steps)
     _say("inclined channel measured")
     out.append(dict(workload="inclined Sod channel 800x400 on [0,2]x[0,1] (30 deg, AxisD 0.4), MUSCL + minmod "
                              "+ HLL, IDW ghost fluid, to t = 0.2",
                     steps=n, gpu_value=800 * 400 * n / (ms / 1e3), gpu_ms=ms, status=rc,
                     cpu_value=800 * 400 * cn / cs, cpu_steps=cn, cpu_cores=threads,
-                    cpu_note="first %d steps of the run" % cn))
+                    cpu_note="first %d steps of the run" % cn, cpu_reference=ref))
     # isentropic vortex 1024^2, FP64 and mixed
     nv = 1024
     h = 10.0 / nv
@@ -529,6 +555,8 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=500, help="steps of the per-launch kernel timing pass")
     ap.add_argument("--cpu-vortex-steps", type=int, default=24)
     ap.add_argument("--cpu-incline-steps", type=int, default=120)
+    ap.add_argument("--ref-incline-steps", type=int, default=20,
+                    help="steps of the inclined channel run by the reference's own build")
     ap.add_argument("--rows", type=int, default=0, help="fixed rows per strip (skips the autotuner)")
     ap.add_argument("--no-tune", action="store_true", help="default tile shape, no autotuning")
     ap.add_argument("--tune-steps", type=int, default=20)
