@@ -24,6 +24,8 @@ using namespace ibtfo;
 // ------------------------------------------------------------------ arena
 namespace ibtfo {
 static const size_t ARENA_ALIGN = 256;
+// admissions per progress call while no chunk is running (the next call starts one)
+static const int ADMIT_BURST = 8;
 
 int Arena::init(size_t bytes) {
     cap_ = bytes / ARENA_ALIGN * ARENA_ALIGN;
@@ -457,17 +459,23 @@ extern "C" int ibtfo_batch_progress(ibtfo_batch* b, int* active, int* queued, in
         HIPCHK(hipGraphLaunch(b->graph, b->run));
         launched = true;
     }
-    // admissions overlap the chunk: geometry and initial state on the admission stream
+    // admissions overlap the chunk (geometry and initial state on the admission
+    // stream): they go on while the chunk runs, at least one per call, so that
+    // the GPU never waits for a whole wave of set-ups; with no chunk running,
+    // a short burst is admitted and the next call starts the GPU
     int free_slots = 0;
     for (auto& s : b->slots) free_slots += s.ctx == nullptr;
     free_slots -= (int)b->pending.size();
     const double tnow = b->now();
+    int tried = 0;
     while (!b->queue.empty() && free_slots > 0 && b->queue.front().arrival <= tnow) {
+        if (launched ? (tried > 0 && hipStreamQuery(b->run) != hipErrorNotReady) : tried >= ADMIT_BURST) break;
         int rs = 0, rg = 0, rr = 0;
         for (auto& p : b->pending) { rs += step_blocks(p.first); rg += ghost_blocks(p.first); rr += ring_blocks(p.first); }
         int a = admit_one(b, rs, rg, rr);
         if (a == 0) break;
         if (a == 1) free_slots--;
+        tried++;
     }
     if (launched) {
         HIPCHK(hipStreamSynchronize(b->run));
