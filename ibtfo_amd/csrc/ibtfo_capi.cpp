@@ -188,9 +188,8 @@ int ctx_create(const ibtfo_config* cfg, Arena* arena, hipStream_t stream, int ro
         hipMemsetAsync(x->phi, 0, L.plane * sizeof(double), x->stream) != hipSuccess ||
         hipMemsetAsync(x->gnx, 0, L.plane * sizeof(double), x->stream) != hipSuccess ||
         hipMemsetAsync(x->gny, 0, L.plane * sizeof(double), x->stream) != hipSuccess ||
-        hipMemsetAsync(x->corner, 0, 2 * 2 * L.ng * 4 * sizeof(double), x->stream) != hipSuccess ||
-        hipStreamSynchronize(x->stream) != hipSuccess)
-        return fail("initial clear");
+        hipMemsetAsync(x->corner, 0, 2 * 2 * L.ng * 4 * sizeof(double), x->stream) != hipSuccess)
+        return fail("initial clear");  // stream-ordered before everything else on the solve's stream
     const double Psy_L = cfg->hi_x - cfg->lo_x;
     x->dxc = Psy_L / cfg->N_x;  // Psy_coord.cpp:21-22 (y spaced with dx as well)
     x->Li = -x->dxc * L.ng + cfg->lo_x + x->dxc / 2;
@@ -213,6 +212,7 @@ extern "C" int ibtfo_destroy(ibtfo_ctx* x) {
 
 extern "C" int ibtfo_set_stream(ibtfo_ctx* x, void* s) {
     if (!x) return IBTFO_E_ARG;
+    HIPCHK(hipStreamSynchronize(x->stream));  // work queued on the old stream ends before the new one starts
     x->stream = s ? (hipStream_t)s : x->own_stream;
     if (x->graph) { (void)hipGraphExecDestroy(x->graph); x->graph = nullptr; x->graph_chunk = 0; }
     return IBTFO_OK;

@@ -117,7 +117,8 @@ int ibtfo_inp_string(const char* path, const char* key, char* buf, int cap);
  * scheme index, ghost list, control block) on the current HIP device. */
 int ibtfo_create(const ibtfo_config* cfg, ibtfo_ctx** out);
 int ibtfo_destroy(ibtfo_ctx* ctx);
-/* Run on this HIP stream (hipStream_t, default priority); NULL = the context's own stream. */
+/* Run on this HIP stream (hipStream_t, default priority); NULL = the context's own stream.
+ * Waits for the work already queued on the previous stream first. */
 int ibtfo_set_stream(ibtfo_ctx* ctx, void* stream);
 /* sizes: M = N_x + 2 ng columns, N = N_y + 2 ng rows */
 int ibtfo_dims(const ibtfo_ctx* ctx, int* M, int* N);
