@@ -67,6 +67,26 @@ def test_alone_equals_mix_with_staggered_arrivals(prec):
     b.close()
 
 
+def test_many_queued_at_once_equal_alone():
+    """24 requests queued at once on 16 slots: a burst is admitted, the rest
+    while chunks run and as slots free up; every result equals its run alone
+    bit for bit."""
+    from ibtfo_amd import serving as S
+    st, items = _items("FP64", n=24, seed=11)
+    alone = [_run_alone(it, "FP64") for it in items]
+    b = S.Batch(_bc("FP64"))
+    tickets = {b.submit(**it): i for i, it in enumerate(items)}
+    b.drain()
+    assert b.stats().finished == len(items)
+    for t, i in tickets.items():
+        r, U = b.result(t)
+        ra, Ua = alone[i]
+        assert r.status == 0 and r.iter == items[i]["cfg"].max_iter
+        assert nan_equal_bits(U, Ua) == 0, i
+        assert r.dt == ra.dt and r.now_t == ra.now_t
+    b.close()
+
+
 def test_batch_equals_single_solve_path():
     import ibtfo_amd as ib
     st, items = _items("FP64", n=3)
