@@ -523,14 +523,15 @@ def pmc_summary(args):
     rows = " --rows %d" % args.rows if args.rows else ""
     out["commands"] = [
         "cd /tmp && export TMPDIR=/tmp",
-        "timeout -s KILL 120 rocprofv3 --pmc FETCH_SIZE -d $REPO/gpurun_out/pmc_fetch -o fetch --output-format csv "
+        "timeout -s KILL 120 rocprofv3 --pmc FETCH_SIZE -d $OUT/pmc_fetch -o fetch --output-format csv "
         "-- python3 $REPO/bench.py --pmc-probe --steps 5" + rows,
-        "timeout -s KILL 120 rocprofv3 --pmc WRITE_SIZE -d $REPO/gpurun_out/pmc_write -o write --output-format csv "
+        "timeout -s KILL 120 rocprofv3 --pmc WRITE_SIZE -d $OUT/pmc_write -o write --output-format csv "
         "-- python3 $REPO/bench.py --pmc-probe --steps 5" + rows,
-        "python3 bench.py --pmc-fetch gpurun_out/pmc_fetch --pmc-write gpurun_out/pmc_write --pmc-out "
+        "python3 bench.py --pmc-fetch $OUT/pmc_fetch --pmc-write $OUT/pmc_write --pmc-out "
         "profiles/%s%s" % (TRAFFIC_FILE, rows)]
     out["note"] = ("FETCH_SIZE / WRITE_SIZE are KiB per dispatch (mean over the step-kernel dispatches); each is "
-                   "scaled by copy_bytes / (1024 * the same counter for a 256 MiB elementwise copy)")
+                   "scaled by copy_bytes / (1024 * the same counter for a 256 MiB elementwise copy); $REPO is the "
+                   "repository root, $OUT the folder the counter runs write to")
     print(json.dumps(out, indent=1))
     if args.pmc_out:
         with open(args.pmc_out, "w") as f:

@@ -12,7 +12,8 @@ oracle/ref_simplebound.cpp adds the current signatures, forwarding to them.
 Compiler: g++ -O2 -fopenacc (OpenACC host fallback, sequential) with
 -ffp-contract=off, so the fixture arithmetic is plain IEEE double in source order.
 Nothing is copied out of /root/reference; when it is absent this script exits
-without building (the GPU box only uses prebuilt files).
+without building (a machine without the reference sources uses the libraries
+built where they were present).
 
 Usage: python3 oracle/build_ref.py [-j N] [--list]
 """
