@@ -725,6 +725,27 @@ extern "C" int ibtfo_initialize(ibtfo_ctx* x, int kind, const double* prm, int n
     if (!x) return IBTFO_E_ARG;
     if (!x->have_geometry) { set_error("ibtfo_initialize needs ibtfo_geometry first"); return IBTFO_E_STATE; }
     const Layout& L = x->L;
+    if (kind == IBTFO_INIT_UNIFORM && prm && nprm >= 4) {
+        // on the device, stream-ordered: the same values as the host initialiser
+        // and upload (the conserved state computed here in the same order)
+        const double gam = x->cfg.gamma, rho = prm[0], u = prm[1], v = prm[2], p = prm[3];
+        const double U[4] = {rho, rho * u, rho * v, p / (gam - 1) + 0.5 * rho * (u * u + v * v)};
+        if (x->cfg.boundary == IBTFO_SIMPLE_BOUND && !x->unew0 &&
+            dalloc(x, &x->unew0, 4 * L.plane * sizeof(double)) != hipSuccess) {
+            set_error("ibtfo_initialize: out of device memory");
+            return IBTFO_E_NOMEM;
+        }
+        HIPCHK(launch_uniform_state(x->state[0], x->state[1],
+                                    x->cfg.boundary == IBTFO_SIMPLE_BOUND ? x->unew0 : nullptr, L.plane, U, x->ctl,
+                                    x->stream));
+        x->corner_host.assign(2 * 2 * L.ng * 4, 0.0);
+        for (size_t q = 0; q < x->corner_host.size(); q++) x->corner_host[q] = U[q % 4];
+        HIPCHK(hipMemcpyAsync(x->corner, x->corner_host.data(), x->corner_host.size() * sizeof(double),
+                              hipMemcpyHostToDevice, x->stream));
+        x->have_state = 1;
+        x->prepared = 0;
+        return IBTFO_OK;
+    }
     const size_t cells = (size_t)L.M * L.N;
     std::vector<double> xs(cells), ys(cells), U(cells * 4);
     std::vector<uint8_t> tp(L.plane), t(cells);

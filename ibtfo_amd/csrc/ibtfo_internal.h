@@ -203,6 +203,10 @@ hipError_t launch_edge(const EdgeArgs& a, hipStream_t s);
 hipError_t launch_fill_edge(const FillArgs& f, const EdgeArgs& e, hipStream_t s);  // fill phase 1 + ring
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s);
+// a uniform state (conserved U[0..3]) into every element of s0, s1 and s2 (if
+// not null); buffer 0 made current, the step flag cleared
+hipError_t launch_uniform_state(double* s0, double* s1, double* s2, long long plane, const double* U, Ctl* ctl,
+                                hipStream_t s);
 hipError_t launch_finish(const FinishArgs& f, hipStream_t s);
 // batched forms: one launch over many solves (maps: block -> {slot, first index})
 hipError_t launch_step_batch(int scheme, int rs, int lim, int prec, const StepArgs* slots, const int2* map,

@@ -1170,6 +1170,32 @@ hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_f
     hipLaunchKernelGGL(k_edge_ref, dim3((nring + 255) / 256), dim3(256), 0, s, a, corner, corner_from_input);
     return hipGetLastError();
 }
+// A uniform initial state written on the device into up to three state
+// buffers (every element of each of the 4 planes); block 0 also makes buffer 0
+// current and clears the step flag, as an upload of the state does.
+__global__ void k_uniform_state(double* s0, double* s1, double* s2, long long plane, double u0, double u1,
+                                double u2, double u3, Ctl* ctl) {
+    const long long n = 4 * plane;
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(q / plane);
+        const double v = k == 0 ? u0 : (k == 1 ? u1 : (k == 2 ? u2 : u3));
+        s0[q] = v;
+        s1[q] = v;
+        if (s2) s2[q] = v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->cur = 0;
+        ctl->ran = 0;
+    }
+}
+hipError_t launch_uniform_state(double* s0, double* s1, double* s2, long long plane, const double* U, Ctl* ctl,
+                                hipStream_t s) {
+    const long long n = 4 * plane;
+    const int blocks = (int)std::min<long long>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_uniform_state, dim3(blocks), dim3(256), 0, s, s0, s1, s2, plane, U[0], U[1], U[2], U[3], ctl);
+    return hipGetLastError();
+}
+
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s) {
     if (a.ngroups == 0) return hipSuccess;
     hipLaunchKernelGGL(k_merge, dim3(merge_blocks(a.ngroups)), dim3(MERGE_THREADS), 0, s, a);
