@@ -225,7 +225,8 @@ int ibtfo_device_state(ibtfo_ctx* ctx, double** planes4, int* pitch, int* offset
  *     solves (one launch per kernel of the step sequence covers every solve),
  *   - meanwhile admits queued requests whose device memory fits the remaining
  *     HBM budget (an arena reserved at creation: no hipMalloc per request) and
- *     builds their geometry and initial state on a second stream,
+ *     builds their geometry and initial state on a second stream, for as long
+ *     as the chunk runs (at least one; up to 8 when no solve is active),
  *   - then retires the solves that reached their step count, downloading the
  *     state of those that asked for it and returning their memory.
  * All solves of one batch share the kernel selection of ibtfo_batch_config;
