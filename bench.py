@@ -586,8 +586,8 @@ def main():
     if args.pmc_fetch and args.pmc_write:
         return pmc_summary(args)
     if args.gpus != 1:
-        print("bench.py: this build runs the whole domain on one GPU; --gpus %d is not supported "
-              "(multi-GPU slab decomposition is a design note only)" % args.gpus, file=sys.stderr)
+        print("bench.py: the solver keeps the whole domain on one GPU; --gpus %d is not supported "
+              "(a multi-GPU slab decomposition is described, not implemented)" % args.gpus, file=sys.stderr)
         return 2
     if args.pmc_probe:
         return pmc_probe(args)
