@@ -2,7 +2,8 @@
 //
 // A plain C++ restatement of the reference's per-timestep Euler finite-volume
 // update, used by tests/, by __graft_entry__.smoke() and by bench.py's CPU
-// baseline leg as the checker. Nothing in the product (ibtfo_amd/, include/)
+// baseline leg as the result to compare against. Nothing in the product
+// (ibtfo_amd/, include/)
 // links, loads or calls this file.
 //
 // Layout follows the reference exactly: AoS state `U[(M*j+i)*4+k]`, geometry

@@ -1,7 +1,8 @@
 """ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
 Imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as
-the checker. The product (ibtfo_amd/) never imports this module.
+the result to compare against. The product (ibtfo_amd/) never imports this
+module.
 
 `Case` describes one problem the way the reference's .inp file + Makefile slots
 do; `run_oracle` drives liboracle.so (the C++ restatement) in the order of
