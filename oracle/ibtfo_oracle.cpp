@@ -3,8 +3,7 @@
 // A plain C++ restatement of the reference's per-timestep Euler finite-volume
 // update, used by tests/, by __graft_entry__.smoke() and by bench.py's CPU
 // baseline leg as the result to compare against. Nothing in the product
-// (ibtfo_amd/, include/)
-// links, loads or calls this file.
+// (ibtfo_amd/, include/) links, loads or calls this file.
 //
 // Layout follows the reference exactly: AoS state `U[(M*j+i)*4+k]`, geometry
 // `XY[(M*j+i)*6+c]` with c = {x, y, phi, n_x, n_y, type}, a scratch record
