@@ -350,7 +350,7 @@ def test_png_file_equals_mask(oracle_mod, tmp_path):
     cfg = ib.config(N_x=96, N_y=48, hi_x=2.0, hi_y=1.0, Boundary="IDWBaseGFM", quirks=0, Edge="inflow")
     img = _mask_image(100, 52)
     p = tmp_path / "mask.png"
-    PIL.fromarray(img[..., :3], "RGB").save(p)
+    PIL.fromarray(np.ascontiguousarray(img[..., :3])).save(p)  # uint8 x 3 -> RGB
     a = ib.Solver(cfg)
     a.geometry_mask(img)
     b = ib.Solver(cfg)

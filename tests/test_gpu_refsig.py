@@ -51,7 +51,7 @@ def test_reference_signature_driver(oracle_mod, tmp_path, boundary, riemann):
     from ibtfo_amd import workloads as W
     assert os.path.exists(DRIVER), "tests/drivers/ref_sig_driver is not built (run __graft_entry__.build())"
     nx, ny, steps = 128, 64, 40
-    PIL.fromarray(W.bodies_mask(nx, ny), "RGBA").save(tmp_path / "Demo.png")
+    PIL.fromarray(W.bodies_mask(nx, ny)).save(tmp_path / "Demo.png")  # uint8 x 4 -> RGBA
     (tmp_path / "case.inp").write_text(INP.format(nx=nx, ny=ny, steps=steps, boundary=boundary, riemann=riemann))
     out = tmp_path / "out"
     out.mkdir()
