@@ -141,7 +141,10 @@ int ibtfo_geometry_png(ibtfo_ctx* ctx, const char* path);
 /* Decodes a PNG file into rgba (cap bytes; NULL to query the size only). */
 int ibtfo_png_decode(const char* path, unsigned char* rgba, long long cap, int* width, int* height);
 
-/* Initialize hook; evaluated on the host, uploaded as both U_OLD and U_NEW. */
+/* Initialize hook (Test/<Case>/Initialize.cpp): the state becomes both U_OLD and
+ * U_NEW. IBTFO_INIT_UNIFORM is written on the device, ordered on the solve's
+ * stream (no synchronisation); the other kinds are evaluated on the host and
+ * uploaded. */
 int ibtfo_initialize(ibtfo_ctx* ctx, int init_kind, const double* prm, int nprm);
 /* Boundary once (the drivers' call before the loop for GFM variants,
  * Test/InclineSod/InclineSod.cpp:81), then ComputeDt; resets time/iteration. */
