@@ -160,6 +160,19 @@ static int ring_blocks(const ibtfo_ctx* x) {
     return (nring + EDGE_THREADS - 1) / EDGE_THREADS;
 }
 
+// The same counts from a request's configuration, before its context exists
+// (the ghost count needs the geometry and is checked at admission).
+static int cfg_step_blocks(const ibtfo_config& c, int rows) {
+    const int nbx = (c.N_x + STRIP_W - 1) / STRIP_W;
+    const int r = std::max(1, std::min(rows, c.N_y));
+    return nbx * ((c.N_y + r - 1) / r);
+}
+static int cfg_ring_blocks(const ibtfo_config& c) {
+    if (c.edge == IBTFO_EDGE_REFERENCE) return 0;
+    const long long M = c.N_x + 2 * c.num_ghost_cell, N = c.N_y + 2 * c.num_ghost_cell;
+    return (int)((M * N - (long long)c.N_x * c.N_y + EDGE_THREADS - 1) / EDGE_THREADS);
+}
+
 extern "C" int ibtfo_batch_config_default(ibtfo_batch_config* bc) {
     if (!bc) return IBTFO_E_ARG;
     memset(bc, 0, sizeof *bc);
@@ -257,9 +270,20 @@ extern "C" int ibtfo_batch_submit(ibtfo_batch* b, const ibtfo_request* r, long l
         set_error("ibtfo_batch_submit: the request's slot selection differs from the batch's");
         return IBTFO_E_ARG;
     }
-    if (ibtfo_footprint(&c) > (long long)b->arena.capacity()) {
-        set_error("ibtfo_batch_submit: the solve needs %lld bytes, more than the batch's budget",
-                  ibtfo_footprint(&c));
+    // a per-cell level set is staged in device memory during the geometry
+    const long long need = ibtfo_footprint(&c) +
+        (r->ls_kind == IBTFO_LS_FIELD
+             ? (long long)(c.N_x + 2 * c.num_ghost_cell) * (c.N_y + 2 * c.num_ghost_cell) * (long long)sizeof(double)
+             : 0);
+    if (need > (long long)b->arena.capacity()) {
+        set_error("ibtfo_batch_submit: the solve needs %lld bytes, more than the batch's budget", need);
+        return IBTFO_E_NOMEM;
+    }
+    if (c.N_x > 0 && c.N_y > 0 &&
+        (cfg_step_blocks(c, k.rows) > k.max_step_blocks || cfg_ring_blocks(c) > k.max_ring_blocks)) {
+        set_error("ibtfo_batch_submit: the solve needs %d step blocks and %d ring blocks, more than the batch's "
+                  "capacity (%d, %d)", cfg_step_blocks(c, k.rows), cfg_ring_blocks(c), k.max_step_blocks,
+                  k.max_ring_blocks);
         return IBTFO_E_NOMEM;
     }
     if ((r->ls_nprm > 0 && !r->ls_prm) || (r->init_nprm > 0 && !r->init_prm) ||
@@ -283,6 +307,19 @@ extern "C" int ibtfo_batch_submit(ibtfo_batch* b, const ibtfo_request* r, long l
     b->queue.push_back(std::move(q));
     if (ticket) *ticket = b->queue.back().ticket;
     return IBTFO_OK;
+}
+
+// Retires the queued request at the head with status rc (it never ran).
+static void retire_head(ibtfo_batch* b, int rc) {
+    Req& q = b->queue.front();
+    Done d{};
+    d.r.status = rc;
+    d.r.t_arrival = q.arrival;
+    d.r.t_admit = d.r.t_done = b->now();
+    b->done[q.ticket] = std::move(d);
+    b->ready.push_back(q.ticket);
+    b->finished++;
+    b->queue.pop_front();
 }
 
 // Builds the queued request at the head into a context on the admission stream.
@@ -317,14 +354,7 @@ static int admit_one(ibtfo_batch* b, int reserved_step, int reserved_ghost, int 
         return 1;
     }
     if (x) free_ctx(x);
-    Done d{};
-    d.r.status = rc;
-    d.r.t_arrival = q.arrival;
-    d.r.t_admit = d.r.t_done = b->now();
-    b->done[q.ticket] = std::move(d);
-    b->ready.push_back(q.ticket);
-    b->finished++;
-    b->queue.pop_front();
+    retire_head(b, rc);
     return -1;
 }
 
@@ -452,6 +482,7 @@ extern "C" int ibtfo_batch_progress(ibtfo_batch* b, int* active, int* queued, in
     int rc;
     int nact = 0;
     for (auto& s : b->slots) nact += s.ctx != nullptr;
+    if (nact == 0) b->used_step = b->used_ghost = b->used_ring = 0;  // tables not yet rebuilt after the last retirements
     bool launched = false;
     if (nact > 0) {
         if (b->dirty && (rc = rebuild_tables(b))) return rc;
@@ -473,6 +504,14 @@ extern "C" int ibtfo_batch_progress(ibtfo_batch* b, int* active, int* queued, in
         int rs = 0, rg = 0, rr = 0;
         for (auto& p : b->pending) { rs += step_blocks(p.first); rg += ghost_blocks(p.first); rr += ring_blocks(p.first); }
         int a = admit_one(b, rs, rg, rr);
+        if (a == 0 && nact == 0 && b->pending.empty()) {
+            // the batch is empty and the request still does not fit (its ghost
+            // list exceeds the ghost-block capacity, or the arena): it never will
+            set_error("ibtfo_batch_progress: request %lld does not fit an empty batch", b->queue.front().ticket);
+            retire_head(b, IBTFO_E_NOMEM);
+            tried++;
+            continue;
+        }
         if (a == 0) break;
         if (a == 1) free_slots--;
         tried++;

@@ -47,6 +47,18 @@ template <class P> static hipError_t dalloc(ibtfo_ctx* x, P** p, size_t bytes) {
     }
     return hipMalloc((void**)p, bytes);
 }
+// A captured step graph holds the kernel arguments of its capture (buffer
+// pointers, ghost / group counts, grid shape): anything that changes them
+// drops the graph, and the next ibtfo_step_graph captures a new one.
+static void drop_graph(ibtfo_ctx* x) {
+    if (x->graph) {
+        (void)hipStreamSynchronize(x->stream);  // no replay of it still in flight
+        (void)hipGraphExecDestroy(x->graph);
+    }
+    x->graph = nullptr;
+    x->graph_chunk = 0;
+}
+
 static void dfree(ibtfo_ctx* x, void* p) {
     if (!p) return;
     if (x->arena) x->arena->free(p);
@@ -214,7 +226,7 @@ extern "C" int ibtfo_set_stream(ibtfo_ctx* x, void* s) {
     if (!x) return IBTFO_E_ARG;
     HIPCHK(hipStreamSynchronize(x->stream));  // work queued on the old stream ends before the new one starts
     x->stream = s ? (hipStream_t)s : x->own_stream;
-    if (x->graph) { (void)hipGraphExecDestroy(x->graph); x->graph = nullptr; x->graph_chunk = 0; }
+    drop_graph(x);
     return IBTFO_OK;
 }
 
@@ -385,6 +397,10 @@ static hipError_t launch_fill_levels(ibtfo_ctx* x, FillArgs f) {
 }
 
 static int geometry_finish(ibtfo_ctx* x, GeomArgs& g, double* dphi) {
+    // the ghost list, cut-cell records and merge groups are rebuilt below (new
+    // buffers and counts): a graph captured over the old ones must not replay
+    drop_graph(x);
+    x->prepared = 0;
     hipError_t ge = geom_run(g, x->stream);
     if (ge != hipSuccess) { dfree(x, dphi); HIPCHK(ge); }
     if (x->cfg.boundary == IBTFO_CUTCELL) {
@@ -949,6 +965,19 @@ extern "C" int ibtfo_step(ibtfo_ctx* x, int nsteps) {
     return IBTFO_OK;
 }
 
+extern "C" int ibtfo_step_dt(ibtfo_ctx* x, double dt) {
+    if (!x) return IBTFO_E_ARG;
+    if (!(dt > 0.0) || !std::isfinite(dt)) { set_error("ibtfo_step_dt: dt = %g", dt); return IBTFO_E_ARG; }
+    if (!x->prepared) { set_error("ibtfo_step_dt before ibtfo_prepare"); return IBTFO_E_STATE; }
+    Ctl c;
+    int rc = read_ctl_head(x, c);
+    if (rc) return rc;
+    c.dt = dt;   // the caller's dt for this step
+    c.done = 0;  // the caller keeps the clock and decides when to stop
+    if ((rc = write_ctl_head(x, c))) return rc;
+    return enqueue_step(x);
+}
+
 extern "C" int ibtfo_step_graph(ibtfo_ctx* x, int nsteps, int chunk) {
     if (!x || nsteps < 0 || chunk <= 0) return IBTFO_E_ARG;
     if (!x->prepared) { set_error("ibtfo_step_graph before ibtfo_prepare"); return IBTFO_E_STATE; }
@@ -1081,7 +1110,7 @@ extern "C" int ibtfo_set_rows(ibtfo_ctx* x, int rows) {
         x->mass_cap = nbx * nby;
     }
     x->rows_override = rows;
-    if (x->graph) { (void)hipGraphExecDestroy(x->graph); x->graph = nullptr; x->graph_chunk = 0; }
+    drop_graph(x);
     build_args(x);
     return IBTFO_OK;
 }

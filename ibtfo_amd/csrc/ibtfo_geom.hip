@@ -115,7 +115,7 @@ __global__ void k_geom_classify(GeomArgs g) {
     g.type[o] = t;
 }
 
-// Mirror_IDX.cpp:176-206: the ii in [0, M-1) with x_ii <= xm < x_{ii+1}; the same
+// Mirror_IDX.cpp:21-40: the ii in [0, M-1) with x_ii <= xm < x_{ii+1}; the same
 // half-open test on the candidates around the closed-form estimate. No match
 // leaves 0 (the value a zero-initialised record keeps in the reference).
 __device__ int donor_idx(double v, double base, double h, int n) {
@@ -128,7 +128,7 @@ __device__ int donor_idx(double v, double base, double h, int n) {
     return r;
 }
 
-// Scheme_Index for one ghost: IDWBaseGFM/GFM_Method.cpp:126-208,
+// Scheme_Index for one ghost: IDWBaseGFM/GFM_Method.cpp:20-110,
 // LinearBaseGFM/GFM_Method.cpp:19-44 + GFM_Bound.cpp:93-575 (donor choice),
 // BCGFM/BCGFM_Method.cpp:49-205, HBCGFM/HBCGFM_Method.cpp:56-91.
 __global__ void k_geom_ghosts(GeomArgs g) {

@@ -35,7 +35,7 @@ int host_initialize(int kind, const double* prm, int nprm, int nx, int ny, int n
                     to_cons(gam, s[0], s[1], s[2], s[3], u);
                     break;
                 }
-                case IBTFO_INIT_INCLINE: {  // Test/InclineSod/Initialize.cpp:96-126
+                case IBTFO_INIT_INCLINE: {  // Test/InclineSod/Initialize.cpp:38-72
                     double angle = 3.1415926535 * prm[0] / 180;
                     if (t == 0 || t == 2) {
                         const double* s =

@@ -5,8 +5,9 @@
 // library's message and aborts, since these signatures have no error channel.
 //
 // Mapping (reference header:line -> here):
-//   TimeAdvance   Src/Base/TimeAdvance/TimeAdvance.H:13-26  -> ibtfo_step(ctx, 1): the
-//                 fused step also applies Boundary, Global_Boundary and ComputeDt
+//   TimeAdvance   Src/Base/TimeAdvance/TimeAdvance.H:13-26  -> ibtfo_step_dt(ctx, dt): one
+//                 step with the driver's dt; the fused step also applies Boundary,
+//                 Global_Boundary and ComputeDt
 //   Boundary      Src/BoundaryCondition/Boundary.H:11-20    -> before the first step:
 //                 ibtfo_boundary; after a TimeAdvance: already applied
 //   Global_Boundary Src/BoundaryCondition/Boundary.H:22-29  -> applied by the step
@@ -171,12 +172,14 @@ static void ensure_prepared(Bound* b) {
     check(ibtfo_prepare(b->ctx, 0), "prepare");
 }
 
-void TimeAdvance(const double, const double, const int, const int, const int, const double, double, double* U_OLD,
-                 double*, double*, double*, double*, const int ndevices, const int device) {
+void TimeAdvance(const double, const double, const int, const int, const int, const double, double dt,
+                 double* U_OLD, double*, double*, double*, double*, const int ndevices, const int device) {
     one_device(ndevices, device, "TimeAdvance");
     Bound* b = find(U_OLD, "TimeAdvance");
     ensure_prepared(b);
-    check(ibtfo_step(b->ctx, 1), "TimeAdvance");
+    // SingleStep.cpp:5-20 advances by the dt it is given (a driver may have
+    // changed the one ComputeDt returned, e.g. clipped it to Psy_time)
+    check(ibtfo_step_dt(b->ctx, dt), "TimeAdvance");
     b->stepped = true;
 }
 

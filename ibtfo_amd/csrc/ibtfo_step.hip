@@ -40,7 +40,7 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// ComputeDt.cpp:55-63 contribution of one fluid cell (std::max keeps the old
+// ComputeDt.cpp:34-46 contribution of one fluid cell (std::max keeps the old
 // maximum when the candidate is NaN); amax accumulates the squared sound speed.
 __device__ __forceinline__ void dt_terms(double gam, double gm1, const double* U, double& amax, double& umax,
                                          double& vmax, int& bad) {
@@ -97,7 +97,7 @@ __device__ __forceinline__ void block_partials(double* part, int bid, double ama
     }
 }
 
-// ComputeDt (ComputeDt.cpp:46-67) and CaculateMass (CaculateMass.cpp:27-38) from
+// ComputeDt (ComputeDt.cpp:23-49) and CaculateMass (CaculateMass.cpp:27-38) from
 // the block partials, then the host loop's bookkeeping of Test/Sod/Sod.cpp:154-177:
 // dt for the next step, now_t += dt, iter++, stop test, log entry. `advance` = 0
 // for the stand-alone ComputeDt (no step taken; `reset` starts the run clock).
@@ -716,7 +716,7 @@ __global__ __launch_bounds__(64) void k_step_lax(StepArgs a) {
     block_partials(a.mass_part, blockIdx.y * a.nbx + blockIdx.x, amax, umax, vmax, msum, bad);
 }
 
-// ComputeDt + CaculateMass of the current state (ComputeDt.cpp:46-67,
+// ComputeDt + CaculateMass of the current state (ComputeDt.cpp:23-49,
 // CaculateMass.cpp:27-38); with `reset`, also starts the run clock.
 __global__ __launch_bounds__(64) void k_reduce(ReduceArgs a) {
     const Layout L = a.L;

@@ -82,7 +82,7 @@ enum ibtfo_init_kind {
     IBTFO_INIT_SOD_X = 3,     /* Sod split at x = prm[0], prm[1..8] = L, R states */
     IBTFO_INIT_UNIFORM = 4,   /* prm {rho,u,v,p} */
     IBTFO_INIT_SHU_VORTEX = 5, /* isentropic vortex, prm {x0, y0, beta, rho_inf, u_inf, v_inf, p_inf} */
-    IBTFO_INIT_SHOCK = 6       /* Test/PNGGeo/Initialize.cpp:14-59: prm {x_s, Ms, rho, u, v, p}; that state
+    IBTFO_INIT_SHOCK = 6       /* Test/PNGGeo/Initialize.cpp:14-54: prm {x_s, Ms, rho, u, v, p}; that state
                                   where x > x_s, its Mach-Ms normal-shock image behind (the hook's
                                   x_s = 0.1, Ms = 1.3) */
 };
@@ -182,7 +182,13 @@ int ibtfo_vtk_write_host(const ibtfo_config* cfg, int iter, double now_t, const 
  * iter++ -- stopping, on the device, when now_t >= Psy_time or iter >= max_iter.
  * Asynchronous on the context stream; dt never leaves the device. */
 int ibtfo_step(ibtfo_ctx* ctx, int nsteps);
-/* Same, replayed from a captured HIP graph of `chunk` steps (captured on first use). */
+/* One step with the caller's dt: TimeAdvance's dt argument (Src/Base/TimeAdvance/TimeAdvance.H:13-26,
+ * passed on to Advance by SingleStep.cpp:5-20) followed by Boundary, Global_Boundary and ComputeDt,
+ * whatever the device clock's stop test says (the caller keeps its own clock, e.g. a driver that
+ * clips the last step to Psy_time). Synchronises before queueing the step. */
+int ibtfo_step_dt(ibtfo_ctx* ctx, double dt);
+/* Same as ibtfo_step, replayed from a captured HIP graph of `chunk` steps (captured on first use;
+ * a new geometry, stream or tile shape drops the captured graph). */
 int ibtfo_step_graph(ibtfo_ctx* ctx, int nsteps, int chunk);
 /* Synchronises; iteration count, physical time and the current dt. */
 int ibtfo_status(ibtfo_ctx* ctx, int* iter, double* now_t, double* dt);

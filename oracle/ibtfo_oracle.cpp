@@ -44,7 +44,7 @@ enum { BC_REGULAR = 0, BC_IDW = 1, BC_LINEAR = 2, BC_RIEMANN = 3, BC_BCGFM = 4, 
 enum { EDGE_REFERENCE = 0, EDGE_PERIODIC = 1, EDGE_INFLOW = 2, EDGE_PNGGEO = 3 };
 
 // Sound speed. The reference computes std::pow(x, 0.5) (HLL/Riemann_solver.cpp:144,
-// ComputeDt.cpp:60); glibc's pow is not always the correctly rounded root. Mode 1
+// ComputeDt.cpp:42); glibc's pow is not always the correctly rounded root. Mode 1
 // swaps in sqrt so a run can be compared bit-for-bit with an IEEE sqrt device path.
 int g_root_mode = 0;
 inline double root(double x) { return g_root_mode ? std::sqrt(x) : std::pow(x, 0.5); }
@@ -635,7 +635,7 @@ void or_advance(int scheme, int rs, int lim, int quirks, double L, double H, int
     advance(c, scheme, rs, lim, dt, U_OLD, U_NEW, XY);
 }
 
-// ComputeDt.cpp:28-67
+// ComputeDt.cpp:23-49
 double or_compute_dt(double L, double H, int nx, int ny, int ng, double gam, double cfl,
                      const double* U, const double* XY) {
     Grid g(nx, ny, ng);
@@ -952,7 +952,7 @@ static void mirror_idx(const Grid& g, double* S, int W, const double* XY, int k)
                     if (ym < XY[g.c(0, jj + 1) * 6 + 1] && ym >= XY[g.c(0, jj) * 6 + 1]) S[g.c(i, j) * W + k + 3] = jj;
             }
 }
-// inverse-distance weights over the 4 donors, IDWBaseGFM/GFM_Method.cpp:152-211;
+// inverse-distance weights over the 4 donors, IDWBaseGFM/GFM_Method.cpp:52-110;
 // `keep(type)` picks the donors that get a weight.
 template <class Keep>
 static void idw_weights(const Grid& g, double* S, int W, const double* XY, int kp, int ka, Keep keep) {
@@ -1280,7 +1280,7 @@ void or_global_boundary(int kind, const double* prm, int nx, int ny, int ng, dou
 // 3: Sod split at x = prm[0] with prm[1..8] = left/right primitive states (no reference hook);
 // 4: uniform state prm = {rho,u,v,p} (no reference hook);
 // 5: isentropic vortex, prm = {x0, y0, beta, rho_inf, u_inf, v_inf, p_inf} (no reference hook);
-// 6: Test/PNGGeo/Initialize.cpp:14-59, a shock of Mach prm[1] at x = prm[0]: the
+// 6: Test/PNGGeo/Initialize.cpp:14-54, a shock of Mach prm[1] at x = prm[0]: the
 //    state prm[2..5] = (rho, u, v, p) where x > prm[0], its normal-shock image
 //    behind (x <= prm[0]).
 // prm for kind 0: {rho_L,u_L,v_L,p_L, rho_R,u_R,v_R,p_R}.
@@ -1467,22 +1467,27 @@ void or_time_advance(int time, int scheme, int rs, int lim, int bc, int edge, co
 //   while (now_t < t_end && iter < max_iter): TimeAdvance(dt); Boundary; Global_Boundary;
 //       dt = ComputeDt; mass; now_t += dt; iter++
 // Returns the number of steps taken; per-step dt/time/mass go to the log arrays (may be null).
+// clip = 1: a driver of the same order that shortens the last step to end at t_end
+// (step_dt = min(dt, t_end - now_t)) and advances its clock by the dt it stepped
+// with; TimeAdvance takes that dt (Src/Base/TimeAdvance/SingleStep/SingleStep.cpp:5-20).
 int or_run(int time, int scheme, int rs, int lim, int bc, int edge, const double* edge_prm, int quirks, double L,
            double H, int nx, int ny, int ng, double gam, double cfl, double t_end, int max_iter, int init_boundary,
            double* U_OLD, double* U_NEW, const double* XY, const double* S, double* T, double* dt_log,
-           double* t_log, double* mass_log) {
+           double* t_log, double* mass_log, int clip) {
     if (init_boundary) or_boundary(bc, nx, ny, ng, gam, U_OLD, U_NEW, XY, S, quirks);
     double dt = or_compute_dt(L, H, nx, ny, ng, gam, cfl, U_OLD, XY);
     double now_t = 0;
     int iter = 0;
     while (now_t < t_end && iter < max_iter) {
-        or_time_advance(time, scheme, rs, lim, bc, edge, edge_prm, quirks, L, H, nx, ny, ng, gam, dt, U_OLD, T,
-                        U_NEW, XY, S);
+        double step_dt = dt;
+        if (clip && now_t + step_dt > t_end) step_dt = t_end - now_t;
+        or_time_advance(time, scheme, rs, lim, bc, edge, edge_prm, quirks, L, H, nx, ny, ng, gam, step_dt, U_OLD,
+                        T, U_NEW, XY, S);
         or_boundary(bc, nx, ny, ng, gam, U_OLD, U_NEW, XY, S, quirks);
         or_global_boundary(edge, edge_prm, nx, ny, ng, gam, U_OLD, U_NEW);
         dt = or_compute_dt(L, H, nx, ny, ng, gam, cfl, U_OLD, XY);
         double m = or_mass(L, H, nx, ny, ng, U_OLD, XY);
-        now_t = now_t + dt;
+        now_t = now_t + (clip ? step_dt : dt);
         if (dt_log) dt_log[iter] = dt;
         if (t_log) t_log[iter] = now_t;
         if (mass_log) mass_log[iter] = m;

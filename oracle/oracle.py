@@ -229,8 +229,9 @@ def mask_phi_cells(case, rgba, cells):
     return phi
 
 
-def run_oracle(case, nsteps, root_mode=0, state=None):
-    """Whole run like the reference driver, at most nsteps steps. Returns a dict."""
+def run_oracle(case, nsteps, root_mode=0, state=None, clip=False):
+    """Whole run like the reference driver, at most nsteps steps. Returns a dict.
+    clip: the driver shortens the last step to end at Psy_time (see or_run)."""
     L = lib()
     L.or_set_root_mode(int(root_mode))
     g = case
@@ -245,7 +246,7 @@ def run_oracle(case, nsteps, root_mode=0, state=None):
                  EDGES[g.edge], ptr(ep), g.quirks, C.c_double(g.L), C.c_double(g.H), g.N_x, g.N_y,
                  g.num_ghost_cell, C.c_double(g.gamma), C.c_double(g.CFL_number), C.c_double(g.Psy_time),
                  min(g.max_iter, nsteps), g.init_boundary(), ptr(st["U_OLD"]), ptr(st["U_NEW"]), ptr(st["XY"]),
-                 ptr(st["S"]), ptr(T), ptr(dt_log), ptr(t_log), ptr(m_log))
+                 ptr(st["S"]), ptr(T), ptr(dt_log), ptr(t_log), ptr(m_log), int(bool(clip)))
     L.or_set_root_mode(0)
     L.or_set_cut(None)
     st.update(steps=n, dt=dt_log[:n], t=t_log[:n], mass=m_log[:n])

@@ -84,15 +84,20 @@ int main(int argc, char** argv) {
     ibtfo_ref_update_device(U_OLD.data());
     Boundary(N_x, N_y, ng, gamma, U_OLD.data(), U_NEW.data(), XYCOORD.data(), SCHEME_IDX.data(), 1, 0);
     ComputeDt(Psy_L, Psy_H, N_x, N_y, ng, gamma, CFL_number, U_OLD.data(), XYCOORD.data(), &dt, 1, 0);
+    // clip_last_step = 1: a driver that shortens the last step to end exactly at
+    // Psy_time and advances its clock by the dt it stepped with
+    const int clip = (int)num(filename, "clip_last_step", 0);
     std::vector<double> log;
     while (now_t < Psy_time && iter < max_iter) {
-        TimeAdvance(Psy_L, Psy_H, N_x, N_y, ng, gamma, dt, U_OLD.data(), U_TMP.data(), U_NEW.data(),
+        double step_dt = dt;
+        if (clip && now_t + step_dt > Psy_time) step_dt = Psy_time - now_t;
+        TimeAdvance(Psy_L, Psy_H, N_x, N_y, ng, gamma, step_dt, U_OLD.data(), U_TMP.data(), U_NEW.data(),
                     XYCOORD.data(), SCHEME_IDX.data(), 1, 0);
         Boundary(N_x, N_y, ng, gamma, U_OLD.data(), U_NEW.data(), XYCOORD.data(), SCHEME_IDX.data(), 1, 0);
         Global_Boundary(N_x, N_y, ng, gamma, U_OLD.data(), U_NEW.data(), 1, 0);
         ComputeDt(Psy_L, Psy_H, N_x, N_y, ng, gamma, CFL_number, U_OLD.data(), XYCOORD.data(), &dt, 1, 0);
         CaculateMass(Psy_L, Psy_H, N_x, N_y, ng, U_OLD.data(), XYCOORD.data(), &mass, 1, 0);
-        now_t = now_t + dt;
+        now_t = now_t + (clip ? step_dt : dt);
         iter++;
         log.push_back(dt);
         log.push_back(now_t);

@@ -168,3 +168,39 @@ def test_refused_requests():
     for bad in ["CutCell", "SimpleBound"]:
         with pytest.raises(ib.IbtfoError, match="batch engine"):
             S.Batch(_bc("FP64", Boundary=bad))
+
+
+def test_request_over_the_grid_capacity_is_refused_or_retired():
+    """A request whose step strips exceed the batch's grid capacity is refused
+    at submit; one whose ghost list exceeds the ghost-block capacity (known only
+    after its geometry) is retired with IBTFO_E_NOMEM once the batch is empty,
+    and the requests behind it still run."""
+    import ibtfo_amd as ib
+    from ibtfo_amd import serving as S
+    st, items = _items("FP64", n=3, steps=8)
+    b = S.Batch(_bc("FP64", max_step_blocks=8))
+    with pytest.raises(ib.IbtfoError, match="step blocks"):
+        b.submit(**items[0])
+    b.close()
+    ghosts = []
+    for it in items:
+        s = ib.Solver(it["cfg"])
+        s.geometry(*it["levelset"])
+        ghosts.append(s.num_ghosts())
+        s.close()
+    # capacity for the smallest ghost list only
+    small = min(ghosts)
+    cap = (small + 127) // 128
+    big = [i for i, g in enumerate(ghosts) if (g + 127) // 128 > cap]
+    assert big, ghosts
+    b = S.Batch(_bc("FP64", max_ghost_blocks=cap))
+    tickets = {b.submit(**it): i for i, it in enumerate(items)}
+    b.drain(max_rounds=10000)
+    assert b.stats().finished == len(items)
+    for t, i in tickets.items():
+        r, _ = b.result(t, state=False)
+        if i in big:
+            assert r.status == ib.E_NOMEM and r.iter == 0
+        else:
+            assert r.status == 0 and r.iter == items[i]["cfg"].max_iter
+    b.close()

@@ -188,6 +188,41 @@ def test_graph_replay_matches_eager(oracle_mod):
     assert a["steps"] == b["steps"] == 40
 
 
+@pytest.mark.parametrize("bound", ["IDWBaseGFM", "CutCell"])
+def test_graph_dropped_when_geometry_changes(bound):
+    """A captured step graph holds the ghost list, cut-cell records and merge
+    groups of its capture. A new geometry (another body, more ghosts) must not
+    replay it: graph steps after geometry + prepare equal eager steps bit for bit."""
+    import ibtfo_amd as ib
+    cfg = ib.config(N_x=128, N_y=64, lo_x=0.0, lo_y=0.0, hi_x=2.0, hi_y=1.0, Psy_time=1e30, max_iter=10 ** 9,
+                    CFL_number=0.5, Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer", Boundary=bound,
+                    Edge="inflow", quirks=0, edge_state=(1.4, 3.0, 0.0, 1.0))
+
+    def run(chunk):
+        s = ib.Solver(cfg)
+        try:
+            s.geometry("ellipse", (0.5, 0.5, 0.1, 0.05, 20.0))
+            s.initialize("uniform", (1.4, 3.0, 0.0, 1.0))
+            s.prepare(True)
+            s.step(16, graph_chunk=chunk)
+            first = s.num_ghosts()
+            s.geometry("ellipse", (0.9, 0.45, 0.25, 0.12, 60.0))
+            assert s.num_ghosts() > first
+            s.initialize("uniform", (1.4, 3.0, 0.0, 1.0))
+            s.prepare(True)
+            s.step(16, graph_chunk=chunk)
+            it = s.status()[0]
+            return s.state(), it
+        finally:
+            s.close()
+
+    print("graph after a geometry change, %s" % bound, flush=True)
+    a, ia = run(8)
+    b, ib_ = run(0)
+    assert ia == ib_ == 16
+    assert nan_equal_bits(a, b) == 0
+
+
 def _bodies_case(O, nx=256, ny=128, **kw):
     from ibtfo_amd import workloads as W
     c = O.Case(N_x=nx, N_y=ny, lo_x=0.0, lo_y=0.0, hi_x=2.0, hi_y=1.0, Psy_time=1e30, max_iter=10 ** 9,
