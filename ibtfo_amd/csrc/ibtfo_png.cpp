@@ -143,7 +143,18 @@ int png_decode(const unsigned char* buf, size_t len, std::vector<unsigned char>&
                 if (k < trns.size()) a = trns[k];
             } else if (ctype == 2) {
                 r = sample(row, 3 * x); g = sample(row, 3 * x + 1); b = sample(row, 3 * x + 2);
-                if (trns.size() >= 6 && depth == 8 && r == trns[1] && g == trns[3] && b == trns[5]) a = 0;
+                if (trns.size() >= 6) {
+                    // the transparent colour is matched on the full samples, before
+                    // 16-bit samples are cut to 8 (as libpng expands tRNS first)
+                    bool eq = true;
+                    for (int q = 0; q < 3; q++) {
+                        unsigned t = (unsigned)trns[2 * q] << 8 | trns[2 * q + 1];
+                        unsigned full = depth == 16 ? ((unsigned)row[6 * x + 2 * q] << 8 | row[6 * x + 2 * q + 1])
+                                                    : row[3 * x + q];
+                        eq = eq && full == t;
+                    }
+                    if (eq) a = 0;
+                }
             } else if (ctype == 4) {
                 r = g = b = sample(row, 2 * x);
                 a = sample(row, 2 * x + 1);

@@ -40,8 +40,8 @@ CASE_HOOKS = {
     "vortex": [TEST + "/Vortex/Initialize.cpp", TEST + "/Vortex/Level_Set_function.cpp",
                TEST + "/Vortex/Global_Boudary.cpp"],
     # Test/PNGGeo's initialiser and edge hook (single-device signatures, adapted by
-    # oracle/ref_pnggeo.cpp) over InclineSod's level set: its own Level_Set needs
-    # libpng, which the image lacks
+    # oracle/ref_pnggeo.cpp) over InclineSod's level set (its own mask level set is
+    # built on its own: libref_pngmask.so)
     "pngshock": [TEST + "/PNGGeo/Initialize.cpp", TEST + "/PNGGeo/Global_Boudary.cpp",
                  TEST + "/InclineSod/Level_Set_function.cpp", os.path.join(HERE, "ref_pnggeo.cpp")],
 }
@@ -66,6 +66,12 @@ def combos():
     for b in ["IDWBaseGFM", "LinearBaseGFM", "RiemannBaseGFM", "BCGFM", "HBCGFM"]:
         out.append(("incline", "MUSCL", "HLL", "minmod", b))
     out.append(("incline", "MHM", "HLL", "minmod", "IDWBaseGFM"))
+    # PVRS in 2-D: the inclined channel, and the row-split hook with a
+    # downward-moving upper state, whose collision takes the y solver's
+    # branch with the stray store (PVRS/Riemann_solver.cpp:541-551)
+    for s in ["MUSCL", "MHM"]:
+        out.append(("incline", s, "PVRS", "minmod", "IDWBaseGFM"))
+        out.append(("sod", s, "PVRS", "minmod", "Regular"))
     out.append(("incline", "MUSCL", "StegerWarming", "minmod", "IDWBaseGFM"))
     out.append(("vortex", "MUSCL", "HLL", "minmod", "IDWBaseGFM"))
     out.append(("incline", "MUSCL", "HLL", "minmod", "SimpleBound"))
@@ -160,6 +166,51 @@ def build_writedata():
     return out, "built"
 
 
+# libpng 1.6.37 (the version the reference's README names): the system's
+# runtime library, with the headers of the same release that conda ships (the
+# system has no development headers). Linked by path, so nothing else under
+# /opt/conda (its older libstdc++) enters the link.
+PNG_INC = "/opt/conda/include"
+PNG_SO = "/usr/lib/x86_64-linux-gnu/libpng16.so.16"
+
+
+def build_pngmask():
+    """Test/PNGGeo/Level_Set.cpp (the mask level set) with the reference's PNG
+    reader (Src/Utils/PngProcess.cpp), ParamReader and Psy_coord, and
+    oracle/ref_pngmask.cpp's C entry points, linked against libpng.
+
+    Level_Set.cpp is compiled without -fopenacc: under OpenACC's default data
+    rules a scalar referenced in a parallel construct is firstprivate, so g++
+    keeps the host's edge count (`edg_num++` under `acc atomic update`,
+    Level_Set.cpp:53-77) at 0 and the edge list new double[0] is then overrun
+    (:79-95; AddressSanitizer finds the write at :90). Read as serial C++ the
+    loops count the edges, which is what the code evidently intends. Its
+    scratch arrays are zeroed (operator new[] wrapped to calloc, see
+    ref_pngmask.cpp), so the unset border gradients read as no edge."""
+    out = os.path.join(OUT, "libref_pngmask.so")
+    ls_src = TEST + "/PNGGeo/Level_Set.cpp"
+    srcs = [SRC + "/Utils/PngProcess.cpp", SRC + "/Utils/ParamReader.cpp", SRC + "/Geometry/Psy_coord.cpp",
+            os.path.join(HERE, "ref_pngmask.cpp")]
+    if not os.path.exists(os.path.join(PNG_INC, "png.h")) or not os.path.exists(PNG_SO):
+        return out, "FAILED\nlibpng 1.6 (headers under %s, %s) not found" % (PNG_INC, PNG_SO)
+    if os.path.exists(out) and os.path.getmtime(out) >= max(os.path.getmtime(s) for s in srcs + [ls_src]):
+        return out, "up-to-date"
+    inc = ["-I" + d for d in [SRC + "/Geometry", SRC + "/Utils", SRC + "/Base/Euler/Advection/MUSCL", PNG_INC]]
+    obj = out + ".Level_Set.o"
+    r = subprocess.run(["g++", "-O2", "-ffp-contract=off", "-fPIC", "-c", "-w", "-o", obj] + inc + [ls_src],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        return out, "FAILED\n" + r.stderr[-3000:]
+    cmd = ["g++", "-O2", "-fopenacc", "-ffp-contract=off", "-fPIC", "-shared", "-w", "-o", out + ".tmp"] + inc + \
+        srcs + [obj, PNG_SO, "-Wl,--wrap=_Znam", "-Wl,--wrap=_ZdaPv"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    os.remove(obj)
+    if r.returncode != 0:
+        return out, "FAILED\n" + r.stderr[-3000:]
+    os.replace(out + ".tmp", out)
+    return out, "built"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=min(8, os.cpu_count() or 1))
@@ -179,11 +230,12 @@ def main():
             if msg.startswith("FAILED"):
                 bad += 1
                 print(os.path.basename(out), msg)
-    out, msg = build_writedata()
-    if msg.startswith("FAILED"):
-        bad += 1
-        print(os.path.basename(out), msg)
-    print("build_ref: %d libraries, %d failed" % (len(combos()) + 1, bad))
+    for fn in (build_writedata, build_pngmask):
+        out, msg = fn()
+        if msg.startswith("FAILED"):
+            bad += 1
+            print(os.path.basename(out), msg)
+    print("build_ref: %d libraries, %d failed" % (len(combos()) + 2, bad))
     return 1 if bad else 0
 
 

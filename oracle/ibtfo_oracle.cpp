@@ -277,8 +277,15 @@ void riemann_pvrs_x(Ctx& c) {
             F[0] = o.f[0]; F[1] = o.f[1]; F[2] = o.f[2]; F[3] = o.f[3];
         }
 }
+// 1: the reference's stray store of PVRS/Riemann_solver.cpp:550 (reference mode);
+// 0: dropped -- the definition the GPU path implements (see riemann_pvrs_y)
+int g_pvrs_stray = 1;
+long long g_stray_faces = 0, g_stray_gold = 0;  // faces taking the branch; of them, stores landing in a G_OLD slot
 void riemann_pvrs_y(Ctx& c) {
-    // sequential: the reference's stray store (:550) lands in another cell's record
+    // sequential: the reference's stray store (:550) writes the energy flux of
+    // face record (i, j) through the state index macro, Index(i, j, 3) = 4(Mj+i)+3,
+    // into the scratch record of cell (Mj+i)/7, slot (Mj+i) % 7; the face's own
+    // energy slot keeps the aliased x flux. With g_pvrs_stray = 0 only the latter.
     for (int j = 0; j < c.g.N; j++)
         for (int i = 0; i < c.g.M; i++) {
             const double *UD = c.slot(i, j, S_UD), *UU = c.slot(i, j, S_UU);
@@ -287,8 +294,13 @@ void riemann_pvrs_y(Ctx& c) {
             double* G = c.slot(i, j, S_GOLD);
             // y flux order: mass, x-momentum (tangential), y-momentum (normal), energy
             G[0] = o.f[0]; G[1] = o.f[2]; G[2] = o.f[1];
-            if (o.stray && c.quirks) c.T[c.g.c(i, j) * 4 + 3] = o.f[3];
-            else G[3] = o.f[3];
+            if (o.stray && c.quirks) {
+                g_stray_faces++;
+                g_stray_gold += (c.g.c(i, j) * 4 + 3) % c.W / 4 == S_GOLD;
+                if (g_pvrs_stray) c.T[c.g.c(i, j) * 4 + 3] = o.f[3];
+            } else {
+                G[3] = o.f[3];
+            }
         }
 }
 
@@ -578,6 +590,12 @@ inline void prim_to_cons(double gam, double rho, double u, double v, double p, d
 extern "C" {
 
 void or_set_root_mode(int mode) { g_root_mode = mode; }
+void or_set_pvrs_stray(int on) { g_pvrs_stray = on; }
+// counts of the PVRS y solver's stray-store branch since the last call (reset on read)
+void or_pvrs_stray_counts(long long* faces, long long* gold) {
+    *faces = g_stray_faces; *gold = g_stray_gold;
+    g_stray_faces = g_stray_gold = 0;
+}
 int or_num_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();
@@ -625,6 +643,14 @@ void or_face_flux(int rs, int ydir, double gam, const double* Ulo, const double*
             else out[k] = (SR * Flo[k] - SL * Fhi[k] + SR * SL * (Uhi[k] - Ulo[k])) / (SR - SL);
         }
     }
+}
+
+// The HLL wave-speed estimates of one face (S[0] = S_lo, S[1] = S_hi), in the
+// reference form (quirks = 1) or the corrected one (quirks = 0), for unit tests.
+void or_hll_speeds(int quirks, int ydir, double gam, const double* Ulo, const double* Uhi, double* S) {
+    Prim a = ydir ? prim_y(gam, Ulo) : prim_x(gam, Ulo);
+    Prim b = ydir ? prim_y(gam, Uhi) : prim_x(gam, Uhi);
+    hll_speeds(gam, a, b, S[0], S[1], quirks);
 }
 
 // Advance (one scheme pass) -- *_Scheme.cpp. T must hold or_tmp_width(scheme)

@@ -229,11 +229,14 @@ def mask_phi_cells(case, rgba, cells):
     return phi
 
 
-def run_oracle(case, nsteps, root_mode=0, state=None, clip=False):
+def run_oracle(case, nsteps, root_mode=0, state=None, clip=False, pvrs_stray=True):
     """Whole run like the reference driver, at most nsteps steps. Returns a dict.
-    clip: the driver shortens the last step to end at Psy_time (see or_run)."""
+    clip: the driver shortens the last step to end at Psy_time (see or_run).
+    pvrs_stray: reference mode keeps the PVRS y solver's stray store
+    (PVRS/Riemann_solver.cpp:550); False drops it, as the GPU path does."""
     L = lib()
     L.or_set_root_mode(int(root_mode))
+    L.or_set_pvrs_stray(int(bool(pvrs_stray)))
     g = case
     st = setup(g) if state is None else state
     T = np.zeros(g.M * g.N * L.or_tmp_width(SCHEMES[g.Scheme]))
@@ -248,6 +251,7 @@ def run_oracle(case, nsteps, root_mode=0, state=None, clip=False):
                  min(g.max_iter, nsteps), g.init_boundary(), ptr(st["U_OLD"]), ptr(st["U_NEW"]), ptr(st["XY"]),
                  ptr(st["S"]), ptr(T), ptr(dt_log), ptr(t_log), ptr(m_log), int(bool(clip)))
     L.or_set_root_mode(0)
+    L.or_set_pvrs_stray(1)
     L.or_set_cut(None)
     st.update(steps=n, dt=dt_log[:n], t=t_log[:n], mass=m_log[:n])
     return st
@@ -320,6 +324,49 @@ def ref_lib_writedata():
         L.ref_write_data.restype = None
         _ref_cache[name] = L
     return _ref_cache[name]
+
+
+def ref_lib_pngmask():
+    """The reference's PNG reader and mask level set (oracle/_ref/libref_pngmask.so)."""
+    name = "libref_pngmask.so"
+    if name not in _ref_cache:
+        L = C.CDLL(os.path.join(REF_DIR, name), mode=os.RTLD_LOCAL | os.RTLD_NOW)
+        L.ref_png_read.restype = C.c_int
+        L.ref_mask_level_set.restype = C.c_int
+        _ref_cache[name] = L
+    return _ref_cache[name]
+
+
+def ref_pngmask_available():
+    return os.path.exists(os.path.join(REF_DIR, "libref_pngmask.so"))
+
+
+def ref_png_read(path):
+    """Src/Utils/PngProcess.cpp:4-72 on a file: (H, W, 4) uint8 RGBA, W, H."""
+    L = ref_lib_pngmask()
+    W, H = C.c_int(), C.c_int()
+    cap = 1 << 26
+    buf = np.zeros(cap, dtype=np.uint8)
+    rc = L.ref_png_read(os.fsencode(path), buf.ctypes.data_as(C.c_void_p), C.c_longlong(cap), C.byref(W),
+                        C.byref(H))
+    assert rc == 0, rc
+    return buf[:W.value * H.value * 4].reshape(H.value, W.value, 4).copy(), W.value, H.value
+
+
+def ref_mask_level_set(directory, nx, ny, ng, extents):
+    """Psy_coord + Test/PNGGeo/Level_Set.cpp:11-364 over directory/Demo.png:
+    the reference's XYCOORD (M*N*6 doubles). extents = (lo_x, lo_y, hi_x, hi_y)."""
+    L = ref_lib_pngmask()
+    lo_x, lo_y, hi_x, hi_y = extents
+    inp = os.path.join(directory, "mask.inp")
+    with open(inp, "w") as f:
+        f.write("lo_x = %r\nlo_y = %r\nhi_x = %r\nhi_y = %r\nN_x = %d\nN_y = %d\nnum_ghost_cell = %d\n"
+                % (lo_x, lo_y, hi_x, hi_y, nx, ny, ng))
+    XY = np.zeros((nx + 2 * ng) * (ny + 2 * ng) * 6)
+    rc = L.ref_mask_level_set(os.fsencode(directory), os.fsencode(inp), nx, ny, ng, C.c_double(lo_x),
+                              C.c_double(lo_y), C.c_double(hi_x), C.c_double(hi_y), ptr(XY))
+    assert rc == 0, rc
+    return XY
 
 
 def run_ref(case, nsteps):

@@ -43,6 +43,12 @@ def run_cases():
                                            Riemann="StegerWarming", case="incline"), 10))
     cases.append(("vortex_IDW", O.Case(N_x=28, N_y=28, hi_y=1.0, Boundary="IDWBaseGFM", case="vortex",
                                        extra=dict(fa=1.0, f0=0.0, f1=1.0)), 10))
+    # PVRS in 2-D: the inclined channel, and a collision along y whose faces
+    # take the y solver's branch with the stray store (PVRS/Riemann_solver.cpp:550)
+    for s in ["MUSCL", "MHM"]:
+        cases.append(("incline_PVRS_%s" % s, O.Case(N_x=28, N_y=28, hi_y=1.0, Boundary="IDWBaseGFM", Scheme=s,
+                                                    Riemann="PVRS", case="incline"), 10))
+        cases.append(("collision_PVRS_%s" % s, pvrs_collision_case(s, 24), 8))
     # SimpleBound (its sources keep the single-device signatures; oracle/ref_simplebound.cpp adapts them)
     cases.append(("incline_SimpleBound", O.Case(N_x=64, N_y=64, hi_y=1.0, Boundary="SimpleBound", case="incline",
                                                 Psy_time=0.3), 10))
@@ -53,6 +59,13 @@ def run_cases():
                                             case="pngshock", edge="pnggeo", Psy_time=0.3, CFL_number=0.1,
                                             extra=dict(rho_L=1.0, u_L=0.0, v_L=0.0, p_L=0.71429)), 12))
     return cases
+
+
+def pvrs_collision_case(scheme, n):
+    """Row-split hook (Test/Sod/Initialize.cpp): the lower half at rest, the
+    upper half moving down supersonically onto it."""
+    return O.Case(N_x=n, N_y=n, hi_y=1.0, Scheme=scheme, Riemann="PVRS", case="sod", CFL_number=0.3,
+                  extra=dict(rho_L=1.0, u_L=0.3, v_L=0.0, p_L=1.0, rho_R=1.0, u_R=0.0, v_R=-2.0, p_R=2.0))
 
 
 def case_meta(c):

@@ -179,6 +179,33 @@ def test_lax_and_pvrs(oracle_mod):
     assert max(rel_linf(g["U_OLD"], o["U_OLD"])) <= REL_TOL
 
 
+@pytest.mark.parametrize("scheme", ["MUSCL", "MHM"])
+def test_pvrs_2d_reference_mode(oracle_mod, scheme):
+    """PVRS with its y solver in reference mode: the 96^2 inclined channel (IDW)
+    and a supersonic collision along y whose faces take the branch with the
+    reference's stray store (PVRS/Riemann_solver.cpp:550). The GPU path drops
+    that store (its target is another cell's scratch record, raced in the
+    reference's parallel builds) and keeps the aliased x flux in the face's own
+    energy slot: it equals the oracle with the store dropped, within the FP64
+    tolerance (PVRS's pow() calls use the device libm), NaN positions included,
+    and differs from the oracle that keeps the store."""
+    O = oracle_mod
+    c = _case(O, N_x=96, N_y=96, hi_y=1.0, Boundary="IDWBaseGFM", Scheme=scheme, Riemann="PVRS", case="incline",
+              Psy_time=0.3)
+    g = gpu_run(c, 30)
+    o = O.run_oracle(c, 30)
+    assert g["steps"] == o["steps"] == 30
+    assert max(rel_linf(g["U_OLD"], o["U_OLD"])) <= REL_TOL
+    c = _case(O, N_x=40, N_y=40, hi_y=1.0, Scheme=scheme, Riemann="PVRS", case="sod", CFL_number=0.3,
+              extra=dict(rho_L=1.0, u_L=0.3, v_L=0.0, p_L=1.0, rho_R=1.0, u_R=0.0, v_R=-2.0, p_R=2.0))
+    g = gpu_run(c, 3)
+    o = O.run_oracle(c, 3, pvrs_stray=False)
+    keep = O.run_oracle(c, 3)
+    assert np.isfinite(o["U_OLD"]).mean() > 0.9
+    assert max(rel_linf(g["U_OLD"], o["U_OLD"])) <= REL_TOL
+    assert nan_equal_bits(o["U_OLD"], keep["U_OLD"]) > 0
+
+
 def test_graph_replay_matches_eager(oracle_mod):
     O = oracle_mod
     c = _case(O, N_x=128, N_y=32, hi_y=0.25, Scheme="MUSCL", case="sodx", Psy_time=10.0)

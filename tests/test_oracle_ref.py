@@ -72,3 +72,44 @@ def test_vortex_and_rows(oracle_mod):
     _cmp(O, O.Case(N_x=48, N_y=48, hi_y=1.0, Boundary="IDWBaseGFM", case="vortex",
                    extra=dict(fa=1.0, f0=0.0, f1=1.0)), 30)
     _cmp(O, O.Case(N_x=40, N_y=40, hi_y=1.0, case="sod"), 30)
+
+
+def _collision(O, scheme, n):
+    return O.Case(N_x=n, N_y=n, hi_y=1.0, Scheme=scheme, Riemann="PVRS", case="sod", CFL_number=0.3,
+                  extra=dict(rho_L=1.0, u_L=0.3, v_L=0.0, p_L=1.0, rho_R=1.0, u_R=0.0, v_R=-2.0, p_R=2.0))
+
+
+@pytest.mark.parametrize("scheme", ["MUSCL", "MHM"])
+def test_pvrs_2d_inclined_channel(oracle_mod, scheme):
+    """PVRS with its y solver on the inclined channel (IDW ghost fluid)."""
+    O = oracle_mod
+    c = O.Case(N_x=64, N_y=64, hi_y=1.0, Boundary="IDWBaseGFM", Scheme=scheme, Riemann="PVRS", case="incline",
+               Psy_time=0.3)
+    _cmp(O, c, 40)
+
+
+@pytest.mark.parametrize("scheme", ["MUSCL", "MHM"])
+def test_pvrs_2d_stray_store(oracle_mod, scheme):
+    """A supersonic collision along y takes the y solver's branch whose energy
+    flux the reference stores through the state index macro
+    (PVRS/Riemann_solver.cpp:550): into the scratch record of another cell.
+    The oracle reproduces it bit for bit (NaN positions included: the
+    reference's PVRS fan states go non-finite in this flow). The store lands in
+    flux slots and changes the result from the first step on: with it dropped
+    (the GPU path's definition) the oracle differs from the reference, first in
+    the energy of interior row 3 (then copied into the ring by Global_Boundary)."""
+    import ctypes as C
+    O = oracle_mod
+    c = _collision(O, scheme, 40)
+    _cmp(O, c, 8)
+    L = O.lib()
+    f, g = C.c_longlong(), C.c_longlong()
+    L.or_pvrs_stray_counts(C.byref(f), C.byref(g))
+    a = O.run_oracle(c, 1)
+    L.or_pvrs_stray_counts(C.byref(f), C.byref(g))
+    assert f.value > 0 and g.value > 0
+    b = O.run_oracle(c, 1, pvrs_stray=False)
+    d = np.flatnonzero(a["U_OLD"] != b["U_OLD"])
+    assert len(d) > 0
+    rows = sorted({int(q // 4 // c.M) for q in d})
+    assert rows[0] in (0, 1, 2, 3) and all(q % 4 == 3 for q in d)
