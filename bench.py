@@ -325,26 +325,8 @@ def config_legs(args):
 
 
 def _plain_solve(it):
-    """One request through the single-solve path (create, geometry, initial
-    state, boundary + dt, graph-replayed steps, final status)."""
-    import ibtfo_amd as ib
-    s = ib.Solver(it["cfg"])
-    s.geometry(*it["levelset"])
-    s.initialize(*it["init"])
-    s.prepare(True)
-    n, chunk, done = it["cfg"].max_iter, 50, 0
-    while done < n:  # stops early when the solve reaches its Psy_time
-        m = min(chunk, n - done)
-        s.step(m, graph_chunk=chunk if m == chunk else 0)
-        i = s.status()[0]
-        if i < done + m:
-            break
-        done = i
-    st = s.status()
-    U = s.state() if it.get("keep_state") else None
-    types = s.xycoord()[5::6] if it.get("keep_state") else None
-    s.close()
-    return st, U, types
+    from ibtfo_amd import serving as S
+    return S.plain_solve(it)
 
 
 def serving_leg(args):
@@ -397,30 +379,12 @@ def serving_leg(args):
     # L1 density gate on a sample: mixed (batched) and FP64 (plain) against a 2x-refined FP64 solve
     gate = []
     for i in range(min(args.gate_sample, len(stream))):
-        sm = stream[i]
-        it64 = dict(items64[i], keep_state=True)
-        st64, U64, types = _plain_solve(it64)
         b = S.Batch(bc)
         t = b.submit(**dict(items32[i], keep_state=True))
         b.drain()
         r32, U32 = b.result(t)
         b.close()
-        fine = dict(sm, N_x=2 * sm["N_x"], N_y=2 * sm["N_y"], steps=10 ** 9)
-        itf = S.ellipse_items([fine], precision="FP64", keep_state=True)[0]
-        itf["cfg"].Psy_time = st64[1]
-        stf, Uf, tf = _plain_solve(itf)
-        nx, ny = sm["N_x"], sm["N_y"]
-        M, N = nx + 4, ny + 4
-        rf = Uf[0::4].reshape(2 * ny + 4, 2 * nx + 4)[2:-2, 2:-2]
-        tff = tf.reshape(2 * ny + 4, 2 * nx + 4)[2:-2, 2:-2]
-        ref = 0.25 * (rf[0::2, 0::2] + rf[1::2, 0::2] + rf[0::2, 1::2] + rf[1::2, 1::2])
-        okf = (tff[0::2, 0::2] == 0) & (tff[1::2, 0::2] == 0) & (tff[0::2, 1::2] == 0) & (tff[1::2, 1::2] == 0)
-        mask = (types.reshape(N, M)[2:-2, 2:-2] == 0) & okf
-        r64 = U64[0::4].reshape(N, M)[2:-2, 2:-2]
-        r32 = U32[0::4].reshape(N, M)[2:-2, 2:-2]
-        e64 = float(np.mean(np.abs(r64[mask] - ref[mask])))
-        e32 = float(np.mean(np.abs(r32[mask] - ref[mask])))
-        gate.append(dict(grid=[nx, ny], l1_fp64=e64, l1_mixed=e32, ratio=e32 / e64))
+        gate.append(S.l1_gate(stream[i], items64[i], U32))
     return {
         "requests": len(items32), "steps_per_request": args.serve_steps,
         "workload": "immersed ellipse, Mach 1.5-4 inflow, grids 64x64..1024x512 (seed 1234), MUSCL + VanLeer "

@@ -250,5 +250,57 @@ def ellipse_items(stream, Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer", Boun
     return items
 
 
+def plain_solve(item, chunk=50):
+    """One request through the single-solve path (create, geometry, initial
+    state, boundary + dt, graph-replayed steps, final status). Returns (status
+    tuple, U or None, cell types or None); state and types when the item keeps
+    its state."""
+    from . import Solver
+    s = Solver(item["cfg"])
+    try:
+        s.geometry(*item["levelset"])
+        s.initialize(*item["init"])
+        s.prepare(True)
+        n, done = item["cfg"].max_iter, 0
+        while done < n:  # stops early when the solve reaches its Psy_time
+            m = min(chunk, n - done)
+            s.step(m, graph_chunk=chunk if m == chunk else 0)
+            i = s.status()[0]
+            if i < done + m:
+                break
+            done = i
+        st = s.status()
+        U = s.state() if item.get("keep_state") else None
+        types = s.xycoord()[5::6] if item.get("keep_state") else None
+    finally:
+        s.close()
+    return st, U, types
+
+
+def l1_gate(entry, item64, U32, fp64=None):
+    """The fixed-L1 gate of one request: L1(rho_mixed - rho_ref) against
+    L1(rho_fp64 - rho_ref) over the fluid cells, rho_ref being an FP64 solve on
+    the 2x-refined grid to the same time, averaged to the request's grid (its
+    four children all fluid). U32: the mixed-precision final state (AoS);
+    fp64: (status, U, types) of the FP64 solve if already run. Returns a dict."""
+    st64, U64, types = fp64 if fp64 is not None else plain_solve(dict(item64, keep_state=True))
+    fine = dict(entry, N_x=2 * entry["N_x"], N_y=2 * entry["N_y"], steps=10 ** 9)
+    itf = ellipse_items([fine], precision="FP64", keep_state=True)[0]
+    itf["cfg"].Psy_time = st64[1]
+    _, Uf, tf = plain_solve(itf)
+    nx, ny = entry["N_x"], entry["N_y"]
+    M, N = nx + 4, ny + 4
+    rf = Uf[0::4].reshape(2 * ny + 4, 2 * nx + 4)[2:-2, 2:-2]
+    tff = tf.reshape(2 * ny + 4, 2 * nx + 4)[2:-2, 2:-2]
+    ref = 0.25 * (rf[0::2, 0::2] + rf[1::2, 0::2] + rf[0::2, 1::2] + rf[1::2, 1::2])
+    okf = (tff[0::2, 0::2] == 0) & (tff[1::2, 0::2] == 0) & (tff[0::2, 1::2] == 0) & (tff[1::2, 1::2] == 0)
+    mask = (types.reshape(N, M)[2:-2, 2:-2] == 0) & okf
+    r64 = U64[0::4].reshape(N, M)[2:-2, 2:-2]
+    r32 = U32[0::4].reshape(N, M)[2:-2, 2:-2]
+    e64 = float(np.mean(np.abs(r64[mask] - ref[mask])))
+    e32 = float(np.mean(np.abs(r32[mask] - ref[mask])))
+    return dict(grid=[nx, ny], l1_fp64=e64, l1_mixed=e32, ratio=e32 / e64)
+
+
 __all__ = ["Batch", "BatchConfig", "Request", "Result", "batch_config", "serve", "ellipse_stream",
-           "ellipse_items", "OK", "E_NONFINITE"]
+           "ellipse_items", "plain_solve", "l1_gate", "OK", "E_NONFINITE"]
