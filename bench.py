@@ -71,7 +71,7 @@ def workload_geometry(args, s):
         s.geometry("shapes", W.bodies_shapes())
 
 
-def gpu_leg(args, precision="FP64", keep_state=False, riemann=None, boundary=None):
+def gpu_leg(args, precision="FP64", keep_state=False, riemann=None, boundary=None, keep_phi=False):
     import ibtfo_amd as ib
     from ibtfo_amd import workloads as W
     cfg = W.mach3_bodies_config(ib, N_x=args.nx, N_y=args.ny, Riemann=riemann or args.riemann,
@@ -98,15 +98,21 @@ def gpu_leg(args, precision="FP64", keep_state=False, riemann=None, boundary=Non
     if args.warmup:
         s.step(args.warmup)
         s.status()
-    if args.graph_chunk > 0:  # capture the step sequence once; the timed loop replays it
-        s.step(args.graph_chunk, graph_chunk=args.graph_chunk)
-        s.status()
     # K timed steps in 5 equal slices (HIP events around each on the solve's
     # stream): value = all K steps, plus the slices' median
     cells = args.nx * args.ny
     nsl = 5 if args.steps >= 5 else 1
     slices = [args.steps // nsl] * nsl
     slices[-1] += args.steps - sum(slices)
+    chunk = 0
+    if args.graph_chunk > 0:
+        # capture the step sequence once, `chunk` steps long, a divisor of the
+        # slice length: every timed step is replayed from the graph
+        chunk = min(args.graph_chunk, slices[0])
+        while slices[0] % chunk or slices[-1] % chunk:
+            chunk -= 1
+        s.step(chunk, graph_chunk=chunk)
+        s.status()
     slice_ms = [s.time_steps(n, kernel=False)[0] for n in slices]
     total_ms = sum(slice_ms)
     slice_rates = sorted(cells * n / (m / 1e3) for n, m in zip(slices, slice_ms))
@@ -125,52 +131,75 @@ def gpu_leg(args, precision="FP64", keep_state=False, riemann=None, boundary=Non
     out = dict(value=value, ms_per_step=ms_step, kernel_ms=kern_ms, achieved=achieved, traffic=traffic,
                iter=it, t=t, dt=dt, status=rc, nghost=nghost, ncut=ncut, ngroups=ngroups, rows=rows,
                slice_rates=slice_rates, median=slice_rates[len(slice_rates) // 2], geometry_ms=geometry_ms, grid=[nbx, nby],
-               tuned=tuned is not None)
+               tuned=tuned is not None, chunk=chunk, slices=slices)
     if keep_state:
         out["rho"] = s.state()[0::4].copy()
         out["types"] = s.xycoord()[5::6].copy()
+    if keep_phi:
+        out["phi"] = s.xycoord()[2::6].copy()
     s.close()
     return out
 
 
-def cpu_leg(args):
-    """The oracle (C++ restatement of the reference loop, OpenMP) on a bounded
-    sample: the same workload on a 512 x 256 grid for a fixed number of steps."""
-    import ctypes as C
+def _physical_cores():
+    """Physical cores of the machine as lscpu reports them (cores per socket x
+    sockets), or None."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except Exception:
+        return None
+    kv = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    try:
+        return int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_leg(args, phi):
+    """The oracle (C++ restatement of the reference loop, OpenMP) on the same
+    workload at its full size (4096 x 2048, the device's mask level set handed
+    over as a per-cell field), truncated to its first `cpu_steps` steps; and
+    on one thread for its first `cpu_steps_1t` steps. Threads: this process's
+    CPU share (its affinity mask, and OMP_NUM_THREADS where the environment
+    sets it) within the machine's physical cores; ACC_NUM_CORES of the
+    reference's multicore build would be the same number."""
     from oracle import oracle as O
-    from ibtfo_amd import workloads as W
-    nx, ny = 512, 256
-    steps = args.cpu_steps
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    nx, ny = args.nx, args.ny
+    affinity = sorted(os.sched_getaffinity(0))
+    phys = _physical_cores()
+    threads = _cpu_threads(args)
     O.lib().or_set_num_threads(threads)
     case = O.Case(N_x=nx, N_y=ny, lo_x=0.0, lo_y=0.0, hi_x=2.0, hi_y=1.0, Psy_time=1e30, max_iter=10 ** 9,
                   CFL_number=0.5, Scheme="MUSCL", Riemann=args.riemann, Limiter=args.limiter,
                   Boundary=args.boundary, case="uniform", edge="inflow", quirks=0,
                   extra=dict(rho=1.4, u=3.0, v=0.0, p=1.0))
-    if args.geometry == "mask":
-        case.phi = O.mask_phi(case, W.bodies_mask(nx, ny))
-    else:
-        M, N = case.M, case.N
-        dx = 2.0 / nx
-        Li = -dx * 2 + 0.0 + dx / 2
-        x = (Li + np.arange(M) * dx)[None, :].repeat(N, 0)
-        y = (Li + np.arange(N) * dx)[:, None].repeat(M, 1)
-        case.phi = W.shapes_phi(W.bodies_shapes(), x, y).reshape(-1)
+    case.phi = phi
     st = O.setup(case)
     t0 = time.perf_counter()
-    r = O.run_oracle(case, steps, state=st)
+    r = O.run_oracle(case, args.cpu_steps, state=st)
     el = time.perf_counter() - t0
-    # one thread, the first steps of the same sample
+    _say("CPU baseline: %d steps on %d threads in %.1f s" % (r["steps"], threads, el))
+    del st, r["U_OLD"], r["U_NEW"]
     O.lib().or_set_num_threads(1)
     st1 = O.setup(case)
     t1 = time.perf_counter()
     r1 = O.run_oracle(case, args.cpu_steps_1t, state=st1)
     el1 = time.perf_counter() - t1
+    del st1
     O.lib().or_set_num_threads(threads)
+    cpus = "%d-%d" % (affinity[0], affinity[-1]) if affinity == list(range(affinity[0], affinity[-1] + 1)) else \
+        ",".join(map(str, affinity))
     return dict(value=nx * ny * r["steps"] / el, cores=threads, steps=r["steps"], seconds=el,
-                sample="%dx%d cells, %d steps of the same workload (oracle, OpenMP %d threads, OMP_PROC_BIND=%s)"
-                % (nx, ny, r["steps"], threads, os.environ.get("OMP_PROC_BIND")),
-                value_1t=nx * ny * r1["steps"] / el1, steps_1t=r1["steps"])
+                physical_cores=phys, affinity=cpus, acc_num_cores=threads,
+                sample="the workload at full size (%dx%d cells), its first %d steps (oracle, OpenMP %d threads, "
+                       "OMP_PROC_BIND=%s, CPUs %s of a machine with %s physical cores)"
+                % (nx, ny, r["steps"], threads, os.environ.get("OMP_PROC_BIND"), cpus, phys),
+                value_1t=nx * ny * r1["steps"] / el1, steps_1t=r1["steps"], seconds_1t=el1)
 
 
 def copy_bandwidth():
@@ -204,6 +233,21 @@ def copy_bandwidth():
     return gbs
 
 
+def _cfg_frac(cells, steps, ms, stages=1):
+    """Fraction of the HBM roofline of a whole run: 65 algorithmic bytes per cell
+    and step-kernel stage (RK2: two), over the run's time, against 8 TB/s."""
+    return BYTES_PER_CELL * cells * stages * steps / (ms / 1e3) / 1e9 / HBM_PEAK_GBS
+
+
+def _cpu_threads(args):
+    """Threads for the CPU legs: this process's CPU share (affinity mask, and
+    OMP_NUM_THREADS where the environment sets it), within the physical cores."""
+    affinity = len(os.sched_getaffinity(0))
+    t = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or affinity
+    phys = _physical_cores()
+    return min(t, affinity, phys or t)
+
+
 def config_legs(args):
     """The reference's other configurations, the GPU beside the CPU oracle: the
     400x100 Sod tube (MHM + minmod + HLLC, to t = 0.2), the inclined Sod channel
@@ -217,7 +261,7 @@ def config_legs(args):
     import ibtfo_amd as ib
     from ibtfo_amd import workloads as W
     from oracle import oracle as O
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = _cpu_threads(args)
     O.lib().or_set_num_threads(threads)
     out = []
 
@@ -232,9 +276,11 @@ def config_legs(args):
         s.geometry(*ls)
         s.initialize(*init)
         s.prepare(apply_boundary)
+        rows, _ = s.autotune(min(20, max(1, n // 8)))  # probes restore the state and clock
         s.step(0, graph_chunk=chunk)  # capture only
         ms, _ = s.time_steps(n, kernel=False)
         it, t, dt, rc = s.status()
+        s.tuned_rows = rows
         return s, n, ms, t, rc
 
     def cpu_run(case, nsteps):
@@ -276,7 +322,8 @@ def config_legs(args):
     ref = ref_run(c, 10 ** 6, 400 * 100, "the whole run; the x-split Sod hook compiled into the reference's loop")
     _say("Sod tube measured")
     out.append(dict(workload="Sod tube 400x100 on [0,1]x[0,0.25], MHM + minmod + HLLC, Regular, to t = 0.2",
-                    steps=n, gpu_value=400 * 100 * n / (ms / 1e3), gpu_ms=ms, status=rc,
+                    steps=n, gpu_value=400 * 100 * n / (ms / 1e3), gpu_ms=ms, gpu_us_per_step=1e3 * ms / n,
+                    roofline_frac=_cfg_frac(400 * 100, n, ms), tile_rows=s.tuned_rows, status=rc,
                     cpu_value=400 * 100 * cn / cs, cpu_steps=cn, cpu_cores=threads, cpu_reference=ref))
     # inclined Sod channel, 800x400, IDW
     cfg = ib.config(N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Psy_time=0.2, max_iter=10 ** 9, CFL_number=0.5,
@@ -290,7 +337,8 @@ def config_legs(args):
     _say("inclined channel measured")
     out.append(dict(workload="inclined Sod channel 800x400 on [0,2]x[0,1] (30 deg, AxisD 0.4), MUSCL + minmod "
                              "+ HLL, IDW ghost fluid, to t = 0.2",
-                    steps=n, gpu_value=800 * 400 * n / (ms / 1e3), gpu_ms=ms, status=rc,
+                    steps=n, gpu_value=800 * 400 * n / (ms / 1e3), gpu_ms=ms, gpu_us_per_step=1e3 * ms / n,
+                    roofline_frac=_cfg_frac(800 * 400, n, ms), tile_rows=s.tuned_rows, status=rc,
                     cpu_value=800 * 400 * cn / cs, cpu_steps=cn, cpu_cores=threads,
                     cpu_note="first %d steps of the run" % cn, cpu_reference=ref))
     # isentropic vortex 1024^2, FP64 and mixed
@@ -306,7 +354,7 @@ def config_legs(args):
         s, n, ms, t, rc = gpu_run(cfg, ("none", ()), ("shu", (5.0, 5.0, 5.0, 1.0, 1.0, 1.0, 1.0)), False)
         rho = s.state().reshape(nv + 4, nv + 4, 4)[2:-2, 2:-2, 0]
         s.close()
-        vort[prec] = dict(steps=n, ms=ms, t=t, rc=rc,
+        vort[prec] = dict(steps=n, ms=ms, t=t, rc=rc, rows=s.tuned_rows,
                           l1=float(np.mean(np.abs(rho - W.shu_exact(X, Y, t)))), rho=rho)
     _say("isentropic vortex on the GPU measured")
     c = O.Case(N_x=nv, N_y=nv, hi_x=10.0, hi_y=10.0, Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer",
@@ -316,6 +364,8 @@ def config_legs(args):
     out.append(dict(workload="isentropic vortex 1024x1024 on [0,10]^2 (strength 5, free stream (1,1,1,1)), "
                              "periodic, MUSCL + VanLeer + HLL, RK2 (Heun), to t = 10",
                     steps=f["steps"], gpu_value=nv * nv * f["steps"] / (f["ms"] / 1e3), gpu_ms=f["ms"],
+                    gpu_us_per_step=1e3 * f["ms"] / f["steps"],
+                    roofline_frac=_cfg_frac(nv * nv, f["steps"], f["ms"], stages=2), tile_rows=f["rows"],
                     status=f["rc"], mixed_value=nv * nv * m["steps"] / (m["ms"] / 1e3),
                     l1_rho_exact_fp64=f["l1"], l1_rho_exact_mixed=m["l1"], l1_ratio=m["l1"] / f["l1"],
                     l1_rho_mixed_vs_fp64=float(np.mean(np.abs(m["rho"] - f["rho"]))),
@@ -526,9 +576,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default tile shape, no autotuning")
     ap.add_argument("--tune-steps", type=int, default=20)
     ap.add_argument("--graph-chunk", type=int, default=100, help="steps per captured HIP graph (0: plain launches)")
-    ap.add_argument("--cpu-steps", type=int, default=2000)
+    ap.add_argument("--cpu-steps", type=int, default=40, help="steps of the CPU baseline (full grid)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-steps-1t", type=int, default=300)
+    ap.add_argument("--cpu-steps-1t", type=int, default=5, help="steps of the one-thread CPU figure")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-idw", action="store_true", help="skip the IDW ghost-fluid leg")
@@ -560,7 +610,7 @@ def main():
     def progress(what):
         print("bench: %s done (%.0f s)" % (what, time.perf_counter() - t_start), file=sys.stderr, flush=True)
 
-    g = gpu_leg(args, keep_state=not args.no_mixed)
+    g = gpu_leg(args, keep_state=not args.no_mixed, keep_phi=not args.no_cpu)
     progress("FP64 leg")
     mixed = None
     if not args.no_mixed:
@@ -599,11 +649,12 @@ def main():
     copy_gbs = copy_bandwidth()
     cpu = None
     if not args.no_cpu:
-        c = cpu_leg(args)
+        c = cpu_leg(args, g.pop("phi"))
         progress("CPU baseline")
         cpu = dict(value=c["value"], unit="cell-updates/s", cores=c["cores"], kind="port", sample=c["sample"],
+                   ACC_NUM_CORES=c["acc_num_cores"], physical_cores=c["physical_cores"], affinity=c["affinity"],
                    single_thread={"value": c["value_1t"], "cores": 1, "sample": "the first %d steps of the same "
-                                  "sample on one thread" % c["steps_1t"]})
+                                  "workload on one thread" % c["steps_1t"]})
     line = {
         "metric": "cell-updates/s (frac. HBM roofline); batched p50/p99 solve latency at fixed L1",
         "value": g["value"],
@@ -622,7 +673,10 @@ def main():
                    "steps_taken": g["iter"], "status": g["status"],
                    "geometry_ms": g["geometry_ms"],
                    "tile": {"columns": 62, "rows": g["rows"], "grid": g["grid"], "autotuned": g["tuned"]},
-                   "launch": "hipGraph of %d steps" % args.graph_chunk if args.graph_chunk > 0 else "stream",
+                   "launch": ("%d timed slices of %s steps, each a captured HIP graph of %d steps replayed %s"
+                              % (len(g["slices"]), "/".join(map(str, sorted(set(g["slices"])))), g["chunk"],
+                                 "/".join(str(n // g["chunk"]) for n in sorted(set(g["slices"]))) + " times")
+                              if g["chunk"] else "stream launches, no graph"),
                    "parallelism": "single GPU, whole domain resident"},
         "roofline": {"bound": "hbm", "achieved": g["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": g["achieved"] / HBM_PEAK_GBS, "traffic": g["traffic"],

@@ -133,6 +133,10 @@ struct ibtfo_batch {
     int2 *d_smap = nullptr, *h_smap = nullptr, *d_gmap = nullptr, *h_gmap = nullptr, *d_rmap = nullptr,
          *h_rmap = nullptr;
     SlotStatus *d_stat = nullptr, *h_stat = nullptr;
+    // persistent mode: item counts the kernel reads, its barrier counter and error flag
+    int *d_counts = nullptr, *h_counts = nullptr, *d_err = nullptr;
+    unsigned int* d_bar = nullptr;
+    int persist_grid = 0;
     int used_step = 0, used_ghost = 0, used_ring = 0;  // blocks of the resident solves
     hipGraphExec_t graph = nullptr;
     bool dirty = true;
@@ -190,6 +194,7 @@ extern "C" int ibtfo_batch_config_default(ibtfo_batch_config* bc) {
     bc->max_ring_blocks = 16384;
     bc->rows = 16;
     bc->chunk = 50;
+    bc->persistent = 0;
     return IBTFO_OK;
 }
 
@@ -202,16 +207,19 @@ static void batch_free(ibtfo_batch* b) {
     for (auto& p : b->pending) free_ctx(p.first);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (void* p : {(void*)b->d_step, (void*)b->d_fin, (void*)b->d_fill, (void*)b->d_edge, (void*)b->d_smap,
-                    (void*)b->d_gmap, (void*)b->d_rmap, (void*)b->d_stat})
+                    (void*)b->d_gmap, (void*)b->d_rmap, (void*)b->d_stat, (void*)b->d_counts, (void*)b->d_bar,
+                    (void*)b->d_err})
         if (p) (void)hipFree(p);
     for (void* p : {(void*)b->h_step, (void*)b->h_fin, (void*)b->h_fill, (void*)b->h_edge, (void*)b->h_smap,
-                    (void*)b->h_gmap, (void*)b->h_rmap, (void*)b->h_stat})
+                    (void*)b->h_gmap, (void*)b->h_rmap, (void*)b->h_stat, (void*)b->h_counts})
         if (p) (void)hipHostFree(p);
     b->arena.release();
     if (b->run) (void)hipStreamDestroy(b->run);
     if (b->adm) (void)hipStreamDestroy(b->adm);
     delete b;
 }
+
+static int ensure_persistent(ibtfo_batch* b);
 
 extern "C" int ibtfo_batch_create(const ibtfo_batch_config* bc, ibtfo_batch** out) {
     if (!bc || !out) { set_error("null argument"); return IBTFO_E_ARG; }
@@ -249,6 +257,10 @@ extern "C" int ibtfo_batch_create(const ibtfo_batch_config* bc, ibtfo_batch** ou
         return IBTFO_E_NOMEM;
     }
     b->slots.resize(S);
+    if (bc->persistent && (rc = ensure_persistent(b))) {
+        batch_free(b);
+        return rc;
+    }
     *out = b;
     return IBTFO_OK;
 }
@@ -385,6 +397,10 @@ static int rebuild_tables(ibtfo_batch* b) {
     b->used_ghost = ngh;
     b->used_ring = nr;
     hipStream_t s = b->run;
+    if (b->bc.persistent) {
+        b->h_counts[0] = ns; b->h_counts[1] = ngh; b->h_counts[2] = nr;
+        HIPCHK(hipMemcpyAsync(b->d_counts, b->h_counts, 3 * sizeof(int), hipMemcpyHostToDevice, s));
+    }
     HIPCHK(hipMemcpyAsync(b->d_step, b->h_step, S * sizeof(StepArgs), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(b->d_fin, b->h_fin, S * sizeof(FinishArgs), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(b->d_fill, b->h_fill, S * sizeof(FillArgs), hipMemcpyHostToDevice, s));
@@ -427,11 +443,31 @@ static int enqueue_batch_step(ibtfo_batch* b) {
     return bounds();
 }
 
+// The chunk as one persistent launch: every resident solve's steps, phases
+// separated by grid barriers (see k_persist).
+static int enqueue_persistent_chunk(ibtfo_batch* b) {
+    const ibtfo_batch_config& k = b->bc;
+    PersistArgs p;
+    memset(&p, 0, sizeof p);
+    p.step = b->d_step; p.fin = b->d_fin; p.fill = b->d_fill; p.edge = b->d_edge;
+    p.smap = b->d_smap; p.gmap = b->d_gmap; p.rmap = b->d_rmap; p.counts = b->d_counts;
+    p.nslots = k.max_slots;
+    p.nsteps = k.chunk;
+    p.rk2 = k.time == IBTFO_RK2;
+    p.has_fill = k.boundary != IBTFO_REGULAR;
+    p.bar = b->d_bar;
+    p.err = b->d_err;
+    p.spin_limit = 1ll << 22;  // about 4 s of polling before the launch gives up
+    HIPCHK(launch_persist(k.scheme, k.riemann, k.limiter, k.precision, p, b->persist_grid, b->run));
+    return IBTFO_OK;
+}
+
 static int capture_graph(ibtfo_batch* b) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(b->run, hipStreamCaptureModeThreadLocal));
     int rc = IBTFO_OK;
-    for (int i = 0; i < b->bc.chunk && rc == IBTFO_OK; i++) rc = enqueue_batch_step(b);
+    if (b->bc.persistent) rc = enqueue_persistent_chunk(b);
+    for (int i = 0; !b->bc.persistent && i < b->bc.chunk && rc == IBTFO_OK; i++) rc = enqueue_batch_step(b);
     hipError_t e = hipStreamEndCapture(b->run, &g);
     if (e == hipSuccess && rc == IBTFO_OK && g) e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
     if (g) (void)hipGraphDestroy(g);
@@ -444,7 +480,13 @@ static int harvest(ibtfo_batch* b) {
     const int S = b->bc.max_slots;
     HIPCHK(launch_gather(b->d_fin, S, b->d_stat, b->run));
     HIPCHK(hipMemcpyAsync(b->h_stat, b->d_stat, S * sizeof(SlotStatus), hipMemcpyDeviceToHost, b->run));
+    int perr = 0;
+    if (b->bc.persistent) HIPCHK(hipMemcpyAsync(&perr, b->d_err, sizeof(int), hipMemcpyDeviceToHost, b->run));
     HIPCHK(hipStreamSynchronize(b->run));
+    if (perr) {
+        set_error("persistent step kernel: a grid barrier timed out (blocks not resident together)");
+        return IBTFO_E_HIP;
+    }
     for (int i = 0; i < S; i++) {
         Slot& sl = b->slots[i];
         if (!sl.ctx || !b->h_stat[i].done) continue;
@@ -541,6 +583,111 @@ extern "C" int ibtfo_batch_progress(ibtfo_batch* b, int* active, int* queued, in
     if (active) *active = na;
     if (queued) *queued = (int)b->queue.size();
     if (finished) *finished = b->finished;
+    return IBTFO_OK;
+}
+
+// Allocates the persistent kernel's state (item counts, barrier counter, error
+// flag) and sizes its grid, once.
+static int ensure_persistent(ibtfo_batch* b) {
+    if (b->d_counts) return IBTFO_OK;
+    const ibtfo_batch_config& k = b->bc;
+    b->persist_grid = persist_blocks(k.scheme, k.riemann, k.limiter, k.precision);
+    if (b->persist_grid <= 0) { set_error("no resident grid for the persistent step kernel"); return IBTFO_E_HIP; }
+    if (dev_alloc(&b->d_counts, 3) || dev_alloc(&b->d_bar, 1) || dev_alloc(&b->d_err, 1) ||
+        host_alloc(&b->h_counts, 3) || hipMemset(b->d_err, 0, sizeof(int)) != hipSuccess) {
+        set_error("allocation of the persistent kernel's state failed");
+        return IBTFO_E_NOMEM;
+    }
+    return IBTFO_OK;
+}
+
+extern "C" int ibtfo_batch_autotune(ibtfo_batch* b, const ibtfo_request* sample, int n, int probe_chunks,
+                                    int* best_rows, int* best_persistent, double* best_ms_per_step) {
+    if (!b || !sample || n <= 0 || probe_chunks <= 0) { set_error("ibtfo_batch_autotune: bad argument"); return IBTFO_E_ARG; }
+    for (auto& s : b->slots)
+        if (s.ctx) { set_error("ibtfo_batch_autotune: the batch has resident solves"); return IBTFO_E_STATE; }
+    if (!b->queue.empty() || !b->pending.empty()) {
+        set_error("ibtfo_batch_autotune: the batch has queued requests");
+        return IBTFO_E_STATE;
+    }
+    const ibtfo_batch_config keep = b->bc;
+    static const int rows_cand[] = {4, 8, 16, 32};
+    int rc = IBTFO_OK, brow = keep.rows, bper = keep.persistent;
+    double bms = 1e300;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        set_error("ibtfo_batch_autotune: event creation failed");
+        rc = IBTFO_E_HIP;
+    }
+    auto clear = [&]() {
+        (void)hipStreamSynchronize(b->run);
+        (void)hipStreamSynchronize(b->adm);
+        for (auto& s : b->slots)
+            if (s.ctx) { free_ctx(s.ctx); s.ctx = nullptr; }
+        if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+        b->dirty = true;
+        b->used_step = b->used_ghost = b->used_ring = 0;
+    };
+    for (int per = 0; per < 2 && !rc; per++) {
+        if (per && (rc = ensure_persistent(b))) { rc = IBTFO_OK; break; }  // no persistent grid: launches only
+        for (int r : rows_cand) {
+            b->bc.rows = r;
+            b->bc.persistent = per;
+            // the sample as resident solves that run for the whole probe
+            int ns = 0, ng = 0, nr = 0, used = 0;
+            for (int i = 0; i < n && used < b->bc.max_slots && !rc; i++) {
+                ibtfo_config c = sample[i].cfg;
+                c.max_iter = 1 << 30;
+                c.Psy_time = 1e300;
+                ibtfo_ctx* x = nullptr;
+                int e = ctx_create(&c, &b->arena, b->adm, r, &x);
+                if (e == IBTFO_OK)
+                    e = ibtfo_geometry(x, sample[i].ls_kind, sample[i].ls_prm, sample[i].ls_nprm, sample[i].phi);
+                if (e == IBTFO_OK)
+                    e = ibtfo_initialize(x, sample[i].init_kind, sample[i].init_prm, sample[i].init_nprm);
+                if (e == IBTFO_OK) e = ibtfo_prepare(x, c.boundary != IBTFO_REGULAR);
+                if (e == IBTFO_OK && (ns + step_blocks(x) > b->bc.max_step_blocks ||
+                                      ng + ghost_blocks(x) > b->bc.max_ghost_blocks ||
+                                      nr + ring_blocks(x) > b->bc.max_ring_blocks))
+                    e = IBTFO_E_NOMEM;
+                if (e != IBTFO_OK) {  // does not fit (memory or grid capacity): probe with what does
+                    if (x) free_ctx(x);
+                    if (e != IBTFO_E_NOMEM) rc = e;
+                    break;
+                }
+                ns += step_blocks(x); ng += ghost_blocks(x); nr += ring_blocks(x);
+                b->slots[used++].ctx = x;
+            }
+            if (!rc && used == 0) { set_error("ibtfo_batch_autotune: no sample request fits the batch"); rc = IBTFO_E_NOMEM; }
+            float ms = 0;
+            if (!rc) rc = rebuild_tables(b);
+            if (!rc) rc = capture_graph(b);
+            if (!rc && (hipStreamSynchronize(b->adm) != hipSuccess || hipGraphLaunch(b->graph, b->run) != hipSuccess ||
+                        hipEventRecord(e0, b->run) != hipSuccess)) rc = IBTFO_E_HIP;
+            for (int q = 0; q < probe_chunks && !rc; q++)
+                if (hipGraphLaunch(b->graph, b->run) != hipSuccess) rc = IBTFO_E_HIP;
+            if (!rc && (hipEventRecord(e1, b->run) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) rc = IBTFO_E_HIP;
+            int perr = 0;
+            if (!rc && per && hipMemcpy(&perr, b->d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) rc = IBTFO_E_HIP;
+            if (!rc && perr) { set_error("ibtfo_batch_autotune: a grid barrier of the persistent kernel timed out"); rc = IBTFO_E_HIP; }
+            clear();
+            if (rc == IBTFO_E_HIP) set_error("ibtfo_batch_autotune: probe failed (%s)", hipGetErrorString(hipGetLastError()));
+            if (rc) break;
+            const double per_step = ms / (probe_chunks * (double)b->bc.chunk);
+            if (per_step < bms) { bms = per_step; brow = r; bper = per; }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    b->bc = keep;
+    if (rc) return rc;
+    b->bc.rows = brow;
+    b->bc.persistent = bper;
+    if (bper && (rc = ensure_persistent(b))) return rc;
+    if (best_rows) *best_rows = brow;
+    if (best_persistent) *best_persistent = bper;
+    if (best_ms_per_step) *best_ms_per_step = bms;
     return IBTFO_OK;
 }
 

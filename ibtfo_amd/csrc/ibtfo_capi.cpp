@@ -1153,7 +1153,9 @@ extern "C" int ibtfo_autotune(ibtfo_ctx* x, int probe_steps, int* best_rows, dou
         return IBTFO_OK;
     };
     const int keep = x->rows_override;
-    static const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
+    // short strips for small grids (latency: a wave's rows run one after the
+    // other), tall ones for large grids (fewer halo rows re-read per row updated)
+    static const int cand[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
     int best = 0;
     double bms = 1e300;
     hipEvent_t e0 = nullptr, e1 = nullptr;

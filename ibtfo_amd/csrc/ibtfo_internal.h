@@ -189,6 +189,30 @@ constexpr int FOLD_MIN_PARTS = 2048;  // fewer partials: one block reads them al
 constexpr int FILL_THREADS = 128;  // ghost-fill and scatter blocks
 constexpr int EDGE_THREADS = 256;  // edge-ring blocks
 
+// Persistent batched step (k_persist): one launch runs `nsteps` whole steps of
+// every resident solve of a batch, its phases (strips; per-solve finish; ghost
+// fill and edge ring; scatter) separated by grid barriers among the launch's
+// blocks, which are all resident at once (the grid is sized to the kernel's
+// occupancy). The item counts are read from device memory, so a captured launch
+// stays valid while solves come and go.
+struct PersistArgs {
+    const StepArgs* step;
+    const FinishArgs* fin;
+    const FillArgs* fill;
+    const EdgeArgs* edge;
+    const int2* smap;   // strip items {slot, strip}
+    const int2* gmap;   // ghost items {slot, first ghost} (FILL_THREADS ghosts each)
+    const int2* rmap;   // ring items {slot, first ring cell} (EDGE_THREADS cells each)
+    const int* counts;  // device: {strip items, ghost items, ring items}
+    int nslots;
+    int nsteps;
+    int rk2;
+    int has_fill;       // a ghost-fluid boundary (fill + scatter phases)
+    unsigned int* bar;  // barrier arrivals (zeroed before each launch)
+    int* err;           // set when a barrier wait exceeds spin_limit
+    long long spin_limit;
+};
+
 // per-slot status record of the batch engine (k_gather)
 struct SlotStatus {
     int iter, done, status, pad;
@@ -216,6 +240,10 @@ hipError_t launch_fill_batch(const FillArgs* f, const int2* map, int nblocks, hi
 hipError_t launch_scatter_batch(const FillArgs* f, const int2* map, int nblocks, hipStream_t s);
 hipError_t launch_edge_batch(const EdgeArgs* e, const int2* map, int nblocks, hipStream_t s);
 hipError_t launch_gather(const FinishArgs* f, int nslots, SlotStatus* out, hipStream_t s);
+// the persistent batched step: a zeroing of the barrier counter, then one launch of
+// `blocks` blocks (persist_blocks: the grid the kernel can hold resident at once)
+hipError_t launch_persist(int scheme, int rs, int lim, int prec, const PersistArgs& p, int blocks, hipStream_t s);
+int persist_blocks(int scheme, int rs, int lim, int prec);
 hipError_t launch_step_lax(const StepArgs& a, hipStream_t s);
 hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_from_input, hipStream_t s);
 

@@ -15,6 +15,8 @@
 // face c-1/2 from lane t-1 through LDS, and carries the y flux of face j-1/2
 // and its cell's upper y state from the previous row. Each state value is read
 // from HBM once per step (halo re-reads hit L2) and written once.
+#include <type_traits>
+
 #include "ibtfo_internal.h"
 
 namespace ibtfo {
@@ -1013,6 +1015,81 @@ __global__ void k_edge_batch(const EdgeArgs* __restrict__ slots, const int2* __r
     if (m.x >= 0) edge_one(slots[m.x], m.y + threadIdx.x);
 }
 
+// ------------------------------------------------------ persistent batched step
+// Grid barrier among all blocks of a launch (every block resident): arrivals
+// counted in bar[0], which only grows during a launch, so barrier k completes
+// when it reaches (k + 1) x blocks. Device-scope fences before the arrival
+// (release: this block's stores reach the coherence point past its XCD's L2)
+// and after the wait (acquire: no stale lines of other XCDs' stores). A wait
+// longer than spin_limit polls sets *err and every block leaves the launch.
+__device__ __forceinline__ bool grid_sync(const PersistArgs& p, unsigned int& target) {
+    target += gridDim.x;
+    int ok = 1;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(p.bar, 1u);
+        long long spins = 0;
+        while (__hip_atomic_load(p.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > p.spin_limit || __hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                atomicExch(p.err, 1);
+                ok = 0;
+                break;
+            }
+        }
+        __threadfence();
+    }
+    return __shfl(ok, 0, 64) != 0;
+}
+
+template <int SCH, int RS, int LIM, typename T>
+__global__ __launch_bounds__(64) void k_persist(PersistArgs p) {
+    const int G = gridDim.x;
+    unsigned int target = 0;
+    const int nstrip = p.counts[0], nghost = p.counts[1], nring = p.counts[2];
+    for (int s = 0; s < p.nsteps; s++) {
+        for (int stage = p.rk2 ? 1 : 0;;) {
+            // strips of every resident solve (TimeAdvance; reference ring fused)
+            for (int w = blockIdx.x; w < nstrip; w += G) {
+                const int2 m = p.smap[w];
+                const StepArgs& a = p.step[m.x];
+                step_strip<SCH, RS, LIM, T, false>(a, m.y % a.nbx, m.y / a.nbx, stage);
+            }
+            if (!grid_sync(p, target)) return;
+            // ComputeDt / mass / clock per solve (RK2 stage 1: make the stage buffer current)
+            for (int q = blockIdx.x; q < p.nslots; q += G)
+                if (p.fin[q].ctl) finish_body<64>(p.fin[q], stage == 1 ? 2 : 1);
+            if (!grid_sync(p, target)) return;
+            // Boundary phase 1 and Global_Boundary
+            if (p.has_fill)
+                for (int w = blockIdx.x; w < nghost; w += G) {
+                    const int2 m = p.gmap[w];
+                    for (int k = threadIdx.x; k < FILL_THREADS; k += 64) fill_one(p.fill[m.x], m.y + k);
+                }
+            for (int w = blockIdx.x; w < nring; w += G) {
+                const int2 m = p.rmap[w];
+                for (int k = threadIdx.x; k < EDGE_THREADS; k += 64) edge_one(p.edge[m.x], m.y + k);
+            }
+            if (!grid_sync(p, target)) return;
+            // Boundary phase 2
+            if (p.has_fill) {
+                for (int w = blockIdx.x; w < nghost; w += G) {
+                    const int2 m = p.gmap[w];
+                    for (int k = threadIdx.x; k < FILL_THREADS; k += 64) scatter_one(p.fill[m.x], m.y + k);
+                }
+                if (!grid_sync(p, target)) return;
+            }
+            if (stage == 1) { stage = 2; continue; }
+            break;
+        }
+    }
+}
+
+__global__ void k_zero_u32(unsigned int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 // Reference Global_Boundary as a separate pass (used with Lax, whose step
 // kernel does not write the ring): reads U_NEW = the new state (ring of U_NEW
 // taken from the previous state for the corners), writes the ring.
@@ -1105,6 +1182,46 @@ hipError_t launch_step_batch(int scheme, int rs, int lim, int prec, const StepAr
     if (scheme == SCH_LAX) return hipErrorInvalidValue;
     BatchLaunch b{slots, map, nblocks, stage};
     return go_s(scheme, rs, lim, prec, nullptr, &b, s);
+}
+// persistent batched step: the kernel for a slot selection, and its resident grid
+template <int SCH, int RS, int LIM>
+static const void* persist_fn(int prec) {
+    if (prec == IBTFO_MIXED) return (const void*)k_persist<SCH, RS, LIM, float>;
+    return (const void*)k_persist<SCH, RS, LIM, double>;
+}
+static const void* persist_kernel(int scheme, int rs, int lim, int prec) {
+    auto by_lim = [&](auto sch, auto r) -> const void* {
+        constexpr int S = decltype(sch)::value, R = decltype(r)::value;
+        if (lim == LIM_MINMOD) return persist_fn<S, R, LIM_MINMOD>(prec);
+        if (lim == LIM_VANLEER) return persist_fn<S, R, LIM_VANLEER>(prec);
+        return persist_fn<S, R, LIM_VANALBADA>(prec);
+    };
+    auto by_rs = [&](auto sch) -> const void* {
+        if (rs == RS_SW) return by_lim(sch, std::integral_constant<int, RS_SW>());
+        if (rs == RS_PVRS) return by_lim(sch, std::integral_constant<int, RS_PVRS>());
+        return by_lim(sch, std::integral_constant<int, RS_HLL>());
+    };
+    if (scheme == SCH_MHM) return by_rs(std::integral_constant<int, SCH_MHM>());
+    if (scheme == SCH_GODUNOV) return by_rs(std::integral_constant<int, SCH_GODUNOV>());
+    return by_rs(std::integral_constant<int, SCH_MUSCL>());
+}
+int persist_blocks(int scheme, int rs, int lim, int prec) {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (scheme == SCH_LAX) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persist_kernel(scheme, rs, lim, prec), 64, 0) !=
+            hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return per_cu * cus;
+}
+hipError_t launch_persist(int scheme, int rs, int lim, int prec, const PersistArgs& p, int blocks, hipStream_t s) {
+    if (scheme == SCH_LAX || blocks <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, p.bar, 1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    void* args[] = {(void*)&p};
+    return hipLaunchKernel(persist_kernel(scheme, rs, lim, prec), dim3(blocks), dim3(64), args, 0, s);
 }
 hipError_t launch_finish_batch(const FinishArgs* f, int nslots, int advance, hipStream_t s) {
     hipLaunchKernelGGL(k_finish_batch, dim3(nslots), dim3(FINISH_THREADS), 0, s, f, nslots, advance);

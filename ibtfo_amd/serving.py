@@ -21,7 +21,7 @@ class BatchConfig(C.Structure):
     _fields_ = [("scheme", C.c_int), ("riemann", C.c_int), ("limiter", C.c_int), ("boundary", C.c_int),
                 ("time", C.c_int), ("precision", C.c_int), ("quirks", C.c_int), ("hbm_budget", C.c_longlong),
                 ("max_slots", C.c_int), ("max_step_blocks", C.c_int), ("max_ghost_blocks", C.c_int),
-                ("max_ring_blocks", C.c_int), ("rows", C.c_int), ("chunk", C.c_int)]
+                ("max_ring_blocks", C.c_int), ("rows", C.c_int), ("chunk", C.c_int), ("persistent", C.c_int)]
 
 
 class Request(C.Structure):
@@ -54,7 +54,7 @@ def _bind():
     L.ibtfo_batch_clock.argtypes = [C.c_void_p]
     for n in ["ibtfo_batch_config_default", "ibtfo_batch_create", "ibtfo_batch_destroy", "ibtfo_batch_submit",
               "ibtfo_batch_progress", "ibtfo_batch_poll", "ibtfo_batch_result", "ibtfo_batch_release",
-              "ibtfo_batch_stats_get"]:
+              "ibtfo_batch_stats_get", "ibtfo_batch_autotune"]:
         getattr(L, n).restype = C.c_int
     _bound = True
 
@@ -106,6 +106,45 @@ class Batch:
 
     def clock(self):
         return lib().ibtfo_batch_clock(self.h)
+
+    def autotune(self, items, probe_chunks=2):
+        """Picks the batch's strip height and launch mode (persistent kernel or
+        one launch per phase) by timing the sample `items` (submit() keyword
+        dicts) resident together; the batch must be idle. Returns (rows,
+        persistent, ms per batched step)."""
+        reqs, keep = [], []
+        for it in items:
+            r, k = self._request(**it)
+            reqs.append(r)
+            keep.append(k)
+        arr = (Request * len(reqs))(*reqs)
+        rows, per, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().ibtfo_batch_autotune(self.h, arr, len(reqs), int(probe_chunks), C.byref(rows), C.byref(per),
+                                          C.byref(ms)), "ibtfo_batch_autotune")
+        self.bc.rows, self.bc.persistent = rows.value, per.value
+        return rows.value, per.value, ms.value
+
+    def _request(self, cfg, levelset=("none", ()), init=("uniform", (1.0, 0.0, 0.0, 1.0)), keep_state=False,
+                 arrival=-1.0, phi=None):
+        """A Request struct and the arrays it points to (kept alive by the caller)."""
+        from . import INIT, LEVELSET
+        r = Request()
+        r.cfg = cfg
+        ls_kind, ls_prm = levelset
+        r.ls_kind = LEVELSET[ls_kind] if isinstance(ls_kind, str) else int(ls_kind)
+        lp, r.ls_nprm = _arr(ls_prm)
+        r.ls_prm = lp.ctypes.data_as(_D) if lp is not None else None
+        ph = None
+        if phi is not None:
+            ph = np.ascontiguousarray(np.asarray(phi, dtype=np.float64))
+            r.phi = ph.ctypes.data_as(_D)
+        ik, iprm = init
+        r.init_kind = INIT[ik] if isinstance(ik, str) else int(ik)
+        ip, r.init_nprm = _arr(iprm)
+        r.init_prm = ip.ctypes.data_as(_D) if ip is not None else None
+        r.keep_state = int(bool(keep_state))
+        r.arrival = float(arrival)
+        return r, (lp, ph, ip)
 
     def submit(self, cfg, levelset=("none", ()), init=("uniform", (1.0, 0.0, 0.0, 1.0)), keep_state=False,
                arrival=-1.0, phi=None):

@@ -215,7 +215,7 @@ int ibtfo_set_rows(ibtfo_ctx* ctx, int rows);
 /* current rows per strip and the step grid (blocks in x and y) */
 int ibtfo_tile(const ibtfo_ctx* ctx, int* rows, int* nbx, int* nby);
 /* Tile-shape autotuner: times `probe_steps` steps of the current solve for each
- * candidate rows value, restoring the state and clock after each probe, and
+ * candidate rows value (1 .. 256), restoring the state and clock after each probe, and
  * keeps the fastest. Allocates a temporary snapshot of both state buffers.
  * Fails with IBTFO_E_STATE when the run would end within a probe. */
 int ibtfo_autotune(ibtfo_ctx* ctx, int probe_steps, int* best_rows, double* best_ms_per_step);
@@ -253,6 +253,9 @@ typedef struct ibtfo_batch_config {
     int max_ring_blocks;   /* edge-ring grid capacity (256 ring cells per block) */
     int rows;              /* strip height of every solve (0: 16) */
     int chunk;             /* steps per graph replay */
+    int persistent;        /* 1: each replay is one persistent kernel that runs the chunk's steps of
+                              every resident solve, its phases separated by grid barriers (the grid
+                              sized to what the device holds resident); 0: one launch per phase */
 } ibtfo_batch_config;
 
 typedef struct ibtfo_request {
@@ -284,11 +287,20 @@ typedef struct ibtfo_batch_stats {
 } ibtfo_batch_stats;
 
 /* Default batch settings: FP64 MUSCL / HLL / minmod / Regular, 8 GiB budget,
- * 256 slots, capacities for that budget, rows 16, chunk 50. */
+ * 256 slots, capacities for that budget, rows 16, chunk 50, one launch per phase. */
 int ibtfo_batch_config_default(ibtfo_batch_config* bc);
 /* Reserves the arena, the device tables and two non-blocking streams. */
 int ibtfo_batch_create(const ibtfo_batch_config* bc, ibtfo_batch** out);
 int ibtfo_batch_destroy(ibtfo_batch* b);
+/* Tile autotuner of the batch engine: with the batch idle (nothing resident or
+ * queued), makes the sample requests resident (as many as fit; each run without a
+ * stop for the probe), times `probe_chunks` graph replays of a chunk for each
+ * strip height (4, 8, 16, 32 rows) with one launch per phase and with the
+ * persistent kernel, discards the sample, and sets the batch's rows and mode to
+ * the fastest. The best rows, mode and milliseconds per batched step are
+ * returned (pointers may be NULL). */
+int ibtfo_batch_autotune(ibtfo_batch* b, const ibtfo_request* sample, int n, int probe_chunks, int* best_rows,
+                         int* best_persistent, double* best_ms_per_step);
 /* Seconds since the batch was created (the clock of arrivals and results). */
 double ibtfo_batch_clock(const ibtfo_batch* b);
 /* Queues a request (parameters are copied); *ticket identifies it. Rejects

@@ -204,3 +204,64 @@ def test_request_over_the_grid_capacity_is_refused_or_retired():
         else:
             assert r.status == 0 and r.iter == items[i]["cfg"].max_iter
     b.close()
+
+
+@pytest.mark.parametrize("prec,time", [("FP64", "SingleStep"), ("mixed", "SingleStep"), ("FP64", "RK2")])
+def test_persistent_kernel_equals_per_phase_launches(prec, time):
+    """The persistent batched step (one launch per chunk: every resident solve's
+    steps, phases separated by grid barriers) gives the results of the
+    one-launch-per-phase sequence bit for bit, with requests arriving while
+    chunks run and finishing at different times."""
+    from ibtfo_amd import serving as S
+    st, items = _items(prec, n=10, steps=36, seed=5, time=time)
+
+    def run(persistent):
+        b = S.Batch(_bc(prec, time, persistent=persistent, chunk=8))
+        try:
+            tickets = {}
+            for i in range(4):
+                tickets[b.submit(**items[i])] = i
+            b.progress()
+            for i in range(4, len(items)):
+                tickets[b.submit(**items[i])] = i
+                b.progress()
+            b.drain()
+            assert b.stats().finished == len(items)
+            return {i: b.result(t) for t, i in tickets.items()}
+        finally:
+            b.close()
+
+    print("persistent vs per-phase batch, %s %s" % (prec, time), flush=True)
+    a, p = run(0), run(1)
+    for i in range(len(items)):
+        (ra, Ua), (rp, Up) = a[i], p[i]
+        assert ra.status == rp.status == 0 and ra.iter == rp.iter == items[i]["cfg"].max_iter
+        assert ra.dt == rp.dt and ra.now_t == rp.now_t
+        assert nan_equal_bits(Ua, Up) == 0, i
+
+
+def test_batch_autotune_keeps_results():
+    """The batch autotuner times the sample resident together for each strip
+    height and launch mode, discards it and keeps the fastest; the batch then
+    serves requests whose states equal their runs with the default tile, bit
+    for bit (the strip shape changes only the order of the mass diagnostic's
+    partial sums, so mass is compared to rounding)."""
+    from ibtfo_amd import serving as S
+    st, items = _items("FP64", n=6, steps=24, seed=9)
+    b = S.Batch(_bc("FP64"))
+    try:
+        print("batch autotune", flush=True)
+        rows, per, ms = b.autotune(items[:4], probe_chunks=1)
+        assert rows in (4, 8, 16, 32) and per in (0, 1) and ms > 0
+        assert b.stats().active == 0 and b.stats().arena_used == 0
+        tickets = {b.submit(**it): i for i, it in enumerate(items)}
+        b.drain()
+        got = {i: b.result(t) for t, i in tickets.items()}
+    finally:
+        b.close()
+    for i, it in enumerate(items):
+        r0, U0 = _run_alone(it, "FP64")
+        r, U = got[i]
+        assert r.status == 0 and r.iter == r0.iter and r.dt == r0.dt and r.now_t == r0.now_t
+        assert abs(r.mass - r0.mass) <= 1e-12 * abs(r0.mass)
+        assert nan_equal_bits(U, U0) == 0, i
