@@ -160,6 +160,10 @@ struct ReduceArgs {
     int max_iter;
     int nblocks;
     int reset;  // 1: set iter = 0, now_t = 0 (start of a run)
+    // when cur_ctl is set, the state is ubuf[cur_ctl->cur], chosen on the
+    // device (no host read of the current buffer); U is then unused
+    const double* ubuf[2];
+    const Ctl* cur_ctl;
 };
 
 // the dt / mass reduction over the step kernel's per-block partials

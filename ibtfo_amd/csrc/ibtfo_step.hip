@@ -722,7 +722,7 @@ __global__ __launch_bounds__(64) void k_step_lax(StepArgs a) {
 // CaculateMass.cpp:27-38); with `reset`, also starts the run clock.
 __global__ __launch_bounds__(64) void k_reduce(ReduceArgs a) {
     const Layout L = a.L;
-    const double* Us = a.U;
+    const double* Us = a.cur_ctl ? a.ubuf[a.cur_ctl->cur] : a.U;
     const int nint = L.nx * L.ny;
     double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
     int bad = 0;

@@ -199,6 +199,18 @@ int ibtfo_log(ibtfo_ctx* ctx, int count, double* dt, double* t, double* mass);
 int ibtfo_compute_dt(ibtfo_ctx* ctx, double* dt);
 /* CaculateMass on the current state (synchronises). */
 int ibtfo_mass(ibtfo_ctx* ctx, double* mass);
+/* No counterpart in the reference: for callers that cannot allocate or
+ * synchronise (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). ibtfo_compute_dt + ibtfo_mass of the current state, queued on the
+ * solve's stream: out (device memory, 3 doubles) gets {dt, mass, 1 if a
+ * non-physical fluid state was seen else 0}, bit for bit what ibtfo_compute_dt
+ * and ibtfo_mass return. Allocates nothing and does not synchronise: the block
+ * partials and control record live in a fixed device array of the library
+ * (about 360 KB per GPU, held until the process ends), so at most 4096 x 256 =
+ * 1,048,576 interior cells (larger solves: IBTFO_E_ARG). Calls that overlap on
+ * one GPU share that array and race. Needs ibtfo_prepare first; the current
+ * device must be the solve's. */
+int ibtfo_compute_dt_noalloc(ibtfo_ctx* ctx, double* out);
 /* Boundary (the selected variant) once on the current state. */
 int ibtfo_boundary(ibtfo_ctx* ctx);
 
