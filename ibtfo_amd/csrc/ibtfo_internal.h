@@ -17,7 +17,7 @@
 namespace ibtfo {
 
 constexpr int LOGCAP = 4096;  // per-step log ring (dt, time, mass)
-constexpr int STRIP_W = 62;   // interior columns updated by one 64-lane block (lanes 1..62)
+constexpr int STRIP_W = 60;   // interior columns updated by one 64-lane block (lanes 2..61)
 
 struct Layout {
     int nx, ny, ng, M, N;

@@ -6,15 +6,16 @@
 // (now_t += dt, iter++, stop test), for MUSCL / MUSCL-Hancock / Godunov with
 // HLL(=HLLC) / PVRS / Steger-Warming and minmod / VanLeer / VanAlbada.
 //
-// Work decomposition: a 64-lane block owns a strip of 62 interior columns and
+// Work decomposition: a 64-lane block owns a strip of 60 interior columns and
 // `rows` interior rows and marches upward through it. Lane t holds column
-// c = cb - 1 + t (lanes 0 and 63 are the strip's halo columns). Per row, each
-// lane reconstructs both face states of its own cell in x (neighbours through
-// LDS) and in y (a 3-row register window), so each limiter pair is evaluated
-// once; it takes the right state of face c+1/2 from lane t+1 and the x flux of
-// face c-1/2 from lane t-1 through LDS, and carries the y flux of face j-1/2
-// and its cell's upper y state from the previous row. Each state value is read
-// from HBM once per step (halo re-reads hit L2) and written once.
+// c = cb - 2 + t (lanes 0, 1, 62 and 63 are the strip's halo columns). Per
+// row, each lane reconstructs both face states of its own cell in x
+// (neighbours through LDS) and in y (a 3-row register window), so each limiter
+// pair is evaluated once; it takes the right state of face c+1/2 from lane t+1
+// and the x flux of face c-1/2 from lane t-1 through LDS, and carries the y
+// flux of face j-1/2 and its cell's upper y state from the previous row in its
+// own LDS slots. A strip reads its cells plus two halo rows above and below
+// and two halo columns on each side, and writes each of its cells once.
 #include <type_traits>
 
 #include "ibtfo_internal.h"
@@ -426,17 +427,18 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
     const int ng = L.ng, nx = L.nx, ny = L.ny;
 
     const int cb = ng + STRIP_W * bx;  // first interior column of the strip
-    const int c = cb - 1 + lane;
+    const int c = cb - 2 + lane;
     const int cl = min(max(c, 0), L.M - 1);
     const int j0 = ng + a.rows * by;
     const int j1 = min(j0 + a.rows, ny + ng);
-    // lanes 0 and 1 also stage the outer halo columns cb-2 and cb+63
-    const int ecol = min(max(lane == 0 ? cb - 2 : cb + 63, 0), L.M - 1);
-    const int eidx = lane == 0 ? 0 : 65;
 
-    __shared__ double rb[2][4][66];  // row values of columns cb-2 .. cb+63
+    __shared__ double rb[2][4][64];  // row values of columns cb-2 .. cb+61
     __shared__ double hb[4][64];     // lower-face states of the x pairs
     __shared__ T fb[4][64];          // x-face fluxes
+    // carried from one row to the next, per lane: kept in LDS rather than in
+    // registers (fewer VGPRs: four waves per SIMD instead of three)
+    __shared__ T gpb[4][64];         // y flux of the lane's cell's lower face
+    __shared__ double ylb[4][64];    // the lane's cell state on its upper y face
 
     // reference Global_Boundary corners (Test/Sod/Global_Boudary.cpp:62-75 applied to
     // ring rows): U_OLD(i<ng or i>=N_x+ng, j ring) = U_NEW(ng or N_x+ng-1, j)
@@ -455,9 +457,8 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
     }
 
     // column window: w[0..2] = rows j, j+1, j+2 of column c at iteration j;
-    // ylo = the y state of row j on its upper face (from the previous pair)
-    double w[3][4], ex[4] = {0, 0, 0, 0};
-    double ylo[4];
+    // ylb = the y state of row j on its upper face (from the previous pair)
+    double w[3][4];
     {
         double wm[4];
         long long o = L.at(cl, j0 - 2);
@@ -469,52 +470,39 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
 #pragma unroll
             for (int k = 0; k < 4; k++) w[r][k] = Ui[o + k * PL];
         }
-        double hi_unused[4];
+        double hi_unused[4], ylo[4];
         cell_states<SCH, LIM, T>(wm, w[0], w[1], ylo, hi_unused, a.quirks);
-    }
-    if (lane < 2) {
-        long long o = L.at(ecol, j0 - 1);
 #pragma unroll
-        for (int k = 0; k < 4; k++) ex[k] = Ui[o + k * PL];
+        for (int k = 0; k < 4; k++) { ylb[k][lane] = ylo[k]; gpb[k][lane] = T(0); }
     }
 
-    T Gp[4] = {0, 0, 0, 0};
     double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
     int bad = 0;
-    const bool upd_col = lane >= 1 && lane <= STRIP_W && c < nx + ng;
-    const int ln = min(lane + 1, 63);
+    const bool upd_col = lane >= 2 && lane < STRIP_W + 2 && c < nx + ng;
+    const int ln = min(lane + 1, 63), lp = max(lane - 1, 0);
 
     for (int j = j0 - 1; j < j1; j++) {
         const int b = j & 1;
         // prefetch the next window row and next halo value (none after the last
         // row: they would only be discarded)
-        double nw[4], nex[4] = {0, 0, 0, 0};
+        double nw[4];
         if (j + 1 < j1) {
             long long o = L.at(cl, min(j + 3, L.N - 1));
 #pragma unroll
             for (int k = 0; k < 4; k++) nw[k] = Ui[o + k * PL];
-            if (lane < 2) {
-                long long oe = L.at(ecol, min(j + 1, L.N - 1));
-#pragma unroll
-                for (int k = 0; k < 4; k++) nex[k] = Ui[oe + k * PL];
-            }
         }
         T F[4] = {0, 0, 0, 0};
         // x faces of row j; the first iteration (j = j0-1) only needs the y face
         // below row j0, unless that face may take the aliased x flux
         if (j >= j0 || (a.quirks && RS != RS_SW)) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) rb[b][k][lane + 1] = w[0][k];
-            if (lane < 2) {
-#pragma unroll
-                for (int k = 0; k < 4; k++) rb[b][k][eidx] = ex[k];
-            }
+            for (int k = 0; k < 4; k++) rb[b][k][lane] = w[0][k];
             __syncthreads();
             double xm[4], xp[4], xlo[4], xhi[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                xm[k] = rb[b][k][lane];
-                xp[k] = rb[b][k][lane + 2];
+                xm[k] = rb[b][k][lp];
+                xp[k] = rb[b][k][ln];
             }
             cell_states<SCH, LIM, T>(xm, w[0], xp, xlo, xhi, a.quirks);
 #pragma unroll
@@ -529,7 +517,12 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
         double ynlo[4], ynhi[4];
         T G[4];
         cell_states<SCH, LIM, T>(w[0], w[1], w[2], ynlo, ynhi, a.quirks);
-        yface<SCH, RS>(g, a.gm1, dtc, a.quirks, w[0], w[1], ylo, ynhi, F, G);
+        {
+            double ylo[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) ylo[k] = ylb[k][lane];
+            yface<SCH, RS>(g, a.gm1, dtc, a.quirks, w[0], w[1], ylo, ynhi, F, G);
+        }
 
         if (j >= j0) {
 #pragma unroll
@@ -547,10 +540,10 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
                         if (FP64) {
                             double Fm = fb[k][lane - 1];
                             double v = w[0][k] - dq(dt * ((double)F[k] - Fm), r_dx);
-                            un[k] = v - dq(dt * ((double)G[k] - (double)Gp[k]), r_dy);
+                            un[k] = v - dq(dt * ((double)G[k] - (double)gpb[k][lane]), r_dy);
                         } else {  // FP32 fluxes, FP64 accumulation
                             double Fm = (double)fb[k][lane - 1];
-                            un[k] = w[0][k] - dtdx * ((double)F[k] - Fm) - dtdy * ((double)G[k] - (double)Gp[k]);
+                            un[k] = w[0][k] - dtdx * ((double)F[k] - Fm) - dtdy * ((double)G[k] - (double)gpb[k][lane]);
                         }
                     }
                     if (stage == 0) {
@@ -585,7 +578,7 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
                         const double Fk = (double)F[k], Fm = (double)fb[k][lane - 1];
-                        const double Gk = (double)G[k], Gm = (double)Gp[k];
+                        const double Gk = (double)G[k], Gm = (double)gpb[k][lane];
                         double v = w[0][k] - dt * ((cc.ae * Fk - cc.aw * Fm) * dy) / (cc.alpha * dx * dy);
                         un[k] = v - dt * ((cc.an * Gk - cc.as * Gm) * dx + wall[k]) / (cc.alpha * dx * dy);
                     }
@@ -639,12 +632,11 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            Gp[k] = G[k];
-            ylo[k] = ynlo[k];
+            gpb[k][lane] = G[k];
+            ylb[k][lane] = ynlo[k];
             w[0][k] = w[1][k];
             w[1][k] = w[2][k];
             w[2][k] = nw[k];
-            ex[k] = nex[k];
         }
     }
     block_partials(a.mass_part, by * a.nbx + bx, amax, umax, vmax, msum, bad);

@@ -221,7 +221,7 @@ int ibtfo_boundary(ibtfo_ctx* ctx);
  * launch in a second pass of `nsteps` steps (may be NULL to skip). */
 int ibtfo_time_steps(ibtfo_ctx* ctx, int nsteps, double* total_ms, double* step_kernel_ms);
 
-/* Step-kernel tile shape: a block updates a strip of 62 columns x `rows` rows.
+/* Step-kernel tile shape: a block updates a strip of 60 columns x `rows` rows.
  * rows = 0 restores the default shape. May allocate (block partials). */
 int ibtfo_set_rows(ibtfo_ctx* ctx, int rows);
 /* current rows per strip and the step grid (blocks in x and y) */
@@ -260,7 +260,7 @@ typedef struct ibtfo_batch_config {
     int scheme, riemann, limiter, boundary, time, precision, quirks; /* shared kernel selection */
     long long hbm_budget;  /* bytes of device memory for admitted solves */
     int max_slots;         /* solves resident at once */
-    int max_step_blocks;   /* step-kernel grid capacity (strips of 62 columns x rows) */
+    int max_step_blocks;   /* step-kernel grid capacity (strips of 60 columns x rows) */
     int max_ghost_blocks;  /* ghost-fill grid capacity (128 ghosts per block) */
     int max_ring_blocks;   /* edge-ring grid capacity (256 ring cells per block) */
     int rows;              /* strip height of every solve (0: 16) */

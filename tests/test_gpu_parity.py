@@ -299,7 +299,7 @@ def test_tile_shapes_and_autotune_bitexact(oracle_mod):
         s = solver()
         s.set_rows(rows)
         r, nbx, nby = s.tile()
-        assert r == min(rows, 100) and nby == -(-100 // r) and nbx == -(-200 // 62)
+        assert r == min(rows, 100) and nby == -(-100 // r) and nbx == -(-200 // 60)
         s.step(12)
         assert s.status()[0] == 12
         assert nan_equal_bits(s.state(), o["U_OLD"]) == 0, rows
