@@ -37,6 +37,10 @@ sys.path.insert(0, ROOT)
 # the CPU legs' OpenMP threads stay on neighbouring cores (read when the OpenMP
 # runtime starts, with the first load of the oracle library)
 os.environ.setdefault("OMP_PROC_BIND", "close")
+# The process's CPU share, read before any OpenMP runtime starts: with
+# OMP_PROC_BIND set, the runtime binds the initial thread to the first place, and
+# the main thread's affinity mask then names that one CPU.
+AFFINITY = sorted(os.sched_getaffinity(0))
 
 def _say(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
@@ -44,7 +48,12 @@ def _say(msg):
 
 BYTES_PER_CELL = 65.0
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = "traffic_step_mach3_mask_cutcell.json"
+STRIP_COLUMNS = 60  # interior columns per step-kernel strip (ibtfo_tile's grid: ceil(N_x / 60) strips across)
+
+
+def traffic_file(riemann):
+    """profiles/ summary of the HBM counter passes for a flux's step kernel."""
+    return "traffic_step_mach3_mask_cutcell%s.json" % ("" if riemann in (None, "HLL") else "_" + riemann.lower())
 
 
 def workload_name(args):
@@ -95,6 +104,7 @@ def gpu_leg(args, precision="FP64", keep_state=False, riemann=None, boundary=Non
     elif not args.no_tune:
         tuned = s.autotune(args.tune_steps)
     rows, nbx, nby = s.tile()
+    form = "row-parallel" if s.tile_form() else "strip"
     if args.warmup:
         s.step(args.warmup)
         s.status()
@@ -122,15 +132,16 @@ def gpu_leg(args, precision="FP64", keep_state=False, riemann=None, boundary=Non
     value = cells * args.steps / (total_ms / 1e3)
     achieved = BYTES_PER_CELL * cells / (kern_ms / 1e3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
-    if os.path.exists(tpath) and precision == "FP64" and riemann is None:
+    tpath = os.path.join(ROOT, "profiles", traffic_file(riemann or args.riemann))
+    if os.path.exists(tpath) and precision == "FP64" and boundary is None:
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("workload") == workload_name(args):
+        if tr.get("workload") == workload_name(argparse.Namespace(**dict(vars(args), riemann=riemann or args.riemann))):
             traffic = tr.get("bytes_per_launch")
     out = dict(value=value, ms_per_step=ms_step, kernel_ms=kern_ms, achieved=achieved, traffic=traffic,
                iter=it, t=t, dt=dt, status=rc, nghost=nghost, ncut=ncut, ngroups=ngroups, rows=rows,
                slice_rates=slice_rates, median=slice_rates[len(slice_rates) // 2], geometry_ms=geometry_ms, grid=[nbx, nby],
+               form=form,
                tuned=tuned is not None, chunk=chunk, slices=slices)
     if keep_state:
         out["rho"] = s.state()[0::4].copy()
@@ -170,7 +181,7 @@ def cpu_leg(args, phi):
     reference's multicore build would be the same number."""
     from oracle import oracle as O
     nx, ny = args.nx, args.ny
-    affinity = sorted(os.sched_getaffinity(0))
+    affinity = AFFINITY
     phys = _physical_cores()
     threads = _cpu_threads(args)
     O.lib().or_set_num_threads(threads)
@@ -242,7 +253,7 @@ def _cfg_frac(cells, steps, ms, stages=1):
 def _cpu_threads(args):
     """Threads for the CPU legs: this process's CPU share (affinity mask, and
     OMP_NUM_THREADS where the environment sets it), within the physical cores."""
-    affinity = len(os.sched_getaffinity(0))
+    affinity = len(AFFINITY)
     t = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or affinity
     phys = _physical_cores()
     return min(t, affinity, phys or t)
@@ -281,6 +292,7 @@ def config_legs(args):
         ms, _ = s.time_steps(n, kernel=False)
         it, t, dt, rc = s.status()
         s.tuned_rows = rows
+        s.tuned_form = "row-parallel" if s.tile_form() else "strip"
         return s, n, ms, t, rc
 
     def cpu_run(case, nsteps):
@@ -323,7 +335,7 @@ def config_legs(args):
     _say("Sod tube measured")
     out.append(dict(workload="Sod tube 400x100 on [0,1]x[0,0.25], MHM + minmod + HLLC, Regular, to t = 0.2",
                     steps=n, gpu_value=400 * 100 * n / (ms / 1e3), gpu_ms=ms, gpu_us_per_step=1e3 * ms / n,
-                    roofline_frac=_cfg_frac(400 * 100, n, ms), tile_rows=s.tuned_rows, status=rc,
+                    roofline_frac=_cfg_frac(400 * 100, n, ms), tile_rows=s.tuned_rows, tile_form=s.tuned_form, status=rc,
                     cpu_value=400 * 100 * cn / cs, cpu_steps=cn, cpu_cores=threads, cpu_reference=ref))
     # inclined Sod channel, 800x400, IDW
     cfg = ib.config(N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Psy_time=0.2, max_iter=10 ** 9, CFL_number=0.5,
@@ -338,7 +350,7 @@ def config_legs(args):
     out.append(dict(workload="inclined Sod channel 800x400 on [0,2]x[0,1] (30 deg, AxisD 0.4), MUSCL + minmod "
                              "+ HLL, IDW ghost fluid, to t = 0.2",
                     steps=n, gpu_value=800 * 400 * n / (ms / 1e3), gpu_ms=ms, gpu_us_per_step=1e3 * ms / n,
-                    roofline_frac=_cfg_frac(800 * 400, n, ms), tile_rows=s.tuned_rows, status=rc,
+                    roofline_frac=_cfg_frac(800 * 400, n, ms), tile_rows=s.tuned_rows, tile_form=s.tuned_form, status=rc,
                     cpu_value=800 * 400 * cn / cs, cpu_steps=cn, cpu_cores=threads,
                     cpu_note="first %d steps of the run" % cn, cpu_reference=ref))
     # isentropic vortex 1024^2, FP64 and mixed
@@ -354,7 +366,7 @@ def config_legs(args):
         s, n, ms, t, rc = gpu_run(cfg, ("none", ()), ("shu", (5.0, 5.0, 5.0, 1.0, 1.0, 1.0, 1.0)), False)
         rho = s.state().reshape(nv + 4, nv + 4, 4)[2:-2, 2:-2, 0]
         s.close()
-        vort[prec] = dict(steps=n, ms=ms, t=t, rc=rc, rows=s.tuned_rows,
+        vort[prec] = dict(steps=n, ms=ms, t=t, rc=rc, rows=s.tuned_rows, form=s.tuned_form,
                           l1=float(np.mean(np.abs(rho - W.shu_exact(X, Y, t)))), rho=rho)
     _say("isentropic vortex on the GPU measured")
     c = O.Case(N_x=nv, N_y=nv, hi_x=10.0, hi_y=10.0, Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer",
@@ -365,7 +377,7 @@ def config_legs(args):
                              "periodic, MUSCL + VanLeer + HLL, RK2 (Heun), to t = 10",
                     steps=f["steps"], gpu_value=nv * nv * f["steps"] / (f["ms"] / 1e3), gpu_ms=f["ms"],
                     gpu_us_per_step=1e3 * f["ms"] / f["steps"],
-                    roofline_frac=_cfg_frac(nv * nv, f["steps"], f["ms"], stages=2), tile_rows=f["rows"],
+                    roofline_frac=_cfg_frac(nv * nv, f["steps"], f["ms"], stages=2), tile_rows=f["rows"], tile_form=f["form"],
                     status=f["rc"], mixed_value=nv * nv * m["steps"] / (m["ms"] / 1e3),
                     l1_rho_exact_fp64=f["l1"], l1_rho_exact_mixed=m["l1"], l1_ratio=m["l1"] / f["l1"],
                     l1_rho_mixed_vs_fp64=float(np.mean(np.abs(m["rho"] - f["rho"]))),
@@ -535,14 +547,16 @@ def pmc_summary(args):
     out["workload"] = workload_name(args)
     out["tile_rows"] = args.rows or "default"
     rows = " --rows %d" % args.rows if args.rows else ""
+    rs = " --riemann %s" % args.riemann if args.riemann != "HLL" else ""
+    out["riemann"] = args.riemann
     out["commands"] = [
         "cd /tmp && export TMPDIR=/tmp",
         "timeout -s KILL 120 rocprofv3 --pmc FETCH_SIZE -d $OUT/pmc_fetch -o fetch --output-format csv "
-        "-- python3 $REPO/bench.py --pmc-probe --steps 5" + rows,
+        "-- python3 $REPO/bench.py --pmc-probe --steps 5" + rows + rs,
         "timeout -s KILL 120 rocprofv3 --pmc WRITE_SIZE -d $OUT/pmc_write -o write --output-format csv "
-        "-- python3 $REPO/bench.py --pmc-probe --steps 5" + rows,
+        "-- python3 $REPO/bench.py --pmc-probe --steps 5" + rows + rs,
         "python3 bench.py --pmc-fetch $OUT/pmc_fetch --pmc-write $OUT/pmc_write --pmc-out "
-        "profiles/%s%s" % (TRAFFIC_FILE, rows)]
+        "profiles/%s%s%s" % (traffic_file(args.riemann), rows, rs)]
     out["note"] = ("FETCH_SIZE / WRITE_SIZE are KiB per dispatch (mean over the step-kernel dispatches); each is "
                    "scaled by copy_bytes / (1024 * the same counter for a 256 MiB elementwise copy); $REPO is the "
                    "repository root, $OUT the folder the counter runs write to")
@@ -630,8 +644,8 @@ def main():
         w = gpu_leg(args, riemann="StegerWarming")
         sw = {"value": w["value"], "unit": "cell-updates/s", "ms_per_step": w["ms_per_step"],
               "kernel": "k_step<MUSCL,StegerWarming,VanLeer>", "kernel_ms": w["kernel_ms"],
-              "roofline_frac": w["achieved"] / HBM_PEAK_GBS, "tile_rows": w["rows"], "steps_taken": w["iter"],
-              "status": w["status"]}
+              "roofline_frac": w["achieved"] / HBM_PEAK_GBS, "traffic": w["traffic"], "tile_rows": w["rows"],
+              "steps_taken": w["iter"], "status": w["status"]}
         progress("Steger-Warming leg")
     idw = None
     if not args.no_idw and args.boundary != "IDWBaseGFM":
@@ -672,7 +686,8 @@ def main():
                    "N_x": args.nx, "N_y": args.ny, "ghost_cells": g["nghost"], "cut_cells": g["ncut"], "merge_groups": g["ngroups"],
                    "steps_taken": g["iter"], "status": g["status"],
                    "geometry_ms": g["geometry_ms"],
-                   "tile": {"columns": 62, "rows": g["rows"], "grid": g["grid"], "autotuned": g["tuned"]},
+                   "tile": {"columns": STRIP_COLUMNS, "rows": g["rows"], "form": g["form"], "grid": g["grid"],
+                            "autotuned": g["tuned"]},
                    "launch": ("%d timed slices of %s steps, each a captured HIP graph of %d steps replayed %s"
                               % (len(g["slices"]), "/".join(map(str, sorted(set(g["slices"])))), g["chunk"],
                                  "/".join(str(n // g["chunk"]) for n in sorted(set(g["slices"]))) + " times")

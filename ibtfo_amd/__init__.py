@@ -62,7 +62,7 @@ def lib():
                      "ibtfo_initialize", "ibtfo_upload_state", "ibtfo_download_state", "ibtfo_download_geometry",
                      "ibtfo_download_scheme_index", "ibtfo_sch_width", "ibtfo_num_ghosts", "ibtfo_dims",
                      "ibtfo_time_steps", "ibtfo_device_state", "ibtfo_set_stream", "ibtfo_set_rows",
-                     "ibtfo_tile", "ibtfo_autotune", "ibtfo_geometry_mask", "ibtfo_geometry_png",
+                     "ibtfo_set_tile", "ibtfo_tile_ex", "ibtfo_tile", "ibtfo_autotune", "ibtfo_geometry_mask", "ibtfo_geometry_png",
                      "ibtfo_png_decode", "ibtfo_cut_counts", "ibtfo_write_vtk", "ibtfo_vtk_write_host"]:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -256,10 +256,20 @@ class Solver:
     def set_rows(self, rows):
         _check(lib().ibtfo_set_rows(self.h, int(rows)), "ibtfo_set_rows")
 
+    def set_tile(self, rows, row_parallel=False):
+        """rows per strip and the form (row_parallel: one wave per row, 1..7 rows)."""
+        _check(lib().ibtfo_set_tile(self.h, int(rows), int(bool(row_parallel))), "ibtfo_set_tile")
+
     def tile(self):
         r, a, b = C.c_int(), C.c_int(), C.c_int()
         _check(lib().ibtfo_tile(self.h, C.byref(r), C.byref(a), C.byref(b)), "ibtfo_tile")
         return r.value, a.value, b.value
+
+    def tile_form(self):
+        """True when the row-parallel step kernel is selected."""
+        p = C.c_int()
+        _check(lib().ibtfo_tile_ex(self.h, None, None, None, C.byref(p)), "ibtfo_tile_ex")
+        return bool(p.value)
 
     def autotune(self, probe_steps=20):
         """Picks the step kernel's rows per strip by timing; returns (rows, ms/step)."""

@@ -805,6 +805,7 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     a.corner_from_input = c.boundary != IBTFO_REGULAR && c.boundary != IBTFO_SIMPLE_BOUND;
     a.unew_nf = c.boundary == IBTFO_SIMPLE_BOUND ? x->unew0 : nullptr;
     step_grid(L, x->rows_override, &a.nbx, &a.nby, &a.rows);
+    a.rowpar = x->rowpar;
     a.cut = x->cfg.boundary == IBTFO_CUTCELL ? x->cuts : nullptr;
     a.cut_index = x->cut_index;
     if (a.cut) a.type = x->stype;
@@ -1094,14 +1095,22 @@ extern "C" int ibtfo_time_steps(ibtfo_ctx* x, int nsteps, double* total_ms, doub
     return IBTFO_OK;
 }
 
-extern "C" int ibtfo_set_rows(ibtfo_ctx* x, int rows) {
-    if (!x || rows < 0) { set_error("ibtfo_set_rows: bad argument"); return IBTFO_E_ARG; }
+extern "C" int ibtfo_set_tile(ibtfo_ctx* x, int rows, int row_parallel) {
+    if (!x || rows < 0 || (row_parallel != 0 && row_parallel != 1)) {
+        set_error("ibtfo_set_tile: bad argument");
+        return IBTFO_E_ARG;
+    }
+    if (row_parallel && (rows < 1 || rows > ROWPAR_MAX_ROWS || x->cfg.scheme == IBTFO_LAX)) {
+        set_error("ibtfo_set_tile: the row-parallel form takes 1..%d rows and no Lax scheme (rows %d)",
+                  ROWPAR_MAX_ROWS, rows);
+        return IBTFO_E_ARG;
+    }
     int nbx, nby, r;
     step_grid(x->L, rows, &nbx, &nby, &r);
     if (nbx * nby > x->mass_cap) {
         double* p = nullptr;
         if (dalloc(x, &p, PART_STRIDE * (size_t)nbx * nby * sizeof(double)) != hipSuccess) {
-            set_error("ibtfo_set_rows: allocation of %d block partials failed", nbx * nby);
+            set_error("ibtfo_set_tile: allocation of %d block partials failed", nbx * nby);
             return IBTFO_E_NOMEM;
         }
         HIPCHK(hipStreamSynchronize(x->stream));
@@ -1110,10 +1119,13 @@ extern "C" int ibtfo_set_rows(ibtfo_ctx* x, int rows) {
         x->mass_cap = nbx * nby;
     }
     x->rows_override = rows;
+    x->rowpar = row_parallel;
     drop_graph(x);
     build_args(x);
     return IBTFO_OK;
 }
+
+extern "C" int ibtfo_set_rows(ibtfo_ctx* x, int rows) { return ibtfo_set_tile(x, rows, 0); }
 
 extern "C" int ibtfo_tile(const ibtfo_ctx* x, int* rows, int* nbx, int* nby) {
     if (!x) return IBTFO_E_ARG;
@@ -1123,6 +1135,12 @@ extern "C" int ibtfo_tile(const ibtfo_ctx* x, int* rows, int* nbx, int* nby) {
     if (nbx) *nbx = a;
     if (nby) *nby = b;
     return IBTFO_OK;
+}
+
+extern "C" int ibtfo_tile_ex(const ibtfo_ctx* x, int* rows, int* nbx, int* nby, int* row_parallel) {
+    if (!x) return IBTFO_E_ARG;
+    if (row_parallel) *row_parallel = x->rowpar;
+    return ibtfo_tile(x, rows, nbx, nby);
 }
 
 extern "C" int ibtfo_autotune(ibtfo_ctx* x, int probe_steps, int* best_rows, double* best_ms) {
@@ -1152,11 +1170,16 @@ extern "C" int ibtfo_autotune(ibtfo_ctx* x, int probe_steps, int* best_rows, dou
         HIPCHK(hipStreamSynchronize(x->stream));
         return IBTFO_OK;
     };
-    const int keep = x->rows_override;
+    const int keep = x->rows_override, keep_par = x->rowpar;
     // short strips for small grids (latency: a wave's rows run one after the
-    // other), tall ones for large grids (fewer halo rows re-read per row updated)
-    static const int cand[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
-    int best = 0;
+    // other), tall ones for large grids (fewer halo rows re-read per row updated);
+    // then the row-parallel form (each row of a strip on its own wave)
+    struct Cand { int rows, par; };
+    std::vector<Cand> cand;
+    for (int r : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256}) cand.push_back({r, 0});
+    if (x->cfg.scheme != IBTFO_LAX)
+        for (int r = 1; r <= ROWPAR_MAX_ROWS; r++) cand.push_back({r, 1});
+    int best = 0, best_par = 0;
     double bms = 1e300;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     rc = copy_all(true);
@@ -1164,10 +1187,11 @@ extern "C" int ibtfo_autotune(ibtfo_ctx* x, int probe_steps, int* best_rows, dou
         set_error("ibtfo_autotune: event creation failed");
         rc = IBTFO_E_HIP;
     }
-    for (int r : cand) {
+    for (const Cand& cd : cand) {
         if (rc) break;
-        if (r > L.ny) break;
-        if ((rc = ibtfo_set_rows(x, r))) break;
+        const int r = cd.rows;
+        if (r > L.ny) continue;
+        if ((rc = ibtfo_set_tile(x, r, cd.par))) break;
         if ((rc = ibtfo_step(x, 2))) break;
         if (hipEventRecord(e0, x->stream) != hipSuccess) { rc = IBTFO_E_HIP; break; }
         if ((rc = ibtfo_step(x, probe_steps))) break;
@@ -1181,7 +1205,7 @@ extern "C" int ibtfo_autotune(ibtfo_ctx* x, int probe_steps, int* best_rows, dou
         Ctl h;
         if ((rc = read_ctl_head(x, h))) break;
         if (h.done) { set_error("ibtfo_autotune: the run ends within the probe"); rc = IBTFO_E_STATE; break; }
-        if (ms / probe_steps < bms) { bms = ms / probe_steps; best = r; }
+        if (ms / probe_steps < bms) { bms = ms / probe_steps; best = r; best_par = cd.par; }
         if ((rc = copy_all(false))) break;
     }
     if (e0) (void)hipEventDestroy(e0);
@@ -1189,11 +1213,11 @@ extern "C" int ibtfo_autotune(ibtfo_ctx* x, int probe_steps, int* best_rows, dou
     if (rc) {
         (void)copy_all(false);
         (void)hipFree(snap);
-        (void)ibtfo_set_rows(x, keep);
+        (void)ibtfo_set_tile(x, keep, keep_par);
         return rc;
     }
     (void)hipFree(snap);
-    if ((rc = ibtfo_set_rows(x, best))) return rc;
+    if ((rc = ibtfo_set_tile(x, best, best_par))) return rc;
     if (best_rows) *best_rows = best;
     if (best_ms) *best_ms = bms;
     return IBTFO_OK;

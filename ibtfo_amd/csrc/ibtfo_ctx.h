@@ -30,6 +30,7 @@ struct ibtfo_ctx {
     int mass_cap;       // blocks
     double* fold;       // FOLD_MAX records of the two-level finish
     int rows_override;  // step-kernel rows per strip (0 = default shape)
+    int rowpar;         // 1: the row-parallel step kernel (rows + 1 waves per block)
     double* corner;
     double Li, Hi, dxc;  // cell-centre origin and spacing (Psy_coord)
     int have_geometry, have_state, prepared;

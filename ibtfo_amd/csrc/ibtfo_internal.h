@@ -18,6 +18,7 @@ namespace ibtfo {
 
 constexpr int LOGCAP = 4096;  // per-step log ring (dt, time, mass)
 constexpr int STRIP_W = 60;   // interior columns updated by one 64-lane block (lanes 2..61)
+constexpr int ROWPAR_MAX_ROWS = 7;  // rows of a row-parallel strip (blocks of at most 8 waves)
 
 struct Layout {
     int nx, ny, ng, M, N;
@@ -80,6 +81,7 @@ struct StepArgs {
                             // 2 = (U^n + U1 + dt L(U1)) / 2, written over U^n in place
     const CutCell* cut;     // cut-cell records (type-3 cells), else null
     const int* cut_index;   // plane: record of each type-3 cell
+    int rowpar;             // row-parallel form (k_step_rowpar: rows + 1 waves per block)
 };
 
 // Small-cell merging after a cut-cell step (one thread per group).

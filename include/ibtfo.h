@@ -222,14 +222,25 @@ int ibtfo_boundary(ibtfo_ctx* ctx);
 int ibtfo_time_steps(ibtfo_ctx* ctx, int nsteps, double* total_ms, double* step_kernel_ms);
 
 /* Step-kernel tile shape: a block updates a strip of 60 columns x `rows` rows.
- * rows = 0 restores the default shape. May allocate (block partials). */
+ * rows = 0 restores the default shape. May allocate (block partials). Same as
+ * ibtfo_set_tile(ctx, rows, 0). */
 int ibtfo_set_rows(ibtfo_ctx* ctx, int rows);
+/* Tile shape and form: row_parallel = 0 is the marching strip (one wave walks
+ * the strip's rows one after the other); row_parallel = 1 gives each of the
+ * strip's rows (1..7) its own wave, plus one for the face below the strip, so a
+ * step's latency is one row's instead of rows + 1 (small grids). Both forms
+ * give the same state bit for bit. Not for the Lax scheme. */
+int ibtfo_set_tile(ibtfo_ctx* ctx, int rows, int row_parallel);
 /* current rows per strip and the step grid (blocks in x and y) */
 int ibtfo_tile(const ibtfo_ctx* ctx, int* rows, int* nbx, int* nby);
+/* the same, and the form (1: row-parallel) */
+int ibtfo_tile_ex(const ibtfo_ctx* ctx, int* rows, int* nbx, int* nby, int* row_parallel);
 /* Tile-shape autotuner: times `probe_steps` steps of the current solve for each
- * candidate rows value (1 .. 256), restoring the state and clock after each probe, and
- * keeps the fastest. Allocates a temporary snapshot of both state buffers.
- * Fails with IBTFO_E_STATE when the run would end within a probe. */
+ * candidate shape (marching strips of 1 .. 256 rows, row-parallel strips of 1 ..
+ * 7 rows), restoring the state and clock after each probe, and keeps the
+ * fastest (ibtfo_tile_ex reports its form). Allocates a temporary snapshot of
+ * both state buffers. Fails with IBTFO_E_STATE when the run would end within a
+ * probe. */
 int ibtfo_autotune(ibtfo_ctx* ctx, int probe_steps, int* best_rows, double* best_ms_per_step);
 
 /* Device pointers of the current state planes (rho, rho u, rho v, E), their
