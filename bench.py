@@ -417,6 +417,13 @@ def serving_leg(args):
                         max_step_blocks=args.serve_step_blocks, max_ghost_blocks=4096, max_ring_blocks=8192,
                         rows=args.serve_rows, chunk=args.serve_chunk)
     _say("plain per-solve path measured")
+    # strip height and launch form of the batch: the library's batch autotuner on a
+    # sample of the stream resident together (probes run without a stop)
+    b = S.Batch(bc)
+    tuned = b.autotune(items32[: args.serve_tune_sample], probe_chunks=2)
+    bc = b.bc
+    b.close()
+    _say("batch tile autotuned: rows %d, %s" % (tuned[0], "persistent kernel" if tuned[1] else "one launch per phase"))
     # saturation: every request queued at t = 0
     b = S.Batch(bc)
     b.submit(**items32[0])
@@ -462,7 +469,9 @@ def serving_leg(args):
         "l1_gate": {"rule": "L1(rho_mixed - rho_ref) <= 1.1 L1(rho_fp64 - rho_ref), rho_ref = 2x-refined FP64 "
                             "solve averaged to the grid", "samples": gate,
                     "passed": all(g["ratio"] <= 1.1 for g in gate)},
-        "batch": {"slots": args.serve_slots, "budget_gib": args.serve_budget, "rows": args.serve_rows,
+        "batch": {"slots": args.serve_slots, "budget_gib": args.serve_budget, "rows": tuned[0],
+                  "launch": "persistent kernel" if tuned[1] else "one launch per phase",
+                  "autotuned": {"sample": min(args.serve_tune_sample, len(items32)), "ms_per_batched_step": tuned[2]},
                   "chunk": args.serve_chunk},
     }
 
@@ -604,7 +613,8 @@ def main():
     ap.add_argument("--serve-budget", type=int, default=64, help="batch HBM budget in GiB")
     ap.add_argument("--serve-slots", type=int, default=256)
     ap.add_argument("--serve-step-blocks", type=int, default=65536)
-    ap.add_argument("--serve-rows", type=int, default=16)
+    ap.add_argument("--serve-rows", type=int, default=16, help="batch strip height before the batch autotuner")
+    ap.add_argument("--serve-tune-sample", type=int, default=256, help="requests the batch autotuner times")
     ap.add_argument("--serve-chunk", type=int, default=50)
     ap.add_argument("--pmc-probe", action="store_true", help="run the hardware-counter workload and exit")
     ap.add_argument("--pmc-fetch", help="rocprofv3 output folder of the FETCH_SIZE pass (summary mode)")

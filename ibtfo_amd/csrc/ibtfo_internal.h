@@ -171,6 +171,7 @@ struct ReduceArgs {
 // the dt / mass reduction over the step kernel's per-block partials
 constexpr int PART_STRIDE = 8;  // doubles per block: max a^2, max |u|, max |v|, mass, flag, pad
 constexpr int FINISH_THREADS = 1024;
+constexpr int FINISH_SMALL_MAX = 512;  // at most this many partial records: the one-wave finish
 struct FinishArgs {
     Ctl* ctl;
     const double* part;

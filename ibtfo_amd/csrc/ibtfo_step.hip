@@ -109,6 +109,40 @@ __device__ __forceinline__ const double* part_rec(const FinishArgs& f, int q) {
     return q < f.nblocks ? f.part + PART_STRIDE * (long long)q : f.part2 + PART_STRIDE * (long long)(q - f.nblocks);
 }
 
+// ComputeDt's dt from the reduced maxima (A2: max squared sound speed), the
+// status flag, and the clock, log and stop test of a step (advance = 1) or the
+// stand-alone ComputeDt's mass and clock start (advance = 0). now_t and iter
+// are the control block's values before this finish.
+__device__ __forceinline__ void finish_clock(const FinishArgs& f, Ctl* ctl, double A2, double Um, double Vm,
+                                             double m, int bad, int advance, double now_t, int iter) {
+    const double A = ib_sqrt(A2);
+    if (bad) ctl->status |= 1;
+    double dt = f.cfl * f.dx / (A + ((Um < Vm) ? Vm : Um));
+    ctl->dt = dt;
+    if (advance) {
+        double tn = now_t + dt;
+        int it = iter + 1;
+        int slot = iter % LOGCAP;
+        ctl->log_dt[slot] = dt;
+        ctl->log_t[slot] = tn;
+        ctl->log_mass[slot] = m;
+        ctl->mass_now = m;
+        ctl->now_t = tn;
+        ctl->iter = it;
+        ctl->done = !(tn < f.t_end && it < f.max_iter);
+        ctl->cur ^= 1;
+        ctl->ran = 1;
+    } else {
+        ctl->mass_now = m;
+        if (f.reset) {
+            ctl->now_t = 0.0;
+            ctl->iter = 0;
+            ctl->done = !(0.0 < f.t_end && 0 < f.max_iter);
+            ctl->ran = 0;
+        }
+    }
+}
+
 template <int NT> __device__ __forceinline__ void finish_body(const FinishArgs& f, const int advance) {
     Ctl* ctl = f.ctl;
     if (advance && ctl->done) return;  // the step kernel did not run
@@ -158,40 +192,60 @@ template <int NT> __device__ __forceinline__ void finish_body(const FinishArgs& 
         }
         __syncthreads();
     }
-    if (t == 0) {
-        A = ib_sqrt(sA[0]);
-        Um = sU[0];
-        Vm = sV[0];
-        m = sM[0];
-        if (sB[0]) ctl->status |= 1;
-        double dt = f.cfl * f.dx / (A + ((Um < Vm) ? Vm : Um));
-        ctl->dt = dt;
-        if (advance) {
-            double tn = ctl->now_t + dt;
-            int it = ctl->iter + 1;
-            int slot = ctl->iter % LOGCAP;
-            ctl->log_dt[slot] = dt;
-            ctl->log_t[slot] = tn;
-            ctl->log_mass[slot] = m;
-            ctl->mass_now = m;
-            ctl->now_t = tn;
-            ctl->iter = it;
-            ctl->done = !(tn < f.t_end && it < f.max_iter);
-            ctl->cur ^= 1;
-            ctl->ran = 1;
-        } else {
-            ctl->mass_now = m;
-            if (f.reset) {
-                ctl->now_t = 0.0;
-                ctl->iter = 0;
-                ctl->done = !(0.0 < f.t_end && 0 < f.max_iter);
-                ctl->ran = 0;
-            }
-        }
-    }
+    if (t == 0) finish_clock(f, ctl, sA[0], sU[0], sV[0], sM[0], sB[0], advance, ctl->now_t, ctl->iter);
 }
 __global__ __launch_bounds__(FINISH_THREADS) void k_finish(FinishArgs f) {
     finish_body<FINISH_THREADS>(f, f.advance);
+}
+
+// The same finish in one wave, for at most FINISH_SMALL_MAX records (small
+// grids, whose step is a chain of latencies): each lane reads its records in
+// one round trip, the control fields are read beside them rather than first,
+// and the lanes reduce through cross-lane shuffles instead of an LDS tree.
+constexpr int FINISH_SMALL_PER_LANE = FINISH_SMALL_MAX / 64;
+__global__ __launch_bounds__(64) void k_finish_small(FinishArgs f) {
+    Ctl* ctl = f.ctl;
+    const int advance = f.advance;
+    if (advance == 2) {
+        if (threadIdx.x == 0) {
+            if (!ctl->done) {
+                ctl->cur ^= 1;
+                ctl->ran = 1;
+            }
+        }
+        return;
+    }
+    const int t = threadIdx.x;
+    const int nrec = f.nblocks + (f.part2 ? f.nblocks2 : 0);
+    const int done = ctl->done;
+    const double now_t = ctl->now_t;
+    const int iter = ctl->iter;
+    double ra[FINISH_SMALL_PER_LANE], ru[FINISH_SMALL_PER_LANE], rv[FINISH_SMALL_PER_LANE], rm[FINISH_SMALL_PER_LANE],
+        rb[FINISH_SMALL_PER_LANE];
+#pragma unroll
+    for (int b = 0; b < FINISH_SMALL_PER_LANE; b++) {
+        const int q = t + 64 * b;
+        const double* r = part_rec(f, q < nrec ? q : 0);
+        ra[b] = r[0]; ru[b] = r[1]; rv[b] = r[2]; rm[b] = r[3]; rb[b] = r[4];
+    }
+    double A = 0.0, Um = 0.0, Vm = 0.0, m = 0.0;
+    int bad = 0;
+#pragma unroll
+    for (int b = 0; b < FINISH_SMALL_PER_LANE; b++) {
+        if (t + 64 * b >= nrec) break;
+        A = (A < ra[b]) ? ra[b] : A;
+        Um = (Um < ru[b]) ? ru[b] : Um;
+        Vm = (Vm < rv[b]) ? rv[b] : Vm;
+        m += rm[b];
+        bad |= rb[b] != 0.0;
+    }
+    A = wave_max(A);
+    Um = wave_max(Um);
+    Vm = wave_max(Vm);
+    m = wave_sum(m);
+    bad = __any(bad);
+    if (advance && done) return;  // the step kernel did not run
+    if (t == 0) finish_clock(f, ctl, A, Um, Vm, m, bad, advance, now_t, iter);
 }
 
 // Partials of a 256-thread block: wave reductions, then the four waves in order.
@@ -783,16 +837,45 @@ template <int SCH, int RS, int LIM, typename T, bool CUT>
 __device__ __forceinline__ void step_rowpar(const StepArgs& a, const int bx, const int by, const int stage) {
     Ctl* ctl = a.ctl;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const Layout L = a.L;
+    const long long PL = L.plane;
+    const int ng = L.ng, nx = L.nx, ny = L.ny;
+    const int cb = ng + STRIP_W * bx;
+    const int c = cb - 2 + lane;
+    const int cl = min(max(c, 0), L.M - 1);
+    const int j0 = ng + a.rows * by;
+    const int j1 = min(j0 + a.rows, ny + ng);
+    const int j = j0 - 1 + wv;
+    const bool live = j < j1;  // the waves past a short last strip only reach the barriers
+
+    // rows j-1 .. j+2 of column c (in range for every live wave: ng >= 2) of both
+    // state buffers, read together with the control block: the rows' addresses
+    // do not depend on which buffer is current, so no load waits for another
+    // (on a latency-bound small grid that is one memory round trip less)
+    double q0[4][4], q1[4][4];
+    if (live) {
+        const long long o = L.at(cl, j - 1);
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                q0[r][k] = a.buf[0][o + r * L.pitch + k * PL];
+                q1[r][k] = a.buf[1][o + r * L.pitch + k * PL];
+            }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) q0[r][k] = q1[r][k] = 0.0;
+    }
     if (ctl->done) {
         if (bx == 0 && by == 0 && threadIdx.x == 0) ctl->ran = 0;
         return;
     }
-    const Layout L = a.L;
     const int cur = ctl->cur;
     const double dt = ctl->dt;
     const double* __restrict__ Ui = a.buf[cur];
     double* __restrict__ Uo = a.buf[cur ^ 1];
-    const long long PL = L.plane;
     GasT<T> g;
     g.gam = (T)a.gam; g.gm1 = (T)a.gm1; g.gp1 = (T)a.gp1; g.c1 = (T)a.c1;
     g.r_gam = Rcp<T>{(T)a.gam, (T)a.r_gam};
@@ -802,15 +885,6 @@ __device__ __forceinline__ void step_rowpar(const StepArgs& a, const int bx, con
     const T dtc = (T)dt;
     const Rcp<double> r_dx{a.dx, a.r_dx}, r_dy{a.dy, a.r_dy}, r_gm1{a.gm1, a.r_gm1};
     const double dtdx = dt * a.r_dx, dtdy = dt * a.r_dy;  // mixed form only
-    const int ng = L.ng, nx = L.nx, ny = L.ny;
-
-    const int cb = ng + STRIP_W * bx;
-    const int c = cb - 2 + lane;
-    const int cl = min(max(c, 0), L.M - 1);
-    const int j0 = ng + a.rows * by;
-    const int j1 = min(j0 + a.rows, ny + ng);
-    const int j = j0 - 1 + wv;
-    const bool live = j < j1;  // the waves past a short last strip only reach the barriers
 
     __shared__ double rb[ROWPAR_MAX_ROWS + 1][4][64];
     __shared__ double hb[ROWPAR_MAX_ROWS + 1][4][64];
@@ -833,20 +907,13 @@ __device__ __forceinline__ void step_rowpar(const StepArgs& a, const int bx, con
         }
     }
 
-    // rows j-1 .. j+2 of column c (in range for every live wave: ng >= 2)
     double wm[4], w0[4], w1[4], w2[4];
-    if (live) {
-        const long long o = L.at(cl, j - 1);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            wm[k] = Ui[o + k * PL];
-            w0[k] = Ui[o + L.pitch + k * PL];
-            w1[k] = Ui[o + 2 * L.pitch + k * PL];
-            w2[k] = Ui[o + 3 * L.pitch + k * PL];
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 4; k++) wm[k] = w0[k] = w1[k] = w2[k] = 0.0;
+    for (int k = 0; k < 4; k++) {
+        wm[k] = cur ? q1[0][k] : q0[0][k];
+        w0[k] = cur ? q1[1][k] : q0[1][k];
+        w1[k] = cur ? q1[2][k] : q0[2][k];
+        w2[k] = cur ? q1[3][k] : q0[3][k];
     }
     const bool xrow = live && (j >= j0 || (a.quirks && RS != RS_SW));
     const int ln = min(lane + 1, 63), lp = max(lane - 1, 0);
@@ -1612,7 +1679,8 @@ hipError_t launch_finish(const FinishArgs& f, hipStream_t s) {
         hipLaunchKernelGGL(k_fold, dim3(g.nfold), dim3(FOLD_THREADS), 0, s, g);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, f);
+    if (nrec <= FINISH_SMALL_MAX) hipLaunchKernelGGL(k_finish_small, dim3(1), dim3(64), 0, s, f);
+    else hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, f);
     return hipGetLastError();
 }
 

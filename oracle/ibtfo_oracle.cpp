@@ -22,8 +22,10 @@
 //   PVRS Y stray store (PVRS/Riemann_solver.cpp:550) is restated with the
 //   sequential (g++ host) ordering; the reference's parallel builds race there.
 //   Functions with no reference counterpart (periodic / inflow edge policies,
-//   isentropic-vortex and x-split Sod initialisers, cut-cell): parity unpinned
-//   by the reference; checked against closed forms only.
+//   isentropic-vortex and x-split Sod initialisers, cut-cell): no reference
+//   output exists; each is pinned by closed forms and invariants instead
+//   (tests/test_oracle_pins.py, tests/test_oracle_properties.py,
+//   tests/test_oracle_cutcell.py).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
