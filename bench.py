@@ -259,7 +259,7 @@ def _cpu_threads(args):
     return min(t, affinity, phys or t)
 
 
-def config_legs(args):
+def config_legs(args, legs=("sod", "channel", "vortex")):
     """The reference's other configurations, the GPU beside the CPU oracle: the
     400x100 Sod tube (MHM + minmod + HLLC, to t = 0.2), the inclined Sod channel
     (800x400, MUSCL + minmod + HLL, IDW ghost fluid, to t = 0.2) and the
@@ -322,58 +322,63 @@ def config_legs(args):
             os.close(saved)
         return dict(value=cells * r["steps"] / el, steps=r["steps"], cores=1, kind="reference", note=note)
 
-    # 400x100 Sod tube
-    cfg = ib.config(N_x=400, N_y=100, hi_x=1.0, hi_y=0.25, Psy_time=0.2, max_iter=10 ** 9, CFL_number=0.5,
-                    Scheme="MHM", Riemann="HLLC", Limiter="minmod", Boundary="Regular")
-    s, n, ms, t, rc = gpu_run(cfg, ("none", ()), ("sod_x", (0.5, 1, 0, 0, 1, 0.125, 0, 0, 0.1)), False)
-    s.close()
-    c = O.Case(N_x=400, N_y=100, hi_y=0.25, Scheme="MHM", Riemann="HLLC", Limiter="minmod", case="sodx",
-               Psy_time=0.2)
-    cn, cs = cpu_run(c, 10 ** 6)
-    c.Riemann = "HLL"  # the reference's HLLC directory holds HLL's code; its build is the HLL one
-    ref = ref_run(c, 10 ** 6, 400 * 100, "the whole run; the x-split Sod hook compiled into the reference's loop")
-    _say("Sod tube measured")
-    out.append(dict(workload="Sod tube 400x100 on [0,1]x[0,0.25], MHM + minmod + HLLC, Regular, to t = 0.2",
+    def sod_leg():
+        # 400x100 Sod tube
+        cfg = ib.config(N_x=400, N_y=100, hi_x=1.0, hi_y=0.25, Psy_time=0.2, max_iter=10 ** 9, CFL_number=0.5,
+                        Scheme="MHM", Riemann="HLLC", Limiter="minmod", Boundary="Regular")
+        s, n, ms, t, rc = gpu_run(cfg, ("none", ()), ("sod_x", (0.5, 1, 0, 0, 1, 0.125, 0, 0, 0.1)), False)
+        s.close()
+        c = O.Case(N_x=400, N_y=100, hi_y=0.25, Scheme="MHM", Riemann="HLLC", Limiter="minmod", case="sodx",
+                   Psy_time=0.2)
+        cn, cs = cpu_run(c, 10 ** 6)
+        c.Riemann = "HLL"  # the reference's HLLC directory holds HLL's code; its build is the HLL one
+        ref = ref_run(c, 10 ** 6, 400 * 100, "the whole run; the x-split Sod hook compiled into the reference's loop")
+        _say("Sod tube measured")
+        return dict(workload="Sod tube 400x100 on [0,1]x[0,0.25], MHM + minmod + HLLC, Regular, to t = 0.2",
                     steps=n, gpu_value=400 * 100 * n / (ms / 1e3), gpu_ms=ms, gpu_us_per_step=1e3 * ms / n,
                     roofline_frac=_cfg_frac(400 * 100, n, ms), tile_rows=s.tuned_rows, tile_form=s.tuned_form, status=rc,
-                    cpu_value=400 * 100 * cn / cs, cpu_steps=cn, cpu_cores=threads, cpu_reference=ref))
-    # inclined Sod channel, 800x400, IDW
-    cfg = ib.config(N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Psy_time=0.2, max_iter=10 ** 9, CFL_number=0.5,
-                    Scheme="MUSCL", Riemann="HLL", Limiter="minmod", Boundary="IDWBaseGFM")
-    s, n, ms, t, rc = gpu_run(cfg, ("channel", (30.0, 0.4)),
-                              ("incline", (30.0, 1, 0, 0, 1, 0.125, 0, 0, 0.1)), True)
-    s.close()
-    c = O.Case(N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Boundary="IDWBaseGFM", case="incline", Psy_time=0.2)
-    cn, cs = cpu_run(c, args.cpu_incline_steps)
-    ref = ref_run(c, args.ref_incline_steps, 800 * 400, "the first %d steps of the run" % args.ref_incline_steps)
-    _say("inclined channel measured")
-    out.append(dict(workload="inclined Sod channel 800x400 on [0,2]x[0,1] (30 deg, AxisD 0.4), MUSCL + minmod "
+                    cpu_value=400 * 100 * cn / cs, cpu_steps=cn, cpu_cores=threads, cpu_reference=ref)
+
+    def channel_leg():
+        # inclined Sod channel, 800x400, IDW
+        cfg = ib.config(N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Psy_time=0.2, max_iter=10 ** 9, CFL_number=0.5,
+                        Scheme="MUSCL", Riemann="HLL", Limiter="minmod", Boundary="IDWBaseGFM")
+        s, n, ms, t, rc = gpu_run(cfg, ("channel", (30.0, 0.4)),
+                                  ("incline", (30.0, 1, 0, 0, 1, 0.125, 0, 0, 0.1)), True)
+        s.close()
+        c = O.Case(N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Boundary="IDWBaseGFM", case="incline", Psy_time=0.2)
+        cn, cs = cpu_run(c, args.cpu_incline_steps)
+        ref = ref_run(c, args.ref_incline_steps, 800 * 400, "the first %d steps of the run" % args.ref_incline_steps)
+        _say("inclined channel measured")
+        return dict(workload="inclined Sod channel 800x400 on [0,2]x[0,1] (30 deg, AxisD 0.4), MUSCL + minmod "
                              "+ HLL, IDW ghost fluid, to t = 0.2",
                     steps=n, gpu_value=800 * 400 * n / (ms / 1e3), gpu_ms=ms, gpu_us_per_step=1e3 * ms / n,
                     roofline_frac=_cfg_frac(800 * 400, n, ms), tile_rows=s.tuned_rows, tile_form=s.tuned_form, status=rc,
                     cpu_value=800 * 400 * cn / cs, cpu_steps=cn, cpu_cores=threads,
-                    cpu_note="first %d steps of the run" % cn, cpu_reference=ref))
-    # isentropic vortex 1024^2, FP64 and mixed
-    nv = 1024
-    h = 10.0 / nv
-    xc = (np.arange(nv) + 0.5) * h
-    X, Y = np.meshgrid(xc, xc)
-    vort = {}
-    for prec in ["FP64", "mixed"]:
-        cfg = ib.config(N_x=nv, N_y=nv, hi_x=10.0, hi_y=10.0, Psy_time=10.0, max_iter=10 ** 9, CFL_number=0.5,
-                        Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer", Boundary="Regular", Time="RK2",
-                        Edge="periodic", quirks=0, precision=ib.PRECISION[prec])
-        s, n, ms, t, rc = gpu_run(cfg, ("none", ()), ("shu", (5.0, 5.0, 5.0, 1.0, 1.0, 1.0, 1.0)), False)
-        rho = s.state().reshape(nv + 4, nv + 4, 4)[2:-2, 2:-2, 0]
-        s.close()
-        vort[prec] = dict(steps=n, ms=ms, t=t, rc=rc, rows=s.tuned_rows, form=s.tuned_form,
-                          l1=float(np.mean(np.abs(rho - W.shu_exact(X, Y, t)))), rho=rho)
-    _say("isentropic vortex on the GPU measured")
-    c = O.Case(N_x=nv, N_y=nv, hi_x=10.0, hi_y=10.0, Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer",
-               Time="RK2", case="shu", edge="periodic", quirks=0, Psy_time=10.0)
-    cn, cs = cpu_run(c, args.cpu_vortex_steps)
-    f, m = vort["FP64"], vort["mixed"]
-    out.append(dict(workload="isentropic vortex 1024x1024 on [0,10]^2 (strength 5, free stream (1,1,1,1)), "
+                    cpu_note="first %d steps of the run" % cn, cpu_reference=ref)
+
+    def vortex_leg():
+        # isentropic vortex 1024^2, FP64 and mixed
+        nv = 1024
+        h = 10.0 / nv
+        xc = (np.arange(nv) + 0.5) * h
+        X, Y = np.meshgrid(xc, xc)
+        vort = {}
+        for prec in ["FP64", "mixed"]:
+            cfg = ib.config(N_x=nv, N_y=nv, hi_x=10.0, hi_y=10.0, Psy_time=10.0, max_iter=10 ** 9, CFL_number=0.5,
+                            Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer", Boundary="Regular", Time="RK2",
+                            Edge="periodic", quirks=0, precision=ib.PRECISION[prec])
+            s, n, ms, t, rc = gpu_run(cfg, ("none", ()), ("shu", (5.0, 5.0, 5.0, 1.0, 1.0, 1.0, 1.0)), False)
+            rho = s.state().reshape(nv + 4, nv + 4, 4)[2:-2, 2:-2, 0]
+            s.close()
+            vort[prec] = dict(steps=n, ms=ms, t=t, rc=rc, rows=s.tuned_rows, form=s.tuned_form,
+                              l1=float(np.mean(np.abs(rho - W.shu_exact(X, Y, t)))), rho=rho)
+        _say("isentropic vortex on the GPU measured")
+        c = O.Case(N_x=nv, N_y=nv, hi_x=10.0, hi_y=10.0, Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer",
+                   Time="RK2", case="shu", edge="periodic", quirks=0, Psy_time=10.0)
+        cn, cs = cpu_run(c, args.cpu_vortex_steps)
+        f, m = vort["FP64"], vort["mixed"]
+        return dict(workload="isentropic vortex 1024x1024 on [0,10]^2 (strength 5, free stream (1,1,1,1)), "
                              "periodic, MUSCL + VanLeer + HLL, RK2 (Heun), to t = 10",
                     steps=f["steps"], gpu_value=nv * nv * f["steps"] / (f["ms"] / 1e3), gpu_ms=f["ms"],
                     gpu_us_per_step=1e3 * f["ms"] / f["steps"],
@@ -382,7 +387,11 @@ def config_legs(args):
                     l1_rho_exact_fp64=f["l1"], l1_rho_exact_mixed=m["l1"], l1_ratio=m["l1"] / f["l1"],
                     l1_rho_mixed_vs_fp64=float(np.mean(np.abs(m["rho"] - f["rho"]))),
                     cpu_value=nv * nv * cn / cs, cpu_steps=cn, cpu_cores=threads,
-                    cpu_note="first %d steps of the run" % cn))
+                    cpu_note="first %d steps of the run" % cn)
+
+    for name, leg in (("sod", sod_leg), ("channel", channel_leg), ("vortex", vortex_leg)):
+        if name in legs:
+            out.append(leg())
     return out
 
 
@@ -617,6 +626,9 @@ def main():
     ap.add_argument("--serve-tune-sample", type=int, default=256, help="requests the batch autotuner times")
     ap.add_argument("--serve-chunk", type=int, default=50)
     ap.add_argument("--pmc-probe", action="store_true", help="run the hardware-counter workload and exit")
+    ap.add_argument("--configs-only", nargs="?", const="sod,channel,vortex", default=None,
+                    help="run only these of the other configurations' legs (comma list of sod, channel, vortex; "
+                         "default all) and print them (no bench line)")
     ap.add_argument("--pmc-fetch", help="rocprofv3 output folder of the FETCH_SIZE pass (summary mode)")
     ap.add_argument("--pmc-write", help="rocprofv3 output folder of the WRITE_SIZE pass (summary mode)")
     ap.add_argument("--pmc-out", help="where the summary mode writes its JSON")
@@ -629,6 +641,9 @@ def main():
         return 2
     if args.pmc_probe:
         return pmc_probe(args)
+    if args.configs_only:  # the other configurations alone (profiling them), one JSON object
+        print(json.dumps({"configs": config_legs(args, set(args.configs_only.split(",")))}))
+        return 0
     t_start = time.perf_counter()
 
     def progress(what):

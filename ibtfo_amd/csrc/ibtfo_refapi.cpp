@@ -5,6 +5,8 @@
 // library's message and aborts, since these signatures have no error channel.
 //
 // Mapping (reference header:line -> here):
+//   Advance       Src/Base/Euler/Advection/Advance.H:14-26  -> as TimeAdvance (SingleStep
+//                 only: SingleStep's TimeAdvance is this call)
 //   TimeAdvance   Src/Base/TimeAdvance/TimeAdvance.H:13-26  -> ibtfo_step_dt(ctx, dt): one
 //                 step with the driver's dt; the fused step also applies Boundary,
 //                 Global_Boundary and ComputeDt
@@ -180,6 +182,23 @@ void TimeAdvance(const double, const double, const int, const int, const int, co
     // SingleStep.cpp:5-20 advances by the dt it is given (a driver may have
     // changed the one ComputeDt returned, e.g. clipped it to Psy_time)
     check(ibtfo_step_dt(b->ctx, dt), "TimeAdvance");
+    b->stepped = true;
+}
+
+// Advance.H:14-26: the scheme's update of U_NEW from U_OLD with the given dt, which
+// is all SingleStep's TimeAdvance does (SingleStep.cpp:5-20), so the same fused
+// step. Under RK2 the reference's Advance is one forward-Euler stage of Heun's
+// step; the device runs whole steps, so that call is refused.
+void Advance(const double, const double, const int, const int, const int, const double, double dt, double* U_OLD,
+             double*, double*, double*, const int ndevices, const int device) {
+    one_device(ndevices, device, "Advance");
+    Bound* b = find(U_OLD, "Advance");
+    if (b->cfg.time != IBTFO_SINGLESTEP) {
+        fprintf(stderr, "ibtfo: Advance: the solve steps with RK2; call TimeAdvance (whole steps)\n");
+        abort();
+    }
+    ensure_prepared(b);
+    check(ibtfo_step_dt(b->ctx, dt), "Advance");
     b->stepped = true;
 }
 

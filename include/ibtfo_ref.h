@@ -45,6 +45,9 @@ void Boundary(const int N_x, const int N_y, const int num_ghost_cell, const doub
               const int device);                                    /* Src/BoundaryCondition/Boundary.H:11-20 */
 void Global_Boundary(const int N_x, const int N_y, const int num_ghost_cell, const double gamma, double* U_OLD,
                      double* U_NEW, const int ndevices, const int device); /* Src/BoundaryCondition/Boundary.H:22-29 */
+void Advance(const double Psy_L, const double Psy_H, const int N_x, const int N_y, const int num_ghost_cell,
+             const double gamma, double dt, double* U_OLD, double* U_TMP, double* U_NEW, double* XYCOORD,
+             const int ndevices, const int device);    /* Src/Base/Euler/Advection/Advance.H:14-26 */
 void TimeAdvance(const double Psy_L, const double Psy_H, const int N_x, const int N_y, const int num_ghost_cell,
                  const double gamma, double dt, double* U_OLD, double* U_TMP, double* U_NEW, double* XYCOORD,
                  double* SCHEME_IDX, const int ndevices,

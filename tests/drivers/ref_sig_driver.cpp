@@ -1,6 +1,6 @@
 // A driver written the way the reference's test drivers are (Test/Sod/Sod.cpp:
 // read the .inp, allocate the host arrays, Psy_coord, Level_Set, Scheme_Index,
-// Initialize, Boundary, then TimeAdvance / Boundary / Global_Boundary /
+// Initialize, Boundary, then TimeAdvance (or Advance) / Boundary / Global_Boundary /
 // ComputeDt / CaculateMass per step), linked against libibtfo_amd.so through
 // the reference signatures of include/ibtfo_ref.h. The four ibtfo_ref_* calls
 // stand where the reference has its OpenACC data directives.
@@ -87,12 +87,18 @@ int main(int argc, char** argv) {
     // clip_last_step = 1: a driver that shortens the last step to end exactly at
     // Psy_time and advances its clock by the dt it stepped with
     const int clip = (int)num(filename, "clip_last_step", 0);
+    // call_advance = 1: step through Advance, as SingleStep's TimeAdvance does
+    const int call_advance = (int)num(filename, "call_advance", 0);
     std::vector<double> log;
     while (now_t < Psy_time && iter < max_iter) {
         double step_dt = dt;
         if (clip && now_t + step_dt > Psy_time) step_dt = Psy_time - now_t;
-        TimeAdvance(Psy_L, Psy_H, N_x, N_y, ng, gamma, step_dt, U_OLD.data(), U_TMP.data(), U_NEW.data(),
-                    XYCOORD.data(), SCHEME_IDX.data(), 1, 0);
+        if (call_advance)
+            Advance(Psy_L, Psy_H, N_x, N_y, ng, gamma, step_dt, U_OLD.data(), U_TMP.data(), U_NEW.data(),
+                    XYCOORD.data(), 1, 0);
+        else
+            TimeAdvance(Psy_L, Psy_H, N_x, N_y, ng, gamma, step_dt, U_OLD.data(), U_TMP.data(), U_NEW.data(),
+                        XYCOORD.data(), SCHEME_IDX.data(), 1, 0);
         Boundary(N_x, N_y, ng, gamma, U_OLD.data(), U_NEW.data(), XYCOORD.data(), SCHEME_IDX.data(), 1, 0);
         Global_Boundary(N_x, N_y, ng, gamma, U_OLD.data(), U_NEW.data(), 1, 0);
         ComputeDt(Psy_L, Psy_H, N_x, N_y, ng, gamma, CFL_number, U_OLD.data(), XYCOORD.data(), &dt, 1, 0);

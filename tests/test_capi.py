@@ -40,6 +40,7 @@ def test_reference_signatures_are_exported():
                 "Scheme_Index(int, int, int, double*, double*, int, int)",
                 "Boundary(int, int, int, double, double*, double*, double*, double*, int, int)",
                 "Global_Boundary(int, int, int, double, double*, double*, int, int)",
+                "Advance(double, double, int, int, int, double, double, double*, double*, double*, double*, int, int)",
                 "TimeAdvance(double, double, int, int, int, double, double, double*, double*, double*, double*, "
                 "double*, int, int)",
                 "ComputeDt(double, double, int, int, int, double, double, double*, double*, double*, int, int)",

@@ -29,6 +29,7 @@ hi_y = 1.0
 Psy_time = {t_end!r}
 max_iter = {steps}
 clip_last_step = {clip}
+call_advance = {adv}
 CFL_number = 0.5
 gamma = 1.4
 Scheme = MUSCL
@@ -54,7 +55,7 @@ def test_reference_signature_driver(oracle_mod, tmp_path, boundary, riemann):
     nx, ny, steps = 128, 64, 40
     PIL.fromarray(W.bodies_mask(nx, ny)).save(tmp_path / "Demo.png")  # uint8 x 4 -> RGBA
     (tmp_path / "case.inp").write_text(INP.format(nx=nx, ny=ny, steps=steps, boundary=boundary, riemann=riemann,
-                                                  t_end=100.0, clip=0))
+                                                  t_end=100.0, clip=0, adv=0))
     out = tmp_path / "out"
     out.mkdir()
     print("ref_sig_driver %s %s" % (boundary, riemann), flush=True)
@@ -110,7 +111,8 @@ def test_ibtfo_run_writes_reference_vtk(oracle_mod, tmp_path):
 def test_time_advance_takes_the_drivers_dt(oracle_mod, tmp_path):
     """TimeAdvance advances by the dt the driver passes (TimeAdvance.H:13-26,
     SingleStep.cpp:5-20): a driver that clips its last step to end at Psy_time
-    gets the oracle's result of the same clipped loop, bit for bit."""
+    gets the oracle's result of the same clipped loop, bit for bit; the same
+    driver stepping through Advance writes the same files."""
     PIL = pytest.importorskip("PIL.Image")
     from ibtfo_amd import workloads as W
     assert os.path.exists(DRIVER), "tests/drivers/ref_sig_driver is not built (run __graft_entry__.build())"
@@ -132,7 +134,7 @@ def test_time_advance_takes_the_drivers_dt(oracle_mod, tmp_path):
     o = O.run_oracle(c, 100, root_mode=1, clip=True)
     assert o["steps"] in (31, 32)
     (tmp_path / "case.inp").write_text(INP.format(nx=nx, ny=ny, steps=100, boundary="IDWBaseGFM", riemann="HLL",
-                                                  t_end=t_end, clip=1))
+                                                  t_end=t_end, clip=1, adv=0))
     out = tmp_path / "out"
     out.mkdir()
     print("ref_sig_driver clipped last step, t_end=%r" % t_end, flush=True)
@@ -148,3 +150,14 @@ def test_time_advance_takes_the_drivers_dt(oracle_mod, tmp_path):
     # the unclipped loop of the same steps differs: the clipped dt was used
     f = O.run_oracle(c, o["steps"], root_mode=1)
     assert nan_equal_bits(f["U_OLD"], o["U_OLD"]) > 0
+    # the same loop through Advance (Advance.H:14-26), which is what SingleStep's
+    # TimeAdvance calls: the same files
+    (tmp_path / "case.inp").write_text(INP.format(nx=nx, ny=ny, steps=100, boundary="IDWBaseGFM", riemann="HLL",
+                                                  t_end=t_end, clip=1, adv=1))
+    out2 = tmp_path / "out_advance"
+    out2.mkdir()
+    print("ref_sig_driver through Advance", flush=True)
+    r = subprocess.run([DRIVER, "case.inp", str(out2)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert (out2 / "U.bin").read_bytes() == (out / "U.bin").read_bytes()
+    assert (out2 / "log.bin").read_bytes() == (out / "log.bin").read_bytes()
