@@ -41,6 +41,32 @@ def test_sodx_bitexact(oracle_mod, scheme, riemann, limiter):
         assert max(rel_linf(g["U_OLD"], p["U_OLD"])) <= REL_TOL
 
 
+def test_inclined_channel_800x400_full(oracle_mod):
+    """The inclined Sod channel at its own size: 800x400 on [0,2]x[0,1] (30 deg,
+    AxisD 0.4), MUSCL + minmod + HLL, IDW ghost fluid, reference mode, CFL 0.5,
+    to t = 0.2 in graph chunks: GPU == oracle bit for bit (state, every step's dt
+    and time). Against the oracle with the reference's pow(x, 0.5) sound speeds
+    the 340 steps end 2.2e-12 apart (relative L-inf): glibc's pow is not
+    correctly rounded, and its last-bit differences from sqrt grow across the
+    shocks over the run; the two CPU oracles differ by exactly that much, the GPU
+    adding nothing. Bound here: 1e-11."""
+    O = oracle_mod
+    c = _case(O, N_x=800, N_y=400, hi_x=2.0, hi_y=1.0, Boundary="IDWBaseGFM", case="incline", CFL_number=0.5,
+              Psy_time=0.2)
+    g = gpu_run(c, 100000, graph_chunk=50)
+    o = O.run_oracle(c, 100000, root_mode=1)
+    print("inclined channel 800x400 steps", g["steps"], o["steps"], "t", g["t"], flush=True)
+    assert g["steps"] == o["steps"] and g["steps"] > 300
+    assert nan_equal_bits(g["XY"][5::6], o["XY"][5::6]) == 0
+    assert nan_equal_bits(g["U_OLD"], o["U_OLD"]) == 0
+    assert nan_equal_bits(g["dt_log"], o["dt"]) == 0
+    assert nan_equal_bits(g["t_log"], o["t"]) == 0
+    p = O.run_oracle(c, 100000, root_mode=0)
+    assert p["steps"] == g["steps"]
+    assert rel_linf(g["U_OLD"], p["U_OLD"]) == rel_linf(o["U_OLD"], p["U_OLD"])
+    assert max(rel_linf(g["U_OLD"], p["U_OLD"])) <= 1e-11
+
+
 @pytest.mark.parametrize("scheme,quirks", [("MUSCL", 1), ("MHM", 0), ("Godunov", 1)])
 def test_vanalbada_bitexact(oracle_mod, scheme, quirks):
     """The VanAlbada limiter as its evident intent (the reference's macro does not
