@@ -613,7 +613,10 @@ extern "C" int ibtfo_batch_autotune(ibtfo_batch* b, const ibtfo_request* sample,
     const ibtfo_batch_config keep = b->bc;
     static const int rows_cand[] = {4, 8, 16, 32};
     int rc = IBTFO_OK, brow = keep.rows, bper = keep.persistent;
-    double bms = 1e300;
+    // candidates are ranked by cell-updates/s: a shorter strip needs more step
+    // blocks per solve, so fewer of the sample fit the grid capacity and the
+    // batched step has less work
+    double bms = 1e300, brate = -1.0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
         set_error("ibtfo_batch_autotune: event creation failed");
@@ -635,6 +638,7 @@ extern "C" int ibtfo_batch_autotune(ibtfo_batch* b, const ibtfo_request* sample,
             b->bc.persistent = per;
             // the sample as resident solves that run for the whole probe
             int ns = 0, ng = 0, nr = 0, used = 0;
+            double cells = 0.0;
             for (int i = 0; i < n && used < b->bc.max_slots && !rc; i++) {
                 ibtfo_config c = sample[i].cfg;
                 c.max_iter = 1 << 30;
@@ -656,6 +660,7 @@ extern "C" int ibtfo_batch_autotune(ibtfo_batch* b, const ibtfo_request* sample,
                     break;
                 }
                 ns += step_blocks(x); ng += ghost_blocks(x); nr += ring_blocks(x);
+                cells += (double)c.N_x * c.N_y;
                 b->slots[used++].ctx = x;
             }
             if (!rc && used == 0) { set_error("ibtfo_batch_autotune: no sample request fits the batch"); rc = IBTFO_E_NOMEM; }
@@ -675,7 +680,8 @@ extern "C" int ibtfo_batch_autotune(ibtfo_batch* b, const ibtfo_request* sample,
             if (rc == IBTFO_E_HIP) set_error("ibtfo_batch_autotune: probe failed (%s)", hipGetErrorString(hipGetLastError()));
             if (rc) break;
             const double per_step = ms / (probe_chunks * (double)b->bc.chunk);
-            if (per_step < bms) { bms = per_step; brow = r; bper = per; }
+            const double rate = cells / per_step;
+            if (rate > brate) { brate = rate; bms = per_step; brow = r; bper = per; }
         }
     }
     if (e0) (void)hipEventDestroy(e0);

@@ -709,7 +709,9 @@ __global__ __launch_bounds__(64) void k_step(StepArgs a) {
 // Global_Boundary ring writes. The same statements as the update inside
 // step_strip, which keeps its own inline copy: factoring it out of the strip
 // kernel changed that kernel's register allocation (the mixed-precision
-// cut-cell variants went from 128 to 165 VGPRs, 4 to 3 waves per SIMD).
+// cut-cell variants went from 128 to 165 VGPRs, 4 to 3 waves per SIMD). The
+// two copies are held equal by the row-parallel tests, which compare the two
+// kernels' states bit for bit over every scheme, flux, boundary and stage.
 template <int SCH, int RS, int LIM, typename T, bool CUT>
 __device__ __forceinline__ void update_cell(const StepArgs& a, const Layout& L, const int c, const int j,
                                             const double (&w0)[4], const T (&F)[4], const T (&G)[4],

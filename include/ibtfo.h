@@ -320,8 +320,9 @@ int ibtfo_batch_destroy(ibtfo_batch* b);
  * stop for the probe), times `probe_chunks` graph replays of a chunk for each
  * strip height (4, 8, 16, 32 rows) with one launch per phase and with the
  * persistent kernel, discards the sample, and sets the batch's rows and mode to
- * the fastest. The best rows, mode and milliseconds per batched step are
- * returned (pointers may be NULL). */
+ * the pair with the most cell-updates/s (a shorter strip needs more step blocks
+ * per solve, so fewer of the sample may fit the grid capacity). The best rows,
+ * mode and milliseconds per batched step are returned (pointers may be NULL). */
 int ibtfo_batch_autotune(ibtfo_batch* b, const ibtfo_request* sample, int n, int probe_chunks, int* best_rows,
                          int* best_persistent, double* best_ms_per_step);
 /* Seconds since the batch was created (the clock of arrivals and results). */
