@@ -252,6 +252,8 @@ hipError_t launch_gather(const FinishArgs* f, int nslots, SlotStatus* out, hipSt
 hipError_t launch_persist(int scheme, int rs, int lim, int prec, const PersistArgs& p, int blocks, hipStream_t s);
 int persist_blocks(int scheme, int rs, int lim, int prec);
 hipError_t launch_step_lax(const StepArgs& a, hipStream_t s);
+// the row-parallel step kernel (StepArgs::rowpar; 1 <= rows <= ROWPAR_MAX_ROWS), ibtfo_rowpar.hip
+hipError_t launch_step_rowpar(int scheme, int rs, int lim, int prec, const StepArgs& a, hipStream_t s);
 hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_from_input, hipStream_t s);
 
 }  // namespace ibtfo

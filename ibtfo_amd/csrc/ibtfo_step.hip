@@ -19,86 +19,9 @@
 #include <type_traits>
 
 #include "ibtfo_internal.h"
+#include "ibtfo_face.h"
 
 namespace ibtfo {
-
-__device__ __forceinline__ double ld_agent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long ld_agent(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        double w = __shfl_xor(v, o, 64);
-        v = (v < w) ? w : v;
-    }
-    return v;
-}
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// ComputeDt.cpp:34-46 contribution of one fluid cell (std::max keeps the old
-// maximum when the candidate is NaN); amax accumulates the squared sound speed.
-__device__ __forceinline__ void dt_terms(double gam, double gm1, const double* U, double& amax, double& umax,
-                                         double& vmax, int& bad) {
-    PState<double> s = pstate(gm1, U);
-    double rho = s.rho, u = s.u, v = s.v, p = s.p;
-    // the maximum sound speed is taken as sqrt of the maximum a^2: RN(sqrt) is
-    // monotone, so this equals the maximum of the rounded sound speeds
-    double a2 = dq(gam * p, s.ir);
-    amax = (amax < a2) ? a2 : amax;
-    double au = ib_fabs(u), av = ib_fabs(v);
-    umax = (umax < au) ? au : umax;
-    vmax = (vmax < av) ? av : vmax;
-    bad |= !(rho > 0.0 && p > 0.0);
-}
-
-// The corrected mode's density / pressure floors on an updated fluid cell fused
-// with its dt_terms: both need 1/rho, u, v and p, which are evaluated once, in
-// pstate's operation order.
-__device__ __forceinline__ void floor_and_dt(double gam, double gm1, const Rcp<double>& r_gm1, double* q,
-                                             int quirks, double& amax, double& umax, double& vmax, int& bad) {
-    if (!quirks && !(q[0] > RHO_FLOOR)) q[0] = RHO_FLOOR;
-    const double rho = q[0];
-    Rcp<double> ir = rcp_of(rho);
-    double u = dq(q[1], ir), v = dq(q[2], ir);
-    double ke = 0.5 * rho * (u * u + v * v);
-    double p = gm1 * (q[3] - ke);
-    if (!quirks && !(p > P_FLOOR)) {
-        q[3] = dq(P_FLOOR, r_gm1) + ke;
-        p = gm1 * (q[3] - ke);
-    }
-    double a2 = dq(gam * p, ir);
-    amax = (amax < a2) ? a2 : amax;
-    double au = ib_fabs(u), av = ib_fabs(v);
-    umax = (umax < au) ? au : umax;
-    vmax = (vmax < av) ? av : vmax;
-    bad |= !(rho > 0.0 && p > 0.0);
-}
-
-// Per-block partial of the dt / mass reductions: {max a^2, max |u|, max |v|,
-// mass, non-physical flag} at part[PART_STRIDE*bid ..], written with plain
-// stores. The one-block k_finish launched after the step reduces them in block
-// order (deterministic), so no grid-wide fence or atomic is needed: the kernel
-// boundary publishes the partials.
-__device__ __forceinline__ void block_partials(double* part, int bid, double amax, double umax, double vmax,
-                                               double msum, int bad) {
-    amax = wave_max(amax);
-    umax = wave_max(umax);
-    vmax = wave_max(vmax);
-    msum = wave_sum(msum);
-    bad = __any(bad);
-    if (threadIdx.x == 0) {
-        double* q = part + PART_STRIDE * (long long)bid;
-        q[0] = amax; q[1] = umax; q[2] = vmax; q[3] = msum; q[4] = bad ? 1.0 : 0.0;
-    }
-}
 
 // ComputeDt (ComputeDt.cpp:23-49) and CaculateMass (CaculateMass.cpp:27-38) from
 // the block partials, then the host loop's bookkeeping of Test/Sod/Sod.cpp:154-177:
@@ -320,134 +243,6 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_finish_batch(const FinishArg
                                                                  int advance) {
     if ((int)blockIdx.x >= nslots || f[blockIdx.x].ctl == nullptr) return;
     finish_body<FINISH_THREADS>(f[blockIdx.x], advance);
-}
-
-// Primitive view of an FP64 conserved state in the compute type T. FP64:
-// pstate itself (correctly rounded, the reference's order). Mixed precision:
-// 1/rho, u, v and p = (gamma-1)(E - rho |u|^2 / 2) are recovered in FP64 (no
-// correct rounding needed) and then rounded to FP32, because near vacuum the
-// FP32 difference E - rho |u|^2 / 2 would lose the pressure to cancellation.
-template <typename T> __device__ __forceinline__ PState<T> prim_of(double gm1, const double* U);
-template <> __device__ __forceinline__ PState<double> prim_of<double>(double gm1, const double* U) {
-    return pstate(gm1, U);
-}
-template <> __device__ __forceinline__ PState<float> prim_of<float>(double gm1, const double* U) {
-    const double rho = U[0];
-    double y = __builtin_amdgcn_rcp(rho);
-    y = __builtin_fma(y, __builtin_fma(-rho, y, 1.0), y);
-    const double u = U[1] * y, v = U[2] * y;
-    const double p = gm1 * (U[3] - 0.5 * rho * (u * u + v * v));
-    PState<float> s;
-    s.rho = (float)rho; s.u = (float)u; s.v = (float)v; s.p = (float)p; s.E = (float)U[3];
-    s.ir = Rcp<float>{s.rho, (float)y};
-    return s;
-}
-
-// Face states of one face from the reconstructed FP64 conserved states Ulo/Uhi
-// (in/out; the cell averages for Godunov) and the two cell averages u1, u2
-// beside it: the MUSCL-Hancock predictor and the corrected mode's positivity
-// fallback. T is the compute type of the primitive states and fluxes (double,
-// or float in the mixed-precision form).
-template <int SCH, bool Y, typename T>
-__device__ __forceinline__ void face_states(const GasT<T>& g, double gm1, T dt, int quirks, const double* u1,
-                                            const double* u2, double* Ulo, double* Uhi, PState<T>& L,
-                                            PState<T>& R) {
-    L = prim_of<T>(gm1, Ulo);
-    R = prim_of<T>(gm1, Uhi);
-    if (SCH == SCH_MHM) {  // MHM_Scheme.cpp:61-96 / :148-183 (same increment on both sides of the face)
-        T Fl[4], Fh[4];
-        phys_flux<Y>(L, Fl);
-        phys_flux<Y>(R, Fh);
-        const Rcp<T>& rh = Y ? g.r_dy : g.r_dx;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            T inc = dq(T(0.5) * (Fl[k] - Fh[k]) * dt, rh);
-            Ulo[k] = Ulo[k] + inc;
-            Uhi[k] = Uhi[k] + inc;
-        }
-        L = prim_of<T>(gm1, Ulo);
-        R = prim_of<T>(gm1, Uhi);
-    }
-    if (SCH != SCH_GODUNOV && !quirks && !(physical(L) && physical(R))) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) { Ulo[k] = u1[k]; Uhi[k] = u2[k]; }
-        L = prim_of<T>(gm1, Ulo);
-        R = prim_of<T>(gm1, Uhi);
-    }
-}
-
-template <bool Y, int RS, typename T>
-__device__ __forceinline__ bool riemann(const GasT<T>& g, const PState<T>& L, const PState<T>& R,
-                                        const double* Ulo, const double* Uhi, T* F, bool quirks, bool& stray) {
-    T ul[4], uh[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) { ul[k] = (T)Ulo[k]; uh[k] = (T)Uhi[k]; }
-    stray = false;
-    if (RS == RS_SW) { sw_face<Y>(g, L, R, F); return false; }
-    if (RS == RS_PVRS) { pvrs_face<Y>(g, ul, uh, F, stray); return false; }
-    return hll_face<Y>(g, L, R, ul, uh, F, quirks);
-}
-
-// x face between cells u1 (left) and u2 (right), reconstructed states UL, UR
-template <int SCH, int RS, typename T>
-__device__ __forceinline__ void xface(const GasT<T>& g, double gm1, T dt, int quirks, const double* u1,
-                                      const double* u2, double* UL, double* UR, T* F) {
-    PState<T> L, R;
-    face_states<SCH, false>(g, gm1, dt, quirks, u1, u2, UL, UR, L, R);
-    bool stray;
-    riemann<false, RS>(g, L, R, UL, UR, F, quirks != 0, stray);
-}
-
-// y face between cells u1 (below) and u2 (above); Fx is the lower cell's x-face
-// flux in the same row, which is what the reference leaves in the aliased G_OLD
-// slot when its y solver stores nothing (HLL/Riemann_solver.cpp:224-226,
-// PVRS/Riemann_solver.cpp:550).
-template <int SCH, int RS, typename T>
-__device__ __forceinline__ void yface(const GasT<T>& g, double gm1, T dt, int quirks, const double* u1,
-                                      const double* u2, double* UD, double* UU, const T* Fx, T* G) {
-    PState<T> D, Uu;
-    face_states<SCH, true>(g, gm1, dt, quirks, u1, u2, UD, UU, D, Uu);
-    bool stray;
-    bool up = riemann<true, RS>(g, D, Uu, UD, UU, G, quirks != 0, stray);
-    if (quirks) {
-        if (RS == RS_PVRS && stray) G[3] = Fx[3];
-        if (RS == RS_HLL && up) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) G[k] = Fx[k];
-        }
-    }
-}
-
-// Both reconstructed states of a cell in one direction (cell averages for Godunov)
-template <int SCH, int LIM, typename T>
-__device__ __forceinline__ void cell_states(const double* um, const double* u0, const double* up, double* lo,
-                                            double* hi, int quirks) {
-    if (SCH == SCH_GODUNOV) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) { lo[k] = u0[k]; hi[k] = u0[k]; }
-    } else if (sizeof(T) == sizeof(double)) {
-        slope_pair<LIM>(um, u0, up, lo, hi, quirks != 0);
-    } else {
-        // mixed precision: neighbour differences in FP64 (exact cancellation of
-        // the common part), the limiter in FP32, the face states back in FP64;
-        // corrected-mode limiters only (the form has no reference counterpart)
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const float a = (float)(u0[k] - um[k]), b = (float)(up[k] - u0[k]);
-            float sl, sh;
-            if (LIM == LIM_VANLEER) {
-                vanleer_pair(a, b, sl, sh);
-            } else if (LIM == LIM_MINMOD) {
-                sl = lim_minmod(a, b);
-                sh = lim_minmod(b, a);
-            } else {
-                sl = lim_vanalbada(a, b);
-                sh = lim_vanalbada(b, a);
-            }
-            lo[k] = u0[k] + (double)(0.5f * sl);
-            hi[k] = u0[k] - (double)(0.5f * sh);
-        }
-    }
 }
 
 // The step body for one 64-lane block: strip (bx, by) of the solve described
@@ -699,295 +494,6 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
 template <int SCH, int RS, int LIM, typename T, bool CUT>
 __global__ __launch_bounds__(64) void k_step(StepArgs a) {
     step_strip<SCH, RS, LIM, T, CUT>(a, blockIdx.x, blockIdx.y, a.stage);
-}
-
-// ------------------------------------------------------------ row-parallel form
-// The update of cell (c, j) from its face fluxes (F, G on its upper x / y faces;
-// fm_lds[k * 64], gm_lds[k * 64] on its lower faces, in the LDS slots of the
-// lanes that computed them): the conservative or cut-cell update, floors,
-// dt/mass partial terms, the RK2 stage average, the store and the fused
-// Global_Boundary ring writes. The same statements as the update inside
-// step_strip, which keeps its own inline copy: factoring it out of the strip
-// kernel changed that kernel's register allocation (the mixed-precision
-// cut-cell variants went from 128 to 165 VGPRs, 4 to 3 waves per SIMD). The
-// two copies are held equal by the row-parallel tests, which compare the two
-// kernels' states bit for bit over every scheme, flux, boundary and stage.
-template <int SCH, int RS, int LIM, typename T, bool CUT>
-__device__ __forceinline__ void update_cell(const StepArgs& a, const Layout& L, const int c, const int j,
-                                            const double (&w0)[4], const T (&F)[4], const T (&G)[4],
-                                            const T* fm_lds, const T* gm_lds, const double dt,
-                                            const Rcp<double>& r_dx, const Rcp<double>& r_dy,
-                                            const Rcp<double>& r_gm1, const double dtdx, const double dtdy,
-                                            double* __restrict__ Uo, const int stage, double& amax, double& umax,
-                                            double& vmax, double& msum, int& bad) {
-    constexpr bool FP64 = sizeof(T) == sizeof(double);
-    const long long PL = L.plane;
-    const double dx = a.dx, dy = a.dy;
-    const int ng = L.ng, nx = L.nx, ny = L.ny;
-    double un[4];
-    const long long o = L.at(c, j);
-    const uint8_t tc = a.type[o];
-    const bool fluid = CUT ? (tc & 3) == 0 : tc == 0;
-    const bool merged = CUT && (tc & MERGED_BIT);
-    if (fluid) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (FP64) {
-                double Fm = fm_lds[k * 64];
-                double v = w0[k] - dq(dt * ((double)F[k] - Fm), r_dx);
-                un[k] = v - dq(dt * ((double)G[k] - (double)gm_lds[k * 64]), r_dy);
-            } else {  // FP32 fluxes, FP64 accumulation
-                double Fm = (double)fm_lds[k * 64];
-                un[k] = w0[k] - dtdx * ((double)F[k] - Fm) - dtdy * ((double)G[k] - (double)gm_lds[k * 64]);
-            }
-        }
-        if (stage == 0) {
-            if (!merged) {
-                floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, amax, umax, vmax, bad);
-            } else {  // a merge target: floored here, its partials come from the merge
-                double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-                int db = 0;
-                floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, d0, d1, d2, db);
-            }
-        } else {
-            // RK2 (Heun): each stage's update takes the floors; the second
-            // stage then averages with U^n, which the output buffer holds
-            double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-            int db = 0;
-            floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, d0, d1, d2, db);
-            if (stage == 2) {
-#pragma unroll
-                for (int k = 0; k < 4; k++) un[k] = 0.5 * (Uo[o + k * PL] + un[k]);
-                dt_terms(a.gam, a.gm1, un, amax, umax, vmax, bad);
-            }
-        }
-        if (!merged) msum += un[0] * dx * dy;
-    } else if (CUT && (tc & 3) == 3) {
-        // cut cell: aperture-weighted face fluxes and the wall's pressure
-        // force S p (S closes the cell's faces), over the fluid volume
-        // alpha dx dy; small cells are merged (and floored) afterwards
-        const CutCell cc = a.cut[a.cut_index[o]];
-        const double rho = w0[0], u0 = w0[1] / rho, v0 = w0[2] / rho;
-        const double pw = a.gm1 * (w0[3] - 0.5 * rho * (u0 * u0 + v0 * v0));
-        const double wall[4] = {0.0, pw * cc.sbx, pw * cc.sby, 0.0};
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const double Fk = (double)F[k], Fm = (double)fm_lds[k * 64];
-            const double Gk = (double)G[k], Gm = (double)gm_lds[k * 64];
-            double v = w0[k] - dt * ((cc.ae * Fk - cc.aw * Fm) * dy) / (cc.alpha * dx * dy);
-            un[k] = v - dt * ((cc.an * Gk - cc.as * Gm) * dx + wall[k]) / (cc.alpha * dx * dy);
-        }
-        // a merge-group cell (small cell or target) takes its floors and
-        // partials from the merge; the others are floored and counted here
-        if (!merged) {
-            floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, amax, umax, vmax, bad);
-            msum += cc.alpha * un[0] * dx * dy;
-        } else if (cc.alpha >= 0.5) {
-            double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-            int db = 0;
-            floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, d0, d1, d2, db);
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 4; k++) un[k] = w0[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) Uo[o + k * PL] = un[k];
-    if (a.edge_fused) {
-        // Test/Sod/Global_Boudary.cpp:24-59 (rows) and :62-75 (columns), which
-        // read U_NEW: the updated state, or for a non-fluid cell under
-        // SimpleBound the value its U_NEW kept
-        double e[4] = {un[0], un[1], un[2], un[3]};
-        if (a.unew_nf && !fluid)
-            for (int k = 0; k < 4; k++) e[k] = a.unew_nf[o + k * PL];
-        if (j < 2 * ng) Uo[L.at(c, 2 * ng - 1 - j) + 2 * PL] = -e[2];
-        if (j == 2 * ng - 1)
-            for (int r = 0; r < ng; r++) {
-                long long q = L.at(c, r);
-                Uo[q] = e[0]; Uo[q + PL] = e[1]; Uo[q + 3 * PL] = e[3];
-            }
-        if (j >= ny) Uo[L.at(c, 2 * (ny + ng) - 1 - j) + 2 * PL] = -e[2];
-        if (j == ny + ng - 1)
-            for (int r = 0; r < ng; r++) {
-                long long q = L.at(c, ny + ng + r);
-                Uo[q] = e[0]; Uo[q + PL] = e[1]; Uo[q + 3 * PL] = e[3];
-            }
-        if (c == ng)
-            for (int r = 0; r < ng; r++) {
-                long long q = L.at(r, j);
-                for (int k = 0; k < 4; k++) Uo[q + k * PL] = e[k];
-            }
-        if (c == nx + ng - 1)
-            for (int r = 0; r < ng; r++) {
-                long long q = L.at(nx + ng + r, j);
-                for (int k = 0; k < 4; k++) Uo[q + k * PL] = e[k];
-            }
-    }
-}
-
-// Row-parallel strip (small grids, where a strip's rows marching one after the
-// other set the step's latency): the block has rows + 1 waves over the same
-// 60-column x `rows` strip as step_strip; wave w takes row j = j0 - 1 + w at
-// once. It loads rows j-1 .. j+2 of its column, reconstructs them, computes
-// the x faces of row j (for j >= j0, or in quirks mode where the y face may
-// take the aliased x flux) and the y face j+1/2, and after a block barrier the
-// waves of rows j0 .. j1-1 update their cells with the y flux of face j-1/2
-// from the wave below. Every face is computed by the same statements as in
-// step_strip, so the state is bit for bit the same; only the grouping of the
-// mass partial sum differs. Block barriers are reached by every wave.
-template <int SCH, int RS, int LIM, typename T, bool CUT>
-__device__ __forceinline__ void step_rowpar(const StepArgs& a, const int bx, const int by, const int stage) {
-    Ctl* ctl = a.ctl;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const Layout L = a.L;
-    const long long PL = L.plane;
-    const int ng = L.ng, nx = L.nx, ny = L.ny;
-    const int cb = ng + STRIP_W * bx;
-    const int c = cb - 2 + lane;
-    const int cl = min(max(c, 0), L.M - 1);
-    const int j0 = ng + a.rows * by;
-    const int j1 = min(j0 + a.rows, ny + ng);
-    const int j = j0 - 1 + wv;
-    const bool live = j < j1;  // the waves past a short last strip only reach the barriers
-
-    // rows j-1 .. j+2 of column c (in range for every live wave: ng >= 2) of both
-    // state buffers, read together with the control block: the rows' addresses
-    // do not depend on which buffer is current, so no load waits for another
-    // (on a latency-bound small grid that is one memory round trip less)
-    double q0[4][4], q1[4][4];
-    if (live) {
-        const long long o = L.at(cl, j - 1);
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                q0[r][k] = a.buf[0][o + r * L.pitch + k * PL];
-                q1[r][k] = a.buf[1][o + r * L.pitch + k * PL];
-            }
-    } else {
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-            for (int k = 0; k < 4; k++) q0[r][k] = q1[r][k] = 0.0;
-    }
-    if (ctl->done) {
-        if (bx == 0 && by == 0 && threadIdx.x == 0) ctl->ran = 0;
-        return;
-    }
-    const int cur = ctl->cur;
-    const double dt = ctl->dt;
-    const double* __restrict__ Ui = a.buf[cur];
-    double* __restrict__ Uo = a.buf[cur ^ 1];
-    GasT<T> g;
-    g.gam = (T)a.gam; g.gm1 = (T)a.gm1; g.gp1 = (T)a.gp1; g.c1 = (T)a.c1;
-    g.r_gam = Rcp<T>{(T)a.gam, (T)a.r_gam};
-    g.r_gm1 = Rcp<T>{(T)a.gm1, (T)a.r_gm1};
-    g.r_dx = Rcp<T>{(T)a.dx, (T)a.r_dx};
-    g.r_dy = Rcp<T>{(T)a.dy, (T)a.r_dy};
-    const T dtc = (T)dt;
-    const Rcp<double> r_dx{a.dx, a.r_dx}, r_dy{a.dy, a.r_dy}, r_gm1{a.gm1, a.r_gm1};
-    const double dtdx = dt * a.r_dx, dtdy = dt * a.r_dy;  // mixed form only
-
-    __shared__ double rb[ROWPAR_MAX_ROWS + 1][4][64];
-    __shared__ double hb[ROWPAR_MAX_ROWS + 1][4][64];
-    __shared__ T fb[ROWPAR_MAX_ROWS + 1][4][64];
-    __shared__ T gb[ROWPAR_MAX_ROWS + 1][4][64];
-    __shared__ double sp[ROWPAR_MAX_ROWS + 1][5];
-
-    // reference Global_Boundary corners (Test/Sod/Global_Boudary.cpp:62-75), as in step_strip
-    if (a.edge_fused && bx == 0 && by == 0 && wv == 0 && lane < 4 * ng * ng) {
-        int side = lane / (2 * ng * ng);
-        int rem = lane % (2 * ng * ng);
-        int ri = rem / ng, cc = rem % ng;
-        int row = ri < ng ? ri : ny + ri;
-        int col = side == 0 ? cc : nx + ng + cc;
-        int scol = side == 0 ? ng : nx + ng - 1;
-        long long o = L.at(col, row);
-        for (int k = 0; k < 4; k++) {
-            double v = a.corner_from_input ? Ui[L.at(scol, row) + k * PL] : a.corner[(side * 2 * ng + ri) * 4 + k];
-            Uo[o + k * PL] = v;
-        }
-    }
-
-    double wm[4], w0[4], w1[4], w2[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        wm[k] = cur ? q1[0][k] : q0[0][k];
-        w0[k] = cur ? q1[1][k] : q0[1][k];
-        w1[k] = cur ? q1[2][k] : q0[2][k];
-        w2[k] = cur ? q1[3][k] : q0[3][k];
-    }
-    const bool xrow = live && (j >= j0 || (a.quirks && RS != RS_SW));
-    const int ln = min(lane + 1, 63), lp = max(lane - 1, 0);
-    T F[4] = {0, 0, 0, 0};
-    double xp[4], xlo[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) rb[wv][k][lane] = w0[k];
-    __syncthreads();
-    if (xrow) {
-        double xm[4], xhi[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            xm[k] = rb[wv][k][lp];
-            xp[k] = rb[wv][k][ln];
-        }
-        cell_states<SCH, LIM, T>(xm, w0, xp, xlo, xhi, a.quirks);
-#pragma unroll
-        for (int k = 0; k < 4; k++) hb[wv][k][lane] = xhi[k];
-    }
-    __syncthreads();
-    if (xrow) {
-        double xr[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) xr[k] = hb[wv][k][ln];
-        xface<SCH, RS>(g, a.gm1, dtc, a.quirks, w0, xp, xlo, xr, F);
-    }
-    T G[4] = {0, 0, 0, 0};
-    if (live) {
-        double ylo[4], hi_unused[4], ynlo[4], ynhi[4];
-        cell_states<SCH, LIM, T>(wm, w0, w1, ylo, hi_unused, a.quirks);
-        cell_states<SCH, LIM, T>(w0, w1, w2, ynlo, ynhi, a.quirks);
-        yface<SCH, RS>(g, a.gm1, dtc, a.quirks, w0, w1, ylo, ynhi, F, G);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        fb[wv][k][lane] = F[k];
-        gb[wv][k][lane] = G[k];
-    }
-    __syncthreads();
-    double amax = 0.0, umax = 0.0, vmax = 0.0, msum = 0.0;
-    int bad = 0;
-    if (live && j >= j0 && lane >= 2 && lane < STRIP_W + 2 && c < nx + ng)
-        update_cell<SCH, RS, LIM, T, CUT>(a, L, c, j, w0, F, G, &fb[wv][0][lane - 1], &gb[wv - 1][0][lane], dt,
-                                          r_dx, r_dy, r_gm1, dtdx, dtdy, Uo, stage, amax, umax, vmax, msum, bad);
-    // one partial record per block: each wave's, then the waves in order
-    amax = wave_max(amax);
-    umax = wave_max(umax);
-    vmax = wave_max(vmax);
-    msum = wave_sum(msum);
-    bad = __any(bad);
-    if (lane == 0) {
-        sp[wv][0] = amax; sp[wv][1] = umax; sp[wv][2] = vmax; sp[wv][3] = msum; sp[wv][4] = bad ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = blockDim.x >> 6;
-        double A = sp[0][0], U = sp[0][1], V = sp[0][2], m = sp[0][3], b = sp[0][4];
-        for (int q = 1; q < nw; q++) {
-            A = (A < sp[q][0]) ? sp[q][0] : A;
-            U = (U < sp[q][1]) ? sp[q][1] : U;
-            V = (V < sp[q][2]) ? sp[q][2] : V;
-            m = m + sp[q][3];
-            b = (b < sp[q][4]) ? sp[q][4] : b;
-        }
-        double* r = a.mass_part + PART_STRIDE * (long long)(by * a.nbx + bx);
-        r[0] = A; r[1] = U; r[2] = V; r[3] = m; r[4] = b;
-    }
-}
-
-template <int SCH, int RS, int LIM, typename T, bool CUT>
-__global__ __launch_bounds__(64 * (ROWPAR_MAX_ROWS + 1)) void k_step_rowpar(StepArgs a) {
-    step_rowpar<SCH, RS, LIM, T, CUT>(a, blockIdx.x, blockIdx.y, a.stage);
 }
 
 // Batched step: one launch over the strips of many solves. Block b runs strip
@@ -1481,15 +987,6 @@ static hipError_t go(int prec, const StepArgs* a, const BatchLaunch* b, hipStrea
         else
             hipLaunchKernelGGL((k_step_batch<SCH, RS, LIM, double>), dim3(b->nblocks), dim3(64), 0, s, b->slots,
                                b->map, b->stage);
-    } else if (a->rowpar) {  // row-parallel form: rows + 1 waves per block
-        const dim3 grid(a->nbx, a->nby), block(64 * (a->rows + 1));
-        if (a->cut) {
-            if (prec == 1) hipLaunchKernelGGL((k_step_rowpar<SCH, RS, LIM, float, true>), grid, block, 0, s, *a);
-            else hipLaunchKernelGGL((k_step_rowpar<SCH, RS, LIM, double, true>), grid, block, 0, s, *a);
-        } else {
-            if (prec == 1) hipLaunchKernelGGL((k_step_rowpar<SCH, RS, LIM, float, false>), grid, block, 0, s, *a);
-            else hipLaunchKernelGGL((k_step_rowpar<SCH, RS, LIM, double, false>), grid, block, 0, s, *a);
-        }
     } else if (a->cut) {  // cut-cell variant (its extra registers stay out of the plain kernel)
         if (prec == 1) hipLaunchKernelGGL((k_step<SCH, RS, LIM, float, true>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
         else hipLaunchKernelGGL((k_step<SCH, RS, LIM, double, true>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
@@ -1519,6 +1016,7 @@ static hipError_t go_s(int scheme, int rs, int lim, int prec, const StepArgs* a,
 }
 hipError_t launch_step(int scheme, int rs, int lim, int prec, const StepArgs& a, hipStream_t s) {
     if (scheme == SCH_LAX) return launch_step_lax(a, s);
+    if (a.rowpar) return launch_step_rowpar(scheme, rs, lim, prec, a, s);
     return go_s(scheme, rs, lim, prec, &a, nullptr, s);
 }
 hipError_t launch_step_batch(int scheme, int rs, int lim, int prec, const StepArgs* slots, const int2* map,
