@@ -120,6 +120,29 @@ def png_decode(path):
     return out
 
 
+RECORD_W = 28  # doubles per cell of the scheme's scratch record (7 slots x 4)
+
+
+def flux_record(gamma, rec, u_slot, f_slot=None, g_slot=None):
+    """Conserve2Flux{,X,Y} on the GPU for every cell of a host scratch record
+    (M*N*28 doubles): the state in slot u_slot, its x flux to slot f_slot and
+    its y flux to slot g_slot (in place)."""
+    assert rec.dtype == np.float64 and rec.flags.c_contiguous and rec.size % RECORD_W == 0
+    base = rec.ctypes.data
+
+    def at(slot):
+        return None if slot is None else C.cast(base + 32 * slot, _D)
+    _check(lib().ibtfo_flux_record_host(C.c_double(gamma), at(u_slot), at(f_slot), at(g_slot),
+                                        C.c_longlong(rec.size // RECORD_W), RECORD_W), "ibtfo_flux_record_host")
+
+
+def flux_record_device(gamma, u_ptr, f_ptr, g_ptr, ncells, stride, stream=None):
+    """ibtfo_flux_record on device addresses (0 or None for an absent f or g)."""
+    _check(lib().ibtfo_flux_record(C.c_double(gamma), C.c_void_p(u_ptr), C.c_void_p(f_ptr or None),
+                                   C.c_void_p(g_ptr or None), C.c_longlong(ncells), int(stride),
+                                   C.c_void_p(stream)), "ibtfo_flux_record")
+
+
 def config_from_inp(path):
     c = Config()
     _check(lib().ibtfo_config_from_inp(path.encode(), C.byref(c)), "ibtfo_config_from_inp")
@@ -252,6 +275,18 @@ class Solver:
 
     def boundary(self):
         _check(lib().ibtfo_boundary(self.h), "ibtfo_boundary")
+
+    def riemann_record(self, rec, ydir):
+        """Riemann_solverX (ydir 0) / Riemann_solverY (1) with this solve's
+        Riemann slot, gamma and quirks on a host scratch record (M*N*28
+        doubles, updated in place)."""
+        assert rec.dtype == np.float64 and rec.flags.c_contiguous and rec.size == self.M * self.N * RECORD_W
+        _check(lib().ibtfo_riemann_record_host(self.h, int(ydir), ptr(rec)), "ibtfo_riemann_record_host")
+
+    def riemann_record_device(self, rec_ptr, ydir):
+        """The same on a device record (an address, e.g. a tensor's data_ptr());
+        queued on the solve's stream."""
+        _check(lib().ibtfo_riemann_record(self.h, int(ydir), C.c_void_p(rec_ptr)), "ibtfo_riemann_record")
 
     def set_rows(self, rows):
         _check(lib().ibtfo_set_rows(self.h, int(rows)), "ibtfo_set_rows")

@@ -18,6 +18,9 @@
 //   WriteData     Src/Utils/WriteData.H:14-27               -> state copied to the
 //                 host U_OLD (the reference's update self), then the same files
 //   Scheme_Index  Src/BoundaryCondition/Scheme_Index.H:13-19 -> built with the geometry
+//   Riemann_solverX/Y Src/Base/Euler/Riemann/Riemann_solver.H:22-42 -> ibtfo_riemann_record_host
+//                 on the driver's U_TMP (the solve's Riemann slot)
+//   Conserve2Flux{,X,Y} Src/Base/Common/Conserve2Flux.H:11-37 -> ibtfo_flux_record_host
 //   Level_Set     Src/Geometry/Level_Set.H:12-18             -> the driver's own
 //                 Level_Set_function hook fills phi on the host; classification,
 //                 normals and ghost records run on the device. With a `mask_file`
@@ -230,4 +233,49 @@ void WriteData(const double, const double, const double, const double, const int
     Bound* b = find(U_OLD, "WriteData");
     if (b->stepped) check(ibtfo_download_state(b->ctx, b->U_OLD), "WriteData");
     check(ibtfo_vtk_write_host(&b->cfg, iter, now_t, U_OLD, XYCOORD, nullptr), "WriteData");
+}
+
+// The scheme's inner operators on the driver's scratch record U_TMP (MUSCL,
+// MHM and Godunov call them with the whole record, all M*N cells). The solve
+// is found by U_OLD; its grid and gamma must be the ones passed.
+static void riemann_record(int dir, const int N_x, const int N_y, const int ng, const double gamma,
+                           double* U_OLD, double* U_TMP, const int ndevices, const int device, const char* who) {
+    one_device(ndevices, device, who);
+    Bound* b = find(U_OLD, who);
+    if (N_x != b->cfg.N_x || N_y != b->cfg.N_y || ng != b->cfg.num_ghost_cell || gamma != b->cfg.gamma) {
+        fprintf(stderr, "ibtfo: %s: grid or gamma differs from the solve's .inp\n", who);
+        abort();
+    }
+    check(ibtfo_riemann_record_host(b->ctx, dir, U_TMP), who);
+}
+
+void Riemann_solverX(const double, const double, const int N_x, const int N_y, const int ng, const double gamma,
+                     double* U_OLD, double* U_TMP, const int ndevices, const int device) {
+    riemann_record(0, N_x, N_y, ng, gamma, U_OLD, U_TMP, ndevices, device, "Riemann_solverX");
+}
+
+void Riemann_solverY(const double, const double, const int N_x, const int N_y, const int ng, const double gamma,
+                     double* U_OLD, double* U_TMP, const int ndevices, const int device) {
+    riemann_record(1, N_x, N_y, ng, gamma, U_OLD, U_TMP, ndevices, device, "Riemann_solverY");
+}
+
+// Conserve2Flux.cpp: M*N states at a stride of one scratch record (28 doubles)
+static long long record_cells(int N_x, int N_y, int ng) { return (long long)(N_x + 2 * ng) * (N_y + 2 * ng); }
+
+void Conserve2Flux(const int N_x, const int N_y, const int ng, const double gamma, double* U, double* F, double* G,
+                   const int ndevices, const int device) {
+    one_device(ndevices, device, "Conserve2Flux");
+    check(ibtfo_flux_record_host(gamma, U, F, G, record_cells(N_x, N_y, ng), 28), "Conserve2Flux");
+}
+
+void Conserve2FluxX(const int N_x, const int N_y, const int ng, const double gamma, double* U, double* F,
+                    const int ndevices, const int device) {
+    one_device(ndevices, device, "Conserve2FluxX");
+    check(ibtfo_flux_record_host(gamma, U, F, nullptr, record_cells(N_x, N_y, ng), 28), "Conserve2FluxX");
+}
+
+void Conserve2FluxY(const int N_x, const int N_y, const int ng, const double gamma, double* U, double* G,
+                    const int ndevices, const int device) {
+    one_device(ndevices, device, "Conserve2FluxY");
+    check(ibtfo_flux_record_host(gamma, U, nullptr, G, record_cells(N_x, N_y, ng), 28), "Conserve2FluxY");
 }

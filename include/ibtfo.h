@@ -248,6 +248,36 @@ int ibtfo_autotune(ibtfo_ctx* ctx, int probe_steps, int* best_rows, double* best
 int ibtfo_device_state(ibtfo_ctx* ctx, double** planes4, int* pitch, int* offset);
 
 /* ------------------------------------------------------------------------
+ * The inner operators on the reference's per-cell scratch record (its U_TMP:
+ * M*N cells x 28 doubles, cell (i,j) at rec[(M*j+i)*28 + 4*slot + k];
+ * Src/Base/Euler/Advection/MUSCL/EQDefine.H:33-44), for drivers that assemble
+ * their own scheme from them. FP64, the step kernel's per-face arithmetic. */
+/* Riemann_solverX (dir 0) / Riemann_solverY (dir 1), Src/Base/Euler/Riemann/
+ * Riemann_solver.H:22-42, with the solve's Riemann slot, gamma and quirks, on
+ * all M*N cells: reads slots 5 and 6 (U_L, U_R / U_D, U_U); HLL and HLLC also
+ * write their physical fluxes to slots 3 and 4 (F_L, F_R / G_D, G_U); the face
+ * flux goes to slot 2 (F_OLD / G_OLD). Reference mode keeps the reference's y
+ * stores: HLL stores nothing when S_D >= 0 (HLL/Riemann_solver.cpp:224-226);
+ * PVRS leaves the energy component on the branch of its stray store
+ * (PVRS/Riemann_solver.cpp:550, whose out-of-record target is not written).
+ * `rec` is device memory; queued on the solve's stream. */
+int ibtfo_riemann_record(ibtfo_ctx* ctx, int dir, double* rec);
+/* The same on a record in host memory (copied in, solved, copied back before
+ * returning). */
+int ibtfo_riemann_record_host(ibtfo_ctx* ctx, int dir, double* rec);
+/* Conserve2Flux / Conserve2FluxX / Conserve2FluxY (Src/Base/Common/Conserve2Flux.H:11-37):
+ * the physical flux of `ncells` conserved states, state c at u[c*stride .. +3],
+ * its x flux to f[c*stride ..] and its y flux to g[c*stride ..] (f or g may be
+ * NULL; stride >= 4 doubles; the reference calls it with stride 28 on slots
+ * of the scratch record). Device memory, queued on `stream` (a hipStream_t;
+ * NULL = the null stream). */
+int ibtfo_flux_record(double gamma, const double* u, double* f, double* g, long long ncells, int stride,
+                      void* stream);
+/* The same on host memory (synchronous). Every state is read before any flux
+ * is written, so f or g may point into the array u points into. */
+int ibtfo_flux_record_host(double gamma, const double* u, double* f, double* g, long long ncells, int stride);
+
+/* ------------------------------------------------------------------------
  * Batch engine: many independent solves (different grids, geometries, inflow
  * states and step counts) served by one device-resident loop. No counterpart in
  * the reference, which runs one solve per process.

@@ -52,6 +52,20 @@ void TimeAdvance(const double Psy_L, const double Psy_H, const int N_x, const in
                  const double gamma, double dt, double* U_OLD, double* U_TMP, double* U_NEW, double* XYCOORD,
                  double* SCHEME_IDX, const int ndevices,
                  const int device);                          /* Src/Base/TimeAdvance/TimeAdvance.H:13-26 */
+/* The scheme's inner operators on a host scratch record U_TMP (M*N cells x 28
+ * doubles; copied to the GPU, computed there and copied back): */
+void Riemann_solverX(const double Psy_L, const double Psy_H, const int N_x, const int N_y,
+                     const int num_ghost_cell, const double gamma, double* U_OLD, double* U_TMP,
+                     const int ndevices, const int device); /* Src/Base/Euler/Riemann/Riemann_solver.H:22-31 */
+void Riemann_solverY(const double Psy_L, const double Psy_H, const int N_x, const int N_y,
+                     const int num_ghost_cell, const double gamma, double* U_OLD, double* U_TMP,
+                     const int ndevices, const int device); /* Src/Base/Euler/Riemann/Riemann_solver.H:33-42 */
+void Conserve2Flux(const int N_x, const int N_y, const int num_ghost_cell, const double gamma, double* U, double* F,
+                   double* G, const int ndevices, const int device);  /* Src/Base/Common/Conserve2Flux.H:11-19 */
+void Conserve2FluxX(const int N_x, const int N_y, const int num_ghost_cell, const double gamma, double* U,
+                    double* F, const int ndevices, const int device); /* Src/Base/Common/Conserve2Flux.H:21-28 */
+void Conserve2FluxY(const int N_x, const int N_y, const int num_ghost_cell, const double gamma, double* U,
+                    double* G, const int ndevices, const int device); /* Src/Base/Common/Conserve2Flux.H:30-37 */
 void ComputeDt(const double Psy_L, const double Psy_H, const int N_x, const int N_y, const int num_ghost_cell,
                const double gamma, const double CFL_number, double* U_OLD, double* XYCOORD, double* TMP_DT,
                const int ndevices, const int device);                  /* Src/Base/Common/ComputeDt.H:12-23 */

@@ -655,6 +655,23 @@ void or_hll_speeds(int quirks, int ydir, double gam, const double* Ulo, const do
     hll_speeds(gam, a, b, S[0], S[1], quirks);
 }
 
+// Riemann_solverX / Riemann_solverY (Riemann_solver.H:22-42) on a whole scratch
+// record T (M*N cells x 28 doubles), as MUSCL/MHM/Godunov call them; the PVRS
+// stray store follows or_set_pvrs_stray.
+void or_riemann_record(int rs, int ydir, int quirks, int nx, int ny, int ng, double gam, double* T) {
+    Ctx c(nx, ny, ng, 1.0, 1.0, gam, 28, T, quirks);
+    if (ydir) riemann_y(c, rs); else riemann_x(c, rs);
+}
+
+// Conserve2Flux / Conserve2FluxX / Conserve2FluxY (Conserve2Flux.cpp:4-112): n
+// states at a stride of `stride` doubles, x flux to F, y flux to G (either null).
+void or_flux_record(double gam, const double* U, double* F, double* G, long long n, int stride) {
+    for (long long c = 0; c < n; c++) {
+        if (F) flux_x(gam, U + c * stride, F + c * stride);
+        if (G) flux_y(gam, U + c * stride, G + c * stride);
+    }
+}
+
 // Advance (one scheme pass) -- *_Scheme.cpp. T must hold or_tmp_width(scheme)
 // doubles per cell and persists between calls like the reference's U_TMP.
 void or_advance(int scheme, int rs, int lim, int quirks, double L, double H, int nx, int ny, int ng,

@@ -266,6 +266,32 @@ def compute_dt(case, U, XY, root_mode=0):
     return v
 
 
+RECORD_W = 28  # doubles per cell of the scheme's scratch record (7 slots x 4)
+
+
+def riemann_record(riemann, ydir, quirks, nx, ny, ng, gamma, rec, root_mode=0, pvrs_stray=True):
+    """Riemann_solverX (ydir 0) / Riemann_solverY (1) on a whole scratch record
+    (M*N*28 doubles, modified in place)."""
+    L = lib()
+    assert rec.size == (nx + 2 * ng) * (ny + 2 * ng) * RECORD_W
+    L.or_set_root_mode(int(root_mode))
+    L.or_set_pvrs_stray(int(bool(pvrs_stray)))
+    L.or_riemann_record(RIEMANNS[riemann], int(ydir), int(quirks), nx, ny, ng, C.c_double(gamma), ptr(rec))
+    L.or_set_root_mode(0)
+    L.or_set_pvrs_stray(1)
+
+
+def flux_record(gamma, rec, u_slot, f_slot=None, g_slot=None):
+    """Conserve2Flux{,X,Y} on every cell of a scratch record: the state in slot
+    u_slot, its x flux to slot f_slot and its y flux to g_slot (in place)."""
+    n = rec.size // RECORD_W
+    base = rec.ctypes.data
+
+    def at(slot):
+        return None if slot is None else C.cast(base + 8 * 4 * slot, _D)
+    lib().or_flux_record(C.c_double(gamma), at(u_slot), at(f_slot), at(g_slot), C.c_longlong(n), RECORD_W)
+
+
 # ------------------------------------------------------------ reference libs
 def ref_lib_name(case):
     return "libref_%s_%s_%s_%s_%s.so" % (case.case, case.Scheme, case.Riemann, case.Limiter, case.Boundary)

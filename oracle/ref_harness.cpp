@@ -32,6 +32,14 @@ void ComputeDt(const double, const double, const int, const int, const int, cons
                double*, double*, const int, const int);
 void CaculateMass(const double, const double, const int, const int, const int, double*, double*, double*,
                   const int, const int);
+void Riemann_solverX(const double, const double, const int, const int, const int, const double, double*, double*,
+                     const int, const int);  // Riemann_solver.H:22-31
+void Riemann_solverY(const double, const double, const int, const int, const int, const double, double*, double*,
+                     const int, const int);  // Riemann_solver.H:33-42
+void Conserve2Flux(const int, const int, const int, const double, double*, double*, double*, const int,
+                   const int);  // Conserve2Flux.H:11-19
+void Conserve2FluxX(const int, const int, const int, const double, double*, double*, const int, const int);
+void Conserve2FluxY(const int, const int, const int, const double, double*, double*, const int, const int);
 
 #ifndef REF_NUM_SCH
 #define REF_NUM_SCH 1
@@ -109,6 +117,24 @@ int ref_run(const char* inp, int nsteps, int init_boundary, double* U_OLD_out, d
 void ref_advance(double L, double H, int nx, int ny, int ng, double gamma, double dt, double* U_OLD, double* U_TMP,
                  double* U_NEW, double* XY) {
     TimeAdvance(L, H, nx, ny, ng, gamma, dt, U_OLD, U_TMP, U_NEW, XY, nullptr, 1, 0);
+}
+
+// The linked Riemann solver's Riemann_solverX (ydir 0) / Riemann_solverY (1) on
+// a whole scratch record (M*N cells x 28 doubles); U_OLD is only named in its
+// data clause. Lax libraries link no Riemann solver.
+#if REF_TMP_W == 28
+void ref_riemann_record(int ydir, int nx, int ny, int ng, double gamma, double* U_OLD, double* U_TMP) {
+    if (ydir) Riemann_solverY(1.0, 1.0, nx, ny, ng, gamma, U_OLD, U_TMP, 1, 0);
+    else Riemann_solverX(1.0, 1.0, nx, ny, ng, gamma, U_OLD, U_TMP, 1, 0);
+}
+#endif
+
+// Conserve2Flux (which 2), Conserve2FluxX (0) or Conserve2FluxY (1) on M*N
+// states at the record stride (28 doubles)
+void ref_conserve2flux(int which, int nx, int ny, int ng, double gamma, double* U, double* F, double* G) {
+    if (which == 2) Conserve2Flux(nx, ny, ng, gamma, U, F, G, 1, 0);
+    else if (which == 1) Conserve2FluxY(nx, ny, ng, gamma, U, G, 1, 0);
+    else Conserve2FluxX(nx, ny, ng, gamma, U, F, 1, 0);
 }
 
 // Parsed .inp numbers()/strings() maps, written as key 0x1f value 0x1e records into buf.

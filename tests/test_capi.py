@@ -4,6 +4,8 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
+
 import ibtfo_amd as ib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -44,8 +46,25 @@ def test_reference_signatures_are_exported():
                 "TimeAdvance(double, double, int, int, int, double, double, double*, double*, double*, double*, "
                 "double*, int, int)",
                 "ComputeDt(double, double, int, int, int, double, double, double*, double*, double*, int, int)",
-                "CaculateMass(double, double, int, int, int, double*, double*, double*, int, int)"]:
+                "CaculateMass(double, double, int, int, int, double*, double*, double*, int, int)",
+                "Riemann_solverX(double, double, int, int, int, double, double*, double*, int, int)",
+                "Riemann_solverY(double, double, int, int, int, double, double*, double*, int, int)",
+                "Conserve2Flux(int, int, int, double, double*, double*, double*, int, int)",
+                "Conserve2FluxX(int, int, int, double, double*, double*, int, int)",
+                "Conserve2FluxY(int, int, int, double, double*, double*, int, int)"]:
         assert sig in out, sig
+
+
+def test_record_operator_arguments_rejected_before_any_device_call():
+    L = ib.lib()
+    rec = np.zeros(28 * 4)
+    u = rec.ctypes.data_as(C.POINTER(C.c_double))
+    for args in [(u, None, None, 4, 28), (None, u, None, 4, 28), (u, u, None, -1, 28), (u, u, None, 4, 3)]:
+        a, f, g, n, stride = args
+        assert L.ibtfo_flux_record_host(C.c_double(1.4), a, f, g, C.c_longlong(n), stride) == ib.E_ARG, args
+        assert L.ibtfo_flux_record(C.c_double(1.4), a, f, g, C.c_longlong(n), stride, None) == ib.E_ARG, args
+    assert L.ibtfo_riemann_record_host(None, 0, u) == ib.E_ARG
+    assert L.ibtfo_riemann_record(None, 1, u) == ib.E_ARG
 
 
 def test_config_struct_matches_header():
