@@ -145,7 +145,7 @@ void ibtfo::free_ctx(ibtfo_ctx* x) {
     for (void* p : {(void*)x->state[0], (void*)x->state[1], (void*)x->type, (void*)x->phi, (void*)x->gnx,
                     (void*)x->gny, (void*)x->ghosts, (void*)x->gvals, (void*)x->dcount, (void*)x->ctl,
                     (void*)x->mass_part, (void*)x->corner, (void*)x->cuts, (void*)x->cut_index, (void*)x->groups,
-                    (void*)x->stype, (void*)x->merge_part, (void*)x->fold, (void*)x->unew0})
+                    (void*)x->stype, (void*)x->merge_part, (void*)x->fold, (void*)x->unew0, (void*)x->df_part})
         dfree(x, p);
     if (x->own_stream) (void)hipStreamDestroy(x->own_stream);
     delete x;
@@ -187,6 +187,8 @@ int ctx_create(const ibtfo_config* cfg, Arena* arena, hipStream_t stream, int ro
     if (dalloc(x, &x->mass_part, PART_STRIDE * (size_t)x->mass_cap * sizeof(double)) != hipSuccess)
         return fail("partials");
     if (dalloc(x, &x->corner, 2 * 2 * L.ng * 4 * sizeof(double)) != hipSuccess) return fail("corner");
+    const size_t df_bytes = 2 * (size_t)DF_MAX_REC * PART_STRIDE * sizeof(double);
+    if (!arena && dalloc(x, &x->df_part, df_bytes) != hipSuccess) return fail("partials");
     if (stream) {
         x->stream = stream;
     } else {
@@ -200,7 +202,8 @@ int ctx_create(const ibtfo_config* cfg, Arena* arena, hipStream_t stream, int ro
         hipMemsetAsync(x->phi, 0, L.plane * sizeof(double), x->stream) != hipSuccess ||
         hipMemsetAsync(x->gnx, 0, L.plane * sizeof(double), x->stream) != hipSuccess ||
         hipMemsetAsync(x->gny, 0, L.plane * sizeof(double), x->stream) != hipSuccess ||
-        hipMemsetAsync(x->corner, 0, 2 * 2 * L.ng * 4 * sizeof(double), x->stream) != hipSuccess)
+        hipMemsetAsync(x->corner, 0, 2 * 2 * L.ng * 4 * sizeof(double), x->stream) != hipSuccess ||
+        (x->df_part && hipMemsetAsync(x->df_part, 0, df_bytes, x->stream) != hipSuccess))
         return fail("initial clear");  // stream-ordered before everything else on the solve's stream
     const double Psy_L = cfg->hi_x - cfg->lo_x;
     x->dxc = Psy_L / cfg->N_x;  // Psy_coord.cpp:21-22 (y spaced with dx as well)
@@ -809,6 +812,13 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     a.cut = x->cfg.boundary == IBTFO_CUTCELL ? x->cuts : nullptr;
     a.cut_index = x->cut_index;
     if (a.cut) a.type = x->stype;
+    // deferred finish: a step is one kernel when nothing runs between the step
+    // kernel and its finish (no ghost fill, edge or merge kernel)
+    a.df_part = x->df_part;
+    a.df_ok = x->df_part && a.rowpar && c.time == IBTFO_SINGLESTEP && c.boundary == IBTFO_REGULAR &&
+              c.edge == IBTFO_EDGE_REFERENCE && c.scheme != IBTFO_LAX && a.nbx * a.nby <= DF_MAX_REC;
+    a.df = 0;
+    a.df_k = 0;
 
     FillArgs& f = x->fa;
     memset(&f, 0, sizeof f);
@@ -885,8 +895,9 @@ static int enqueue_boundaries(ibtfo_ctx* x) {
 
 // One step of the reference loop. `ev` (optional) receives events around each
 // step-kernel launch: ev[0], ev[1] (single step or RK2 stage 1), ev[2], ev[3]
-// (RK2 stage 2).
-static int enqueue_step(ibtfo_ctx* x, hipEvent_t* ev = nullptr) {
+// (RK2 stage 2). k > 0: the k-th step of a deferred-finish batch (one kernel;
+// the next kernel, or the batch's flush, finishes it).
+static int enqueue_step(ibtfo_ctx* x, hipEvent_t* ev = nullptr, int k = 0) {
     const ibtfo_config& c = x->cfg;
     int rc;
     if (c.time == IBTFO_RK2) {
@@ -907,9 +918,15 @@ static int enqueue_step(ibtfo_ctx* x, hipEvent_t* ev = nullptr) {
         HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, s2, x->stream));
         if (ev) HIPCHK(hipEventRecord(ev[3], x->stream));
     } else {
+        StepArgs s = x->sa;
+        if (k > 0) {
+            s.df = k == 1 ? 1 : 2;
+            s.df_k = k & 1;
+        }
         if (ev) HIPCHK(hipEventRecord(ev[0], x->stream));
-        HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, x->sa, x->stream));
+        HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, s, x->stream));
         if (ev) HIPCHK(hipEventRecord(ev[1], x->stream));
+        if (k > 0) return IBTFO_OK;
     }
     if (c.boundary == IBTFO_CUTCELL) {
         // small cells merged (with the merged cells' dt / mass partials), then
@@ -918,6 +935,19 @@ static int enqueue_step(ibtfo_ctx* x, hipEvent_t* ev = nullptr) {
     }
     HIPCHK(launch_finish(x->fin, x->stream));
     return enqueue_boundaries(x);
+}
+
+// n steps, queued as one batch: with the deferred finish, n step kernels and
+// the closing flush, which leaves the control block as the finish kernel would.
+// `ev` (optional): `per` events per step, as enqueue_step takes them.
+static int enqueue_steps(ibtfo_ctx* x, int n, hipEvent_t* ev = nullptr, int per = 0) {
+    const bool df = x->sa.df_ok != 0;
+    for (int s = 0; s < n; s++) {
+        int rc = enqueue_step(x, ev ? ev + (size_t)per * s : nullptr, df ? s + 1 : 0);
+        if (rc) return rc;
+    }
+    if (df && n > 0) HIPCHK(launch_df_flush(x->sa, n & 1, x->stream));
+    return IBTFO_OK;
 }
 
 static int run_reduce(ibtfo_ctx* x, int reset) {
@@ -959,11 +989,7 @@ extern "C" int ibtfo_prepare(ibtfo_ctx* x, int apply_boundary) {
 extern "C" int ibtfo_step(ibtfo_ctx* x, int nsteps) {
     if (!x || nsteps < 0) return IBTFO_E_ARG;
     if (!x->prepared) { set_error("ibtfo_step before ibtfo_prepare"); return IBTFO_E_STATE; }
-    for (int s = 0; s < nsteps; s++) {
-        int rc = enqueue_step(x);
-        if (rc) return rc;
-    }
-    return IBTFO_OK;
+    return enqueue_steps(x, nsteps);
 }
 
 extern "C" int ibtfo_step_dt(ibtfo_ctx* x, double dt) {
@@ -976,7 +1002,7 @@ extern "C" int ibtfo_step_dt(ibtfo_ctx* x, double dt) {
     c.dt = dt;   // the caller's dt for this step
     c.done = 0;  // the caller keeps the clock and decides when to stop
     if ((rc = write_ctl_head(x, c))) return rc;
-    return enqueue_step(x);
+    return enqueue_steps(x, 1);
 }
 
 extern "C" int ibtfo_step_graph(ibtfo_ctx* x, int nsteps, int chunk) {
@@ -987,7 +1013,7 @@ extern "C" int ibtfo_step_graph(ibtfo_ctx* x, int nsteps, int chunk) {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
         int rc = IBTFO_OK;
-        for (int s = 0; s < chunk && rc == IBTFO_OK; s++) rc = enqueue_step(x);
+        rc = enqueue_steps(x, chunk);
         hipError_t e = hipStreamEndCapture(x->stream, &g);
         if (e == hipSuccess && rc == IBTFO_OK && g) e = hipGraphInstantiate(&x->graph, g, nullptr, nullptr, 0);
         if (g) (void)hipGraphDestroy(g);
@@ -1074,7 +1100,7 @@ extern "C" int ibtfo_time_steps(ibtfo_ctx* x, int nsteps, double* total_ms, doub
         const int per = x->cfg.time == IBTFO_RK2 ? 4 : 2;
         std::vector<hipEvent_t> ev((size_t)per * nsteps, nullptr);
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-        for (int s = 0; s < nsteps && rc == IBTFO_OK; s++) rc = enqueue_step(x, &ev[(size_t)per * s]);
+        rc = enqueue_steps(x, nsteps, ev.data(), per);
         double sum = 0;
         if (rc == IBTFO_OK && hipEventSynchronize(ev.back()) != hipSuccess) rc = IBTFO_E_HIP;
         for (int s = 0; s < nsteps && rc == IBTFO_OK; s++)

@@ -46,6 +46,7 @@ struct ibtfo_ctx {
     int ncut, ngroups;
     uint8_t* stype;       // the cut-cell step kernel's type plane: type | MERGED_BIT for group cells
     double* merge_part;   // PART_STRIDE doubles per merge block
+    double* df_part;      // deferred finish: 2 x DF_MAX_REC partial records (single solves only)
     hipGraphExec_t graph;
     int graph_chunk;
     std::vector<double> corner_host;

@@ -27,6 +27,16 @@ struct Layout {
     __host__ __device__ long long at(int i, int j) const { return (long long)off + (long long)j * pitch + i; }
 };
 
+// The clock a step kernel runs with, in the deferred-finish form of small
+// grids (see StepArgs::df): kernel k of a batch of steps computes it from the
+// previous kernel's head and partial records and leaves it in Ctl::df[k & 1]
+// for kernel k + 1; `pending` = 1 when kernel k ran its step, whose finish
+// (dt, clock, log) the next kernel, or the batch's closing flush, applies.
+struct DfHead {
+    double dt, now_t, mass;
+    int iter, done, cur, pending;
+};
+
 // Device control block of one solve: the time loop runs on the device so that
 // steps can be queued (or replayed from a graph) without host round trips.
 struct Ctl {
@@ -39,6 +49,7 @@ struct Ctl {
     int status;     // 0 ok; 1 non-finite value met in the dt reduction
     unsigned int folded;  // fold blocks done in the running finish (0 between launches)
     double mass_now;                      // CaculateMass of the state seen by the last reduction
+    DfHead df[2];                         // deferred finish: the heads of a batch's kernels, by parity
     double log_dt[LOGCAP], log_t[LOGCAP], log_mass[LOGCAP];
 };
 
@@ -82,7 +93,18 @@ struct StepArgs {
     const CutCell* cut;     // cut-cell records (type-3 cells), else null
     const int* cut_index;   // plane: record of each type-3 cell
     int rowpar;             // row-parallel form (k_step_rowpar: rows + 1 waves per block)
+    // Deferred finish (row-parallel form, single-step, Regular boundary with the
+    // fused reference edge hook, at most DF_MAX_REC blocks): no finish kernel
+    // between steps. Kernel k of a batch reduces kernel k-1's partial records
+    // (df_part[(k-1) & 1], in the one-wave finish's order) and advances the
+    // clock itself, every block alike; block (0,0) keeps the log. The batch
+    // ends with k_df_flush, which publishes the last head to the control block.
+    int df_ok;              // the solve qualifies
+    int df;                 // this launch: 0 off; 1 first kernel of a batch (clock from Ctl); 2 later
+    int df_k;               // this launch's parity in its batch (k & 1)
+    double* df_part;        // 2 x DF_MAX_REC partial records
 };
+constexpr int DF_MAX_REC = 256;  // blocks of a deferred-finish step (4 records per lane of one wave)
 
 // Small-cell merging after a cut-cell step (one thread per group).
 struct MergeArgs {
@@ -254,6 +276,8 @@ int persist_blocks(int scheme, int rs, int lim, int prec);
 hipError_t launch_step_lax(const StepArgs& a, hipStream_t s);
 // the row-parallel step kernel (StepArgs::rowpar; 1 <= rows <= ROWPAR_MAX_ROWS), ibtfo_rowpar.hip
 hipError_t launch_step_rowpar(int scheme, int rs, int lim, int prec, const StepArgs& a, hipStream_t s);
+// the closing kernel of a deferred-finish batch whose last kernel had parity k
+hipError_t launch_df_flush(const StepArgs& a, int k, hipStream_t s);
 hipError_t launch_edge_ref(const EdgeArgs& a, const double* corner, int corner_from_input, hipStream_t s);
 
 }  // namespace ibtfo

@@ -131,6 +131,88 @@ __device__ __forceinline__ void update_cell(const StepArgs& a, const Layout& L, 
     }
 }
 
+// Deferred finish: the dt / mass reduction of a step's nrec <= DF_MAX_REC
+// partial records by one wave, in the order of the one-wave finish kernel
+// (k_finish_small: lane t takes records t, t + 64, ... in turn, then the
+// cross-lane max / sum), so dt, mass and the clock come out bit for bit as
+// they do with a finish kernel between the steps.
+constexpr int DF_PER_LANE = DF_MAX_REC / 64;
+__device__ __forceinline__ void df_reduce(const double* part, const int nrec, double& A, double& Um, double& Vm,
+                                          double& m, int& bad) {
+    const int t = threadIdx.x & 63;
+    double ra[DF_PER_LANE], ru[DF_PER_LANE], rv[DF_PER_LANE], rm[DF_PER_LANE], rb[DF_PER_LANE];
+#pragma unroll
+    for (int b = 0; b < DF_PER_LANE; b++) {
+        const int q = t + 64 * b;
+        const double* r = part + PART_STRIDE * (long long)(q < nrec ? q : 0);
+        ra[b] = r[0]; ru[b] = r[1]; rv[b] = r[2]; rm[b] = r[3]; rb[b] = r[4];
+    }
+    A = 0.0; Um = 0.0; Vm = 0.0; m = 0.0;
+    bad = 0;
+#pragma unroll
+    for (int b = 0; b < DF_PER_LANE; b++) {
+        if (t + 64 * b >= nrec) break;
+        A = (A < ra[b]) ? ra[b] : A;
+        Um = (Um < ru[b]) ? ru[b] : Um;
+        Vm = (Vm < rv[b]) ? rv[b] : Vm;
+        m += rm[b];
+        bad |= rb[b] != 0.0;
+    }
+    A = wave_max(A);
+    Um = wave_max(Um);
+    Vm = wave_max(Vm);
+    m = wave_sum(m);
+    bad = __any(bad);
+}
+
+// The finish of the step that ran with head `h` (the finish kernel's clock
+// bookkeeping): dt from the reduced partials, the reference's clock order
+// (now_t advances by the new dt), the log entry of that step by `log_lane`.
+__device__ __forceinline__ DfHead df_advance(const StepArgs& a, const DfHead& h, double A2, double Um, double Vm,
+                                             double m, int bad, bool log_lane) {
+    const double A = ib_sqrt(A2);
+    const double dt = a.cfl * a.dx / (A + ((Um < Vm) ? Vm : Um));
+    DfHead n;
+    n.dt = dt;
+    n.now_t = h.now_t + dt;
+    n.iter = h.iter + 1;
+    n.mass = m;
+    n.done = !(n.now_t < a.t_end && n.iter < a.max_iter);
+    n.cur = h.cur ^ 1;
+    n.pending = 0;
+    if (log_lane) {
+        Ctl* ctl = a.ctl;
+        const int slot = h.iter % LOGCAP;
+        ctl->log_dt[slot] = dt;
+        ctl->log_t[slot] = n.now_t;
+        ctl->log_mass[slot] = m;
+        if (bad) ctl->status |= 1;
+    }
+    return n;
+}
+
+// The head of this launch: from the control block for the first kernel of a
+// batch, else the previous kernel's head advanced over its partial records.
+// Every wave computes it alike; block (0,0) stores it for the next kernel.
+__device__ __forceinline__ DfHead df_head(const StepArgs& a, const bool first_block) {
+    Ctl* ctl = a.ctl;
+    DfHead h;
+    if (a.df == 1) {
+        h.dt = ctl->dt; h.now_t = ctl->now_t; h.mass = ctl->mass_now;
+        h.iter = ctl->iter; h.done = ctl->done; h.cur = ctl->cur; h.pending = 0;
+    } else {
+        const int p = a.df_k ^ 1;
+        const DfHead prev = ctl->df[p];
+        double A, Um, Vm, m;
+        int bad;
+        df_reduce(a.df_part + (long long)p * DF_MAX_REC * PART_STRIDE, a.nbx * a.nby, A, Um, Vm, m, bad);
+        h = prev.pending ? df_advance(a, prev, A, Um, Vm, m, bad, first_block && threadIdx.x == 0) : prev;
+    }
+    h.pending = !h.done;
+    if (first_block && threadIdx.x == 0) ctl->df[a.df_k] = h;
+    return h;
+}
+
 // Row-parallel strip (small grids, where a strip's rows marching one after the
 // other set the step's latency): the block has rows + 1 waves over the same
 // 60-column x `rows` strip as step_strip; wave w takes row j = j0 - 1 + w at
@@ -176,12 +258,21 @@ __device__ __forceinline__ void step_rowpar(const StepArgs& a, const int bx, con
 #pragma unroll
             for (int k = 0; k < 4; k++) q0[r][k] = q1[r][k] = 0.0;
     }
-    if (ctl->done) {
-        if (bx == 0 && by == 0 && threadIdx.x == 0) ctl->ran = 0;
-        return;
+    int cur;
+    double dt;
+    if (a.df) {
+        const DfHead h = df_head(a, bx == 0 && by == 0);
+        if (h.done) return;
+        cur = h.cur;
+        dt = h.dt;
+    } else {
+        if (ctl->done) {
+            if (bx == 0 && by == 0 && threadIdx.x == 0) ctl->ran = 0;
+            return;
+        }
+        cur = ctl->cur;
+        dt = ctl->dt;
     }
-    const int cur = ctl->cur;
-    const double dt = ctl->dt;
     const double* __restrict__ Ui = a.buf[cur];
     double* __restrict__ Uo = a.buf[cur ^ 1];
     GasT<T> g;
@@ -286,9 +377,30 @@ __device__ __forceinline__ void step_rowpar(const StepArgs& a, const int bx, con
             m = m + sp[q][3];
             b = (b < sp[q][4]) ? sp[q][4] : b;
         }
-        double* r = a.mass_part + PART_STRIDE * (long long)(by * a.nbx + bx);
+        double* part = a.df ? a.df_part + (long long)a.df_k * DF_MAX_REC * PART_STRIDE : a.mass_part;
+        double* r = part + PART_STRIDE * (long long)(by * a.nbx + bx);
         r[0] = A; r[1] = U; r[2] = V; r[3] = m; r[4] = b;
     }
+}
+
+// The end of a deferred-finish batch: the finish of its last kernel (parity
+// k), if that kernel ran its step, published to the control block as the
+// finish kernel leaves it (ran = whether that step ran).
+__global__ __launch_bounds__(64) void k_df_flush(StepArgs a, int k) {
+    Ctl* ctl = a.ctl;
+    const DfHead h = ctl->df[k];
+    double A, Um, Vm, m;
+    int bad;
+    df_reduce(a.df_part + (long long)k * DF_MAX_REC * PART_STRIDE, a.nbx * a.nby, A, Um, Vm, m, bad);
+    if (threadIdx.x != 0) return;
+    const DfHead n = h.pending ? df_advance(a, h, A, Um, Vm, m, bad, true) : h;
+    ctl->dt = n.dt;
+    ctl->now_t = n.now_t;
+    ctl->iter = n.iter;
+    ctl->done = n.done;
+    ctl->cur = n.cur;
+    ctl->mass_now = n.mass;
+    ctl->ran = h.pending;
 }
 
 template <int SCH, int RS, int LIM, typename T, bool CUT>
@@ -321,8 +433,15 @@ static hipError_t go_rowpar_r(int rs, int lim, int prec, const StepArgs& a, hipS
     return go_rowpar_l<SCH, RS_HLL>(lim, prec, a, s);
 }
 
+hipError_t launch_df_flush(const StepArgs& a, int k, hipStream_t s) {
+    if (!a.df_ok || a.nbx * a.nby > DF_MAX_REC || (k != 0 && k != 1)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_df_flush, dim3(1), dim3(64), 0, s, a, k);
+    return hipGetLastError();
+}
+
 hipError_t launch_step_rowpar(int scheme, int rs, int lim, int prec, const StepArgs& a, hipStream_t s) {
     if (scheme == SCH_LAX || a.rows < 1 || a.rows > ROWPAR_MAX_ROWS) return hipErrorInvalidValue;
+    if (a.df && (!a.df_ok || a.nbx * a.nby > DF_MAX_REC || a.stage != 0 || !a.df_part)) return hipErrorInvalidValue;
     if (scheme == SCH_MHM) return go_rowpar_r<SCH_MHM>(rs, lim, prec, a, s);
     if (scheme == SCH_GODUNOV) return go_rowpar_r<SCH_GODUNOV>(rs, lim, prec, a, s);
     return go_rowpar_r<SCH_MUSCL>(rs, lim, prec, a, s);
