@@ -77,6 +77,25 @@ def bodies_mask(N_x, N_y, ng=2, L=2.0, H=1.0, seed=20261202):
     return img
 
 
+def write_png(path, rgba):
+    """An (H, W, 4) uint8 image as an 8-bit RGBA PNG (rows unfiltered, one
+    zlib-compressed IDAT chunk), the form the mask level set's reader takes."""
+    import struct
+    import zlib
+    img = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w = img.shape[:2]
+    raw = np.zeros((h, 1 + 4 * w), np.uint8)  # filter byte 0 ("None") before each row
+    raw[:, 1:] = img.reshape(h, 4 * w)
+
+    def chunk(tag, data):
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n")
+        f.write(chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0)))
+        f.write(chunk(b"IDAT", zlib.compress(raw.tobytes(), 6)))
+        f.write(chunk(b"IEND", b""))
+
+
 def shu_exact(x, y, t, gam=1.4, beta=5.0, L=10.0):
     """Density of the isentropic vortex (strength beta, centred at (5, 5) at t = 0)
     advected by the free stream (1, 1) on the periodic [0, L]^2."""
