@@ -113,7 +113,15 @@ def write_vtk_host(cfg, iter_, now_t, U_OLD, XYCOORD, directory):
 def png_decode(path):
     """(H, W, 4) uint8 RGBA of a PNG file as the mask level set sees it."""
     w, h = C.c_int(), C.c_int()
-    _check(lib().ibtfo_png_decode(os.fsencode(path), None, C.c_longlong(0), C.byref(w), C.byref(h)), "ibtfo_png_decode")
+    with open(path, "rb") as f:
+        head = f.read(24)
+    hw = int.from_bytes(head[16:20], "big") if len(head) == 24 else 0
+    hh = int.from_bytes(head[20:24], "big") if len(head) == 24 else 0
+    if head[:8] == b"\x89PNG\r\n\x1a\n" and head[12:16] == b"IHDR" and 0 < hw <= 65536 and 0 < hh <= 65536:
+        w.value, h.value = hw, hh  # the size from the header; the decode below checks the rest
+    else:  # let the decoder say what is wrong
+        _check(lib().ibtfo_png_decode(os.fsencode(path), None, C.c_longlong(0), C.byref(w), C.byref(h)),
+               "ibtfo_png_decode")
     out = np.zeros((h.value, w.value, 4), dtype=np.uint8)
     _check(lib().ibtfo_png_decode(os.fsencode(path), out.ctypes.data_as(C.c_void_p), C.c_longlong(out.size),
                                   C.byref(w), C.byref(h)), "ibtfo_png_decode")

@@ -95,25 +95,48 @@ int png_decode(const unsigned char* buf, size_t len, std::vector<unsigned char>&
     // undo the per-row filters in place
     std::vector<unsigned char> img(stride * h);
     for (unsigned y = 0; y < h; y++) {
-        const unsigned char* in = &raw[y * (stride + 1)];
+        const unsigned char* in = &raw[y * (stride + 1)] + 1;
         unsigned char* out = &img[y * stride];
         const unsigned char* up = y ? &img[(y - 1) * stride] : nullptr;
-        const int f = in[0];
-        for (size_t i = 0; i < stride; i++) {
-            int a = i >= bpp ? out[i - bpp] : 0, b = up ? up[i] : 0, c = (up && i >= bpp) ? up[i - bpp] : 0;
-            int x = in[1 + i];
-            switch (f) {
-                case 0: break;
-                case 1: x += a; break;
-                case 2: x += b; break;
-                case 3: x += (a + b) >> 1; break;
-                case 4: x += paeth(a, b, c); break;
-                default: set_error("PNG: bad filter type %d in row %u", f, y); return IBTFO_E_IO;
-            }
-            out[i] = (unsigned char)x;
+        const int f = in[-1];
+        // a = the byte one pixel to the left, b = the byte above, c = above-left
+        // (0 outside the image); all sums modulo 256
+        switch (f) {
+            case 0:  // None
+                memcpy(out, in, stride);
+                break;
+            case 1:  // Sub
+                for (size_t i = 0; i < stride; i++) out[i] = (unsigned char)(in[i] + (i >= bpp ? out[i - bpp] : 0));
+                break;
+            case 2:  // Up
+                if (up)
+                    for (size_t i = 0; i < stride; i++) out[i] = (unsigned char)(in[i] + up[i]);
+                else
+                    memcpy(out, in, stride);
+                break;
+            case 3:  // Average
+                for (size_t i = 0; i < stride; i++) {
+                    const int a = i >= bpp ? out[i - bpp] : 0, b = up ? up[i] : 0;
+                    out[i] = (unsigned char)(in[i] + ((a + b) >> 1));
+                }
+                break;
+            case 4:  // Paeth
+                for (size_t i = 0; i < stride; i++) {
+                    const int a = i >= bpp ? out[i - bpp] : 0, b = up ? up[i] : 0;
+                    const int c = (up && i >= bpp) ? up[i - bpp] : 0;
+                    out[i] = (unsigned char)(in[i] + paeth(a, b, c));
+                }
+                break;
+            default: set_error("PNG: bad filter type %d in row %u", f, y); return IBTFO_E_IO;
         }
     }
     // to 8-bit RGBA
+    if (ctype == 6 && depth == 8) {  // already 8-bit RGBA
+        rgba.assign(img.begin(), img.end());
+        *width = (int)w;
+        *height = (int)h;
+        return IBTFO_OK;
+    }
     rgba.assign((size_t)w * h * 4, 0);
     auto sample = [&](const unsigned char* row, size_t idx) -> unsigned {  // idx-th sample of a row
         if (depth == 8) return row[idx];
