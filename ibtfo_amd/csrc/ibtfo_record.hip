@@ -13,6 +13,7 @@
 // same face inside a fused step.
 #include <hip/hip_runtime.h>
 
+#include <new>
 #include <vector>
 
 #include "ibtfo_ctx.h"
@@ -160,7 +161,8 @@ extern "C" int ibtfo_riemann_record_host(ibtfo_ctx* x, int dir, double* rec) {
         rc = riemann_on_device(x, dir, d);
         if (rc == IBTFO_OK) e = hipMemcpyAsync(rec, d, bytes, hipMemcpyDeviceToHost, x->stream);
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+    const hipError_t es = hipStreamSynchronize(x->stream);  // nothing queued still uses d
+    if (e == hipSuccess) e = es;
     (void)hipFree(d);
     if (rc == IBTFO_OK && e != hipSuccess) {
         set_error("ibtfo_riemann_record_host: %s", hipGetErrorString(e));
@@ -197,7 +199,15 @@ extern "C" int ibtfo_flux_record_host(double gamma, const double* u, double* f, 
     if (!flux_args_ok("ibtfo_flux_record_host", u, f, g, ncells, stride)) return IBTFO_E_ARG;
     if (ncells == 0) return IBTFO_OK;
     const size_t n4 = (size_t)ncells * 4;
-    std::vector<double> h(n4);
+    std::vector<double> h, hf, hg;
+    try {
+        h.resize(n4);
+        hf.resize(f ? n4 : 0);
+        hg.resize(g ? n4 : 0);
+    } catch (const std::bad_alloc&) {
+        set_error("ibtfo_flux_record_host: out of host memory");
+        return IBTFO_E_NOMEM;
+    }
     for (long long c = 0; c < ncells; c++)
         for (int k = 0; k < 4; k++) h[c * 4 + k] = u[c * stride + k];
     double *du = nullptr, *df = nullptr, *dg = nullptr;
@@ -211,7 +221,6 @@ extern "C" int ibtfo_flux_record_host(double gamma, const double* u, double* f, 
     }
     hipError_t e = hipMemcpy(du, h.data(), n4 * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = launch_flux(gamma, du, df, dg, ncells, 4, nullptr);
-    std::vector<double> hf(f ? n4 : 0), hg(g ? n4 : 0);
     if (e == hipSuccess && f) e = hipMemcpy(hf.data(), df, n4 * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess && g) e = hipMemcpy(hg.data(), dg, n4 * sizeof(double), hipMemcpyDeviceToHost);
     release();
