@@ -136,6 +136,23 @@ def test_flux_record_equals_oracle(oracle_mod, u_slot, f_slot, g_slot):
     assert nan_equal_bits(g, o) == 0
 
 
+def test_flux_record_packed_states(oracle_mod):
+    """Stride 4 (states packed one after another) and a zero-length call."""
+    import ctypes as C
+    import ibtfo_amd as ib
+    O = oracle_mod
+    rec = random_record(NX, NY, NG, seed=61).reshape(-1, 7, 4)
+    u = np.ascontiguousarray(rec[:, SLOT_HI])
+    n = u.shape[0]
+    f_o, g_o = np.zeros_like(u), np.zeros_like(u)
+    O.lib().or_flux_record(C.c_double(GAMMA), O.ptr(u), O.ptr(f_o), O.ptr(g_o), C.c_longlong(n), 4)
+    f_g, g_g = np.zeros_like(u), np.zeros_like(u)
+    L = ib.lib()
+    assert L.ibtfo_flux_record_host(C.c_double(GAMMA), ib.ptr(u), ib.ptr(f_g), ib.ptr(g_g), C.c_longlong(n), 4) == ib.OK
+    assert nan_equal_bits(f_g, f_o) == 0 and nan_equal_bits(g_g, g_o) == 0
+    assert L.ibtfo_flux_record_host(C.c_double(GAMMA), ib.ptr(u), ib.ptr(f_g), None, C.c_longlong(0), 4) == ib.OK
+
+
 INP = """N_x = {nx}
 N_y = {ny}
 num_ghost_cell = {ng}
