@@ -62,7 +62,7 @@ def lib():
                      "ibtfo_initialize", "ibtfo_upload_state", "ibtfo_download_state", "ibtfo_download_geometry",
                      "ibtfo_download_scheme_index", "ibtfo_sch_width", "ibtfo_num_ghosts", "ibtfo_dims",
                      "ibtfo_time_steps", "ibtfo_device_state", "ibtfo_set_stream", "ibtfo_set_rows",
-                     "ibtfo_set_tile", "ibtfo_tile_ex", "ibtfo_tile", "ibtfo_autotune", "ibtfo_geometry_mask", "ibtfo_geometry_png",
+                     "ibtfo_set_tile", "ibtfo_tile_ex", "ibtfo_one_kernel_steps", "ibtfo_tile", "ibtfo_autotune", "ibtfo_geometry_mask", "ibtfo_geometry_png",
                      "ibtfo_png_decode", "ibtfo_cut_counts", "ibtfo_write_vtk", "ibtfo_vtk_write_host"]:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -307,6 +307,10 @@ class Solver:
         r, a, b = C.c_int(), C.c_int(), C.c_int()
         _check(lib().ibtfo_tile(self.h, C.byref(r), C.byref(a), C.byref(b)), "ibtfo_tile")
         return r.value, a.value, b.value
+
+    def one_kernel_steps(self):
+        """True when a step runs as one kernel (the deferred finish of small grids)."""
+        return lib().ibtfo_one_kernel_steps(self.h) == 1
 
     def tile_form(self):
         """True when the row-parallel step kernel is selected."""

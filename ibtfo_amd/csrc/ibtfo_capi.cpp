@@ -783,6 +783,16 @@ extern "C" int ibtfo_initialize(ibtfo_ctx* x, int kind, const double* prm, int n
 }
 
 // -------------------------------------------------------------- time loop
+// The deferred finish (one kernel per step) applies when nothing runs between
+// the step kernel and its finish: row-parallel strips of a single-step solve on
+// the Regular boundary with the fused reference edge hook, at most DF_MAX_REC
+// blocks, and a single solve (its partial records are its own).
+static int df_eligible(const ibtfo_ctx* x, int nblocks) {
+    const ibtfo_config& c = x->cfg;
+    return x->df_part && x->rowpar && c.time == IBTFO_SINGLESTEP && c.boundary == IBTFO_REGULAR &&
+           c.edge == IBTFO_EDGE_REFERENCE && c.scheme != IBTFO_LAX && nblocks <= DF_MAX_REC;
+}
+
 void ibtfo::build_args(ibtfo_ctx* x) {
     const ibtfo_config& c = x->cfg;
     const Layout& L = x->L;
@@ -815,8 +825,7 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     // deferred finish: a step is one kernel when nothing runs between the step
     // kernel and its finish (no ghost fill, edge or merge kernel)
     a.df_part = x->df_part;
-    a.df_ok = x->df_part && a.rowpar && c.time == IBTFO_SINGLESTEP && c.boundary == IBTFO_REGULAR &&
-              c.edge == IBTFO_EDGE_REFERENCE && c.scheme != IBTFO_LAX && a.nbx * a.nby <= DF_MAX_REC;
+    a.df_ok = df_eligible(x, a.nbx * a.nby);
     a.df = 0;
     a.df_k = 0;
 
@@ -1167,6 +1176,13 @@ extern "C" int ibtfo_tile_ex(const ibtfo_ctx* x, int* rows, int* nbx, int* nby, 
     if (!x) return IBTFO_E_ARG;
     if (row_parallel) *row_parallel = x->rowpar;
     return ibtfo_tile(x, rows, nbx, nby);
+}
+
+extern "C" int ibtfo_one_kernel_steps(const ibtfo_ctx* x) {
+    if (!x) return -1;
+    int nbx, nby, r;
+    step_grid(x->L, x->rows_override, &nbx, &nby, &r);
+    return df_eligible(x, nbx * nby) ? 1 : 0;
 }
 
 extern "C" int ibtfo_autotune(ibtfo_ctx* x, int probe_steps, int* best_rows, double* best_ms) {

@@ -235,6 +235,12 @@ int ibtfo_set_tile(ibtfo_ctx* ctx, int rows, int row_parallel);
 int ibtfo_tile(const ibtfo_ctx* ctx, int* rows, int* nbx, int* nby);
 /* the same, and the form (1: row-parallel) */
 int ibtfo_tile_ex(const ibtfo_ctx* ctx, int* rows, int* nbx, int* nby, int* row_parallel);
+/* 1 when a step of this solve runs as one kernel with the current tile (the
+ * deferred finish: row-parallel strips of a single-step solve on the Regular
+ * boundary with the reference edge hook, at most 256 blocks; each ibtfo_step
+ * call or graph replay then ends with one closing kernel), 0 when a finish
+ * kernel follows each step kernel, -1 for a null context. */
+int ibtfo_one_kernel_steps(const ibtfo_ctx* ctx);
 /* Tile-shape autotuner: times `probe_steps` steps of the current solve for each
  * candidate shape (marching strips of 1 .. 256 rows, row-parallel strips of 1 ..
  * 7 rows), restoring the state and clock after each probe, and keeps the

@@ -58,9 +58,12 @@ def _run(case, rows, par, plan):
     """plan: a list of (steps, graph chunk or 0)"""
     s = _solver(case, rows, par)
     try:
+        one = s.one_kernel_steps()
         for n, chunk in plan:
             s.step(n, chunk)
-        return _result(s)
+        r = _result(s)
+        r["one_kernel"] = one
+        return r
     finally:
         s.close()
 
@@ -79,6 +82,7 @@ def test_sod_tube_deferred_equals_finish_kernel(oracle_mod, plan):
     print("deferred finish, Sod tube 400x100:", plan, flush=True)
     a = _run(c, 3, 1, PLANS[plan])   # row-parallel, 7 x 34 blocks: deferred finish
     b = _run(c, 3, 0, PLANS[plan])   # marching strips: a finish kernel per step
+    assert a["one_kernel"] and not b["one_kernel"]
     assert a["it"] == 357
     _same(a, b, plan)
     if plan == "graph100":
@@ -102,6 +106,7 @@ def test_schemes_and_modes(oracle_mod, scheme, riemann, lim, quirks):
     for rows in (1, 4, 7):
         a = _run(c, rows, 1, plan)
         b = _run(c, rows, 0, plan)
+        assert a["one_kernel"] and not b["one_kernel"]
         print("  rows %d: %d steps, status %d" % (rows, a["it"], a["rc"]), flush=True)
         assert a["it"] == 45 or (a["rc"] != 0 and a["it"] > 10)
         _same(a, b, (scheme, riemann, lim, quirks, rows))
@@ -117,6 +122,7 @@ def test_callers_dt_and_timing(oracle_mod):
     for par in (1, 0):
         s = _solver(c, 3, par)
         try:
+            assert s.one_kernel_steps() == bool(par)
             s.step(10)
             for q in range(12):
                 dt = s.status()[2] * (0.5 + 0.04 * q)
@@ -137,4 +143,5 @@ def test_more_blocks_than_the_deferred_form_takes(oracle_mod):
     c = _sod(O)
     a = _run(c, 1, 1, [(120, 100)])
     b = _run(c, 1, 0, [(120, 100)])
+    assert not a["one_kernel"] and not b["one_kernel"]
     _same(a, b, "700 blocks")
