@@ -3,8 +3,9 @@
 
 Workload: 4096 x 2048 cells on [0,2] x [0,1], Mach 3 inflow (rho, u, v, p) =
 (1.4, 3, 0, 1) past 9 seeded bodies (seed 20261202: circles and convex
-polygons) given as a binary mask image (one pixel per cell), whose level set is
-built on the device (mask -> edge pixels -> signed distance), cut-cell boundary
+polygons) given as a binary mask image (one pixel per cell) in a PNG file, which
+the library decodes and whose level set it builds on the device (mask -> edge
+pixels -> signed distance), cut-cell boundary
 with small-cell merging, MUSCL + van Leer limiter, HLL flux (the headline) and
 Steger-Warming flux (second leg), supersonic inflow / zero-gradient edges, FP64;
 a third leg runs the reference's IDW ghost-fluid boundary in place of the cut
@@ -290,7 +291,8 @@ def config_legs(args, legs=("sod", "channel", "vortex")):
     O.lib().or_set_num_threads(threads)
     out = []
 
-    def gpu_run(cfg, ls, init, apply_boundary, chunk=100):
+    def gpu_run(cfg, ls, init, apply_boundary, chunk=None):
+        chunk = args.graph_chunk if chunk is None else chunk  # 0: plain launches
         s = ib.Solver(cfg)
         s.geometry(*ls)
         s.initialize(*init)
