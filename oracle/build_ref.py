@@ -78,6 +78,9 @@ def combos():
     out.append(("pngshock", "MUSCL", "StegerWarming", "minmod", "IDWBaseGFM"))
     out.append(("pngshock", "MUSCL", "HLL", "minmod", "IDWBaseGFM"))
     out.append(("vortex", "MUSCL", "HLL", "minmod", "SimpleBound"))
+    # Lax in 2-D: the row-split Sod hook and the inclined channel with IDW ghost fluid
+    out.append(("sod", "Lax", "HLL", "minmod", "Regular"))
+    out.append(("incline", "Lax", "HLL", "minmod", "IDWBaseGFM"))
     return out
 
 

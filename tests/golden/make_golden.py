@@ -54,6 +54,10 @@ def run_cases():
                                                 Psy_time=0.3), 10))
     cases.append(("vortex_SimpleBound", O.Case(N_x=28, N_y=28, hi_y=1.0, Boundary="SimpleBound", case="vortex",
                                                extra=dict(fa=1.0, f0=0.0, f1=1.0)), 10))
+    # Lax in 2-D: the row-split Sod hook and the inclined channel with IDW ghost fluid
+    cases.append(("sod_Lax", O.Case(N_x=24, N_y=24, hi_y=1.0, Scheme="Lax", case="sod"), 12))
+    cases.append(("incline_Lax_IDW", O.Case(N_x=28, N_y=28, hi_y=1.0, Boundary="IDWBaseGFM", Scheme="Lax",
+                                            case="incline", Psy_time=0.3), 10))
     # Test/PNGGeo's shock initialiser and edge hook (adapted signatures) over InclineSod's channel
     cases.append(("pngshock_SW_IDW", O.Case(N_x=40, N_y=40, hi_y=1.0, Boundary="IDWBaseGFM", Riemann="StegerWarming",
                                             case="pngshock", edge="pnggeo", Psy_time=0.3, CFL_number=0.1,

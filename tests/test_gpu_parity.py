@@ -191,6 +191,21 @@ def test_vortex_annulus(oracle_mod):
     assert nan_equal_bits(g["U_OLD"], o["U_OLD"]) == 0
 
 
+@pytest.mark.parametrize("case,n", [("sod", 64), ("sod", 130), ("incline", 96)])
+def test_lax_2d_bitexact(oracle_mod, case, n):
+    """Lax in 2-D (the oracle's Lax equals the reference's build here too): the
+    row-split Sod hook on the Regular boundary and the inclined channel with
+    IDW ghost fluid, 40 steps: state and every dt bit for bit."""
+    O = oracle_mod
+    kw = dict(Boundary="IDWBaseGFM", Psy_time=0.3) if case == "incline" else {}
+    c = _case(O, N_x=n, N_y=n, hi_y=1.0, Scheme="Lax", case=case, **kw)
+    g = gpu_run(c, 40)
+    o = O.run_oracle(c, 40, root_mode=1)
+    assert g["steps"] == o["steps"] == 40
+    assert nan_equal_bits(g["U_OLD"], o["U_OLD"]) == 0
+    assert nan_equal_bits(g["dt_log"], o["dt"]) == 0
+
+
 def test_lax_and_pvrs(oracle_mod):
     """Lax (no Riemann call) bit-exact; PVRS within the FP64 tolerance (its
     pow() calls for the wave and fan states use the device libm)."""

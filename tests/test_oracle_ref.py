@@ -113,3 +113,12 @@ def test_pvrs_2d_stray_store(oracle_mod, scheme):
     assert len(d) > 0
     rows = sorted({int(q // 4 // c.M) for q in d})
     assert rows[0] in (0, 1, 2, 3) and all(q % 4 == 3 for q in d)
+
+
+@pytest.mark.parametrize("case", ["sod", "incline"])
+def test_lax_2d(oracle_mod, case):
+    """Lax (its own 16-double scratch record, no Riemann solver) in 2-D: the
+    row-split Sod hook (Regular) and the inclined channel with IDW ghost fluid."""
+    O = oracle_mod
+    kw = dict(Boundary="IDWBaseGFM", Psy_time=0.3) if case == "incline" else {}
+    _cmp(O, O.Case(N_x=64, N_y=64, hi_y=1.0, Scheme="Lax", case=case, **kw), 40)
