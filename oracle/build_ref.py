@@ -81,6 +81,10 @@ def combos():
     # Lax in 2-D: the row-split Sod hook and the inclined channel with IDW ghost fluid
     out.append(("sod", "Lax", "HLL", "minmod", "Regular"))
     out.append(("incline", "Lax", "HLL", "minmod", "IDWBaseGFM"))
+    # more scheme / solver / limiter combinations on the IDW channel
+    for s, r, l in [("Godunov", "HLL", "minmod"), ("Godunov", "StegerWarming", "minmod"),
+                    ("MUSCL", "HLL", "VanLeer"), ("MHM", "StegerWarming", "minmod"), ("MHM", "HLL", "VanLeer")]:
+        out.append(("incline", s, r, l, "IDWBaseGFM"))
     return out
 
 

@@ -58,6 +58,11 @@ def run_cases():
     cases.append(("sod_Lax", O.Case(N_x=24, N_y=24, hi_y=1.0, Scheme="Lax", case="sod"), 12))
     cases.append(("incline_Lax_IDW", O.Case(N_x=28, N_y=28, hi_y=1.0, Boundary="IDWBaseGFM", Scheme="Lax",
                                             case="incline", Psy_time=0.3), 10))
+    for s, r, l in [("Godunov", "HLL", "minmod"), ("Godunov", "StegerWarming", "minmod"),
+                    ("MUSCL", "HLL", "VanLeer"), ("MHM", "StegerWarming", "minmod"), ("MHM", "HLL", "VanLeer")]:
+        cases.append(("incline_%s_%s_%s_IDW" % (s, r, l),
+                      O.Case(N_x=28, N_y=28, hi_y=1.0, Boundary="IDWBaseGFM", Scheme=s, Riemann=r, Limiter=l,
+                             case="incline", Psy_time=0.3), 10))
     # Test/PNGGeo's shock initialiser and edge hook (adapted signatures) over InclineSod's channel
     cases.append(("pngshock_SW_IDW", O.Case(N_x=40, N_y=40, hi_y=1.0, Boundary="IDWBaseGFM", Riemann="StegerWarming",
                                             case="pngshock", edge="pnggeo", Psy_time=0.3, CFL_number=0.1,

@@ -191,6 +191,25 @@ def test_vortex_annulus(oracle_mod):
     assert nan_equal_bits(g["U_OLD"], o["U_OLD"]) == 0
 
 
+@pytest.mark.parametrize("scheme,rs,lim", [("Godunov", "HLL", "minmod"), ("Godunov", "StegerWarming", "minmod"),
+                                           ("MUSCL", "HLL", "VanLeer"), ("MHM", "StegerWarming", "minmod"),
+                                           ("MHM", "HLL", "VanLeer")])
+def test_incline_idw_combos_bitexact(oracle_mod, scheme, rs, lim):
+    """Scheme / solver / limiter combinations on the 96^2 IDW channel in
+    reference mode (the oracle's forms equal the reference's builds): state and
+    every dt and time bit for bit over 30 steps. The reference's van Leer macro
+    drives this channel non-finite after about 23 steps, which ends the run (dt
+    is NaN): both sides must end it at the same step with the same NaNs."""
+    O = oracle_mod
+    c = _case(O, N_x=96, N_y=96, hi_y=1.0, Boundary="IDWBaseGFM", Scheme=scheme, Riemann=rs, Limiter=lim,
+              case="incline", Psy_time=0.3)
+    g = gpu_run(c, 30)
+    o = O.run_oracle(c, 30, root_mode=1)
+    assert g["steps"] == o["steps"] and g["steps"] >= (20 if lim == "VanLeer" else 30)
+    assert nan_equal_bits(g["U_OLD"], o["U_OLD"]) == 0
+    assert nan_equal_bits(g["dt_log"], o["dt"]) == 0 and nan_equal_bits(g["t_log"], o["t"]) == 0
+
+
 @pytest.mark.parametrize("case,n", [("sod", 64), ("sod", 130), ("incline", 96)])
 def test_lax_2d_bitexact(oracle_mod, case, n):
     """Lax in 2-D (the oracle's Lax equals the reference's build here too): the

@@ -122,3 +122,14 @@ def test_lax_2d(oracle_mod, case):
     O = oracle_mod
     kw = dict(Boundary="IDWBaseGFM", Psy_time=0.3) if case == "incline" else {}
     _cmp(O, O.Case(N_x=64, N_y=64, hi_y=1.0, Scheme="Lax", case=case, **kw), 40)
+
+
+@pytest.mark.parametrize("scheme,rs,lim", [("Godunov", "HLL", "minmod"), ("Godunov", "StegerWarming", "minmod"),
+                                           ("MUSCL", "HLL", "VanLeer"), ("MHM", "StegerWarming", "minmod"),
+                                           ("MHM", "HLL", "VanLeer")])
+def test_incline_idw_combos(oracle_mod, scheme, rs, lim):
+    """Scheme / solver / limiter combinations in 2-D on the IDW channel,
+    reference mode (the van Leer macro expansion included)."""
+    O = oracle_mod
+    _cmp(O, O.Case(N_x=64, N_y=64, hi_y=1.0, Boundary="IDWBaseGFM", Scheme=scheme, Riemann=rs, Limiter=lim,
+                   case="incline", Psy_time=0.3), 40)
