@@ -246,6 +246,10 @@ static void riemann_record(int dir, const int N_x, const int N_y, const int ng, 
         fprintf(stderr, "ibtfo: %s: grid or gamma differs from the solve's .inp\n", who);
         abort();
     }
+    if (b->cfg.scheme == IBTFO_LAX) {  // its record is 16 doubles a cell, and its build links no Riemann solver
+        fprintf(stderr, "ibtfo: %s: the Lax scheme has no Riemann solver\n", who);
+        abort();
+    }
     check(ibtfo_riemann_record_host(b->ctx, dir, U_TMP), who);
 }
 
@@ -259,23 +263,41 @@ void Riemann_solverY(const double, const double, const int N_x, const int N_y, c
     riemann_record(1, N_x, N_y, ng, gamma, U_OLD, U_TMP, ndevices, device, "Riemann_solverY");
 }
 
-// Conserve2Flux.cpp: M*N states at a stride of one scratch record (28 doubles)
+// Conserve2Flux.cpp: M*N states at a stride of one scratch record, whose width
+// the scheme's EQDefine.H fixes at link time: 4 slots of 4 doubles for Lax
+// (Lax/EQDefine.H:5-6), 7 for the others (MUSCL/EQDefine.H:5-6). These calls
+// carry no solve's array, so the width is the registered solves' scheme's.
 static long long record_cells(int N_x, int N_y, int ng) { return (long long)(N_x + 2 * ng) * (N_y + 2 * ng); }
+static int record_width(const char* who) {
+    int w = 0;
+    for (const auto& kv : g_present) {
+        const int wk = kv.second->cfg.scheme == IBTFO_LAX ? 16 : 28;
+        if (w && w != wk) {
+            fprintf(stderr, "ibtfo: %s: the registered solves' schemes have scratch records of different widths\n", who);
+            abort();
+        }
+        w = wk;
+    }
+    return w ? w : 28;
+}
 
 void Conserve2Flux(const int N_x, const int N_y, const int ng, const double gamma, double* U, double* F, double* G,
                    const int ndevices, const int device) {
     one_device(ndevices, device, "Conserve2Flux");
-    check(ibtfo_flux_record_host(gamma, U, F, G, record_cells(N_x, N_y, ng), 28), "Conserve2Flux");
+    check(ibtfo_flux_record_host(gamma, U, F, G, record_cells(N_x, N_y, ng), record_width("Conserve2Flux")),
+          "Conserve2Flux");
 }
 
 void Conserve2FluxX(const int N_x, const int N_y, const int ng, const double gamma, double* U, double* F,
                     const int ndevices, const int device) {
     one_device(ndevices, device, "Conserve2FluxX");
-    check(ibtfo_flux_record_host(gamma, U, F, nullptr, record_cells(N_x, N_y, ng), 28), "Conserve2FluxX");
+    check(ibtfo_flux_record_host(gamma, U, F, nullptr, record_cells(N_x, N_y, ng), record_width("Conserve2FluxX")),
+          "Conserve2FluxX");
 }
 
 void Conserve2FluxY(const int N_x, const int N_y, const int ng, const double gamma, double* U, double* G,
                     const int ndevices, const int device) {
     one_device(ndevices, device, "Conserve2FluxY");
-    check(ibtfo_flux_record_host(gamma, U, nullptr, G, record_cells(N_x, N_y, ng), 28), "Conserve2FluxY");
+    check(ibtfo_flux_record_host(gamma, U, nullptr, G, record_cells(N_x, N_y, ng), record_width("Conserve2FluxY")),
+          "Conserve2FluxY");
 }

@@ -4,13 +4,15 @@
 // scratch record U_TMP that it fills itself on the host.
 //
 // Usage: ref_record_driver case.inp rec.bin outdir
-// rec.bin holds M*N*28 doubles. For each operator the driver starts from
-// rec.bin and writes the whole record afterwards:
-//   x.bin   Riemann_solverX
-//   y.bin   Riemann_solverY
-//   fx.bin  Conserve2FluxX, slot 5 -> slot 3 (U_L -> F_L)
-//   fy.bin  Conserve2FluxY, slot 6 -> slot 4 (U_U -> G_U)
-//   f.bin   Conserve2Flux, slot 5 -> slots 3 (x) and 0 (y)
+// rec.bin holds M*N*W doubles, W = 28 (16 when the .inp selects Scheme = Lax,
+// whose record has 4 slots: Lax/EQDefine.H:5-6). For each operator the driver
+// starts from rec.bin and writes the whole record afterwards:
+//   x.bin   Riemann_solverX                                 (not for Lax)
+//   y.bin   Riemann_solverY                                 (not for Lax)
+//   fx.bin  Conserve2FluxX, slot 5 -> slot 3 (U_L -> F_L); Lax: slot 0 -> slot 2
+//   fy.bin  Conserve2FluxY, slot 6 -> slot 4 (U_U -> G_U); Lax: slot 0 -> slot 3
+//   f.bin   Conserve2Flux, slot 5 -> slots 3 (x) and 0 (y); Lax: slot 0 -> slots 2 and 3
+//           (Lax/Lax_Scheme.cpp:41)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -50,9 +52,13 @@ int main(int argc, char** argv) {
     const double gamma = num(filename, "gamma", 1.4);
     const double Psy_L = num(filename, "hi_x", 1) - num(filename, "lo_x", 0);
     const double Psy_H = num(filename, "hi_y", 1) - num(filename, "lo_y", 0);
+    char scheme[64] = {0};
+    ibtfo_inp_string(filename, "Scheme", scheme, (int)sizeof scheme);
+    const bool lax = std::string(scheme).rfind("Lax", 0) == 0;
+    const size_t W = lax ? 16 : 28;
     const size_t cells = (size_t)(N_x + 2 * ng) * (N_y + 2 * ng);
     std::vector<double> U_OLD(cells * 4), U_NEW(cells * 4), XYCOORD(cells * 6), SCHEME_IDX(cells * 16);
-    std::vector<double> rec(cells * 28), U_TMP(cells * 28);
+    std::vector<double> rec(cells * W), U_TMP(cells * W);
     FILE* f = fopen(argv[2], "rb");
     if (!f || fread(rec.data(), sizeof(double), rec.size(), f) != rec.size()) {
         fprintf(stderr, "cannot read %s (%zu doubles)\n", argv[2], rec.size());
@@ -66,20 +72,24 @@ int main(int argc, char** argv) {
         return 1;
     }
     double* T = U_TMP.data();
+    if (!lax) {
+        U_TMP = rec;
+        Riemann_solverX(Psy_L, Psy_H, N_x, N_y, ng, gamma, U_OLD.data(), T, 1, 0);
+        save(out + "/x.bin", T, U_TMP.size());
+        U_TMP = rec;
+        Riemann_solverY(Psy_L, Psy_H, N_x, N_y, ng, gamma, U_OLD.data(), T, 1, 0);
+        save(out + "/y.bin", T, U_TMP.size());
+    }
+    const int su = lax ? 0 : 5, sfx = lax ? 2 : 3, suy = lax ? 0 : 6, sgy = lax ? 3 : 4;
     U_TMP = rec;
-    Riemann_solverX(Psy_L, Psy_H, N_x, N_y, ng, gamma, U_OLD.data(), T, 1, 0);
-    save(out + "/x.bin", T, U_TMP.size());
-    U_TMP = rec;
-    Riemann_solverY(Psy_L, Psy_H, N_x, N_y, ng, gamma, U_OLD.data(), T, 1, 0);
-    save(out + "/y.bin", T, U_TMP.size());
-    U_TMP = rec;
-    Conserve2FluxX(N_x, N_y, ng, gamma, &T[5 * 4], &T[3 * 4], 1, 0);
+    Conserve2FluxX(N_x, N_y, ng, gamma, &T[su * 4], &T[sfx * 4], 1, 0);
     save(out + "/fx.bin", T, U_TMP.size());
     U_TMP = rec;
-    Conserve2FluxY(N_x, N_y, ng, gamma, &T[6 * 4], &T[4 * 4], 1, 0);
+    Conserve2FluxY(N_x, N_y, ng, gamma, &T[suy * 4], &T[sgy * 4], 1, 0);
     save(out + "/fy.bin", T, U_TMP.size());
     U_TMP = rec;
-    Conserve2Flux(N_x, N_y, ng, gamma, &T[5 * 4], &T[3 * 4], &T[0], 1, 0);
+    if (lax) Conserve2Flux(N_x, N_y, ng, gamma, &T[0], &T[2 * 4], &T[3 * 4], 1, 0);
+    else Conserve2Flux(N_x, N_y, ng, gamma, &T[5 * 4], &T[3 * 4], &T[0], 1, 0);
     save(out + "/f.bin", T, U_TMP.size());
     ibtfo_ref_exit(U_OLD.data());
     printf("ref_record_driver: %zu cells\n", cells);

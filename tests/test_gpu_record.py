@@ -190,6 +190,35 @@ def test_reference_signature_record_driver(oracle_mod, tmp_path, riemann):
         assert nan_equal_bits(got[k], o) == 0, k
 
 
+def test_reference_signature_lax_record(oracle_mod, tmp_path):
+    """Conserve2Flux{,X,Y} called by a driver whose .inp selects Lax: the
+    library takes Lax's 16-double record (state in slot 0, x flux to slot 2, y
+    flux to slot 3, Lax_Scheme.cpp:41): bit for bit the oracle's."""
+    import ctypes as C
+    O = oracle_mod
+    inp = INP.format(nx=NX, ny=NY, ng=NG, gamma=GAMMA, riemann="HLL").replace("Scheme = MUSCL", "Scheme = Lax")
+    (tmp_path / "case.inp").write_text(inp)
+    rec = random_record(NX, NY, NG, seed=71).reshape(-1, 7, 4)[:, [5, 1, 2, 3]].copy().reshape(-1)
+    rec.tofile(tmp_path / "rec.bin")
+    out = tmp_path / "out"
+    out.mkdir()
+    print("record driver: Lax", flush=True)
+    r = subprocess.run([DRIVER, "case.inp", "rec.bin", str(out)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    n = rec.size // 16
+
+    def at(arr, slot):
+        return C.cast(arr.ctypes.data + 32 * slot, C.POINTER(C.c_double))
+    for k, (fs, gs) in (("fx", (2, None)), ("fy", (None, 3)), ("f", (2, 3))):
+        o = rec.copy()
+        O.lib().or_flux_record(C.c_double(GAMMA), at(o, 0), at(o, fs) if fs else None, at(o, gs) if gs else None,
+                               C.c_longlong(n), 16)
+        got = np.fromfile(out / (k + ".bin"))
+        assert got.size == rec.size and nan_equal_bits(got, o) == 0, k
+    assert not (out / "x.bin").exists()
+
+
 def test_riemann_record_arguments(oracle_mod):
     import ibtfo_amd as ib
     s = _solver("HLL", 1)

@@ -89,3 +89,29 @@ def test_flux_record_closed_form(oracle_mod):
     assert (np.abs(r[:, SLOT_LO_FLUX] - F).max(axis=0) <= 1e-15 * 8 * scale).all()
     scale = np.abs(G).max(axis=0)
     assert (np.abs(r[:, SLOT_HI_FLUX] - G).max(axis=0) <= 1e-15 * 8 * scale).all()
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_conserve2flux_lax_record_equals_reference(oracle_mod, which):
+    """Lax's 16-double record (Lax/EQDefine.H:5-6): the state in slot 0, its x
+    flux to slot 2 and its y flux to slot 3, as Lax_Scheme.cpp:41 calls it,
+    against the reference's Conserve2Flux built with that record width."""
+    O = oracle_mod
+    c = O.Case(N_x=64, N_y=16, hi_y=0.25, Scheme="Lax", case="sodx")
+    if not O.ref_available(c):
+        pytest.skip("oracle/_ref not built")
+    R = O.ref_lib(O.ref_lib_name(c))
+    assert R.ref_tmp_width() == 16
+    rec = random_record(NX, NY, NG, seed=9).reshape(-1, 7, 4)[:, [5, 1, 2, 3]].copy().reshape(-1)
+    a, b = rec.copy(), rec.copy()
+    n = rec.size // 16
+
+    def at(arr, slot):
+        return C.cast(arr.ctypes.data + 8 * 4 * slot, C.POINTER(C.c_double))
+    f = which in (0, 2)
+    g = which in (1, 2)
+    O.lib().or_flux_record(C.c_double(GAMMA), at(a, 0), at(a, 2) if f else None, at(a, 3) if g else None,
+                           C.c_longlong(n), 16)
+    R.ref_conserve2flux(which, NX, NY, NG, C.c_double(GAMMA), at(b, 0), at(b, 2), at(b, 3))
+    assert nan_equal_bits(a, b) == 0
+    assert (a != rec).sum() > 0
