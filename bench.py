@@ -309,6 +309,7 @@ def config_legs(args, legs=("sod", "channel", "vortex")):
         it, t, dt, rc = s.status()
         s.tuned_rows = rows
         s.tuned_form = "row-parallel" if s.tile_form() else "strip"
+        s.tuned_one_kernel = s.one_kernel_steps()  # the deferred finish: one kernel per step
         return s, n, ms, t, rc
 
     def cpu_run(case, nsteps):
@@ -352,7 +353,8 @@ def config_legs(args, legs=("sod", "channel", "vortex")):
         _say("Sod tube measured")
         return dict(workload="Sod tube 400x100 on [0,1]x[0,0.25], MHM + minmod + HLLC, Regular, to t = 0.2",
                     steps=n, gpu_value=400 * 100 * n / (ms / 1e3), gpu_ms=ms, gpu_us_per_step=1e3 * ms / n,
-                    roofline_frac=_cfg_frac(400 * 100, n, ms), tile_rows=s.tuned_rows, tile_form=s.tuned_form, status=rc,
+                    roofline_frac=_cfg_frac(400 * 100, n, ms), tile_rows=s.tuned_rows, tile_form=s.tuned_form, one_kernel_steps=s.tuned_one_kernel,
+                    status=rc,
                     cpu_value=400 * 100 * cn / cs, cpu_steps=cn, cpu_cores=threads, cpu_reference=ref)
 
     def channel_leg():
@@ -369,7 +371,8 @@ def config_legs(args, legs=("sod", "channel", "vortex")):
         return dict(workload="inclined Sod channel 800x400 on [0,2]x[0,1] (30 deg, AxisD 0.4), MUSCL + minmod "
                              "+ HLL, IDW ghost fluid, to t = 0.2",
                     steps=n, gpu_value=800 * 400 * n / (ms / 1e3), gpu_ms=ms, gpu_us_per_step=1e3 * ms / n,
-                    roofline_frac=_cfg_frac(800 * 400, n, ms), tile_rows=s.tuned_rows, tile_form=s.tuned_form, status=rc,
+                    roofline_frac=_cfg_frac(800 * 400, n, ms), tile_rows=s.tuned_rows, tile_form=s.tuned_form, one_kernel_steps=s.tuned_one_kernel,
+                    status=rc,
                     cpu_value=800 * 400 * cn / cs, cpu_steps=cn, cpu_cores=threads,
                     cpu_note="first %d steps of the run" % cn, cpu_reference=ref)
 
