@@ -386,6 +386,10 @@ static int rebuild_tables(ibtfo_batch* b) {
         b->h_fin[i] = x->fin;
         b->h_fill[i] = x->fa;
         b->h_edge[i] = x->ea;
+        // the batch runs every phase of a step as its own launch (the edge ring
+        // included, and the RK2 stage flip), whatever a lone solve would fuse
+        if (b->h_step[i].edge_fused == 2) b->h_step[i].edge_fused = 0;
+        b->h_fin[i].keep_cur = 0;
         for (int lb = 0; lb < step_blocks(x); lb++) b->h_smap[ns++] = make_int2(i, lb);
         for (int gb = 0; gb < ghost_blocks(x); gb++) b->h_gmap[ngh++] = make_int2(i, gb * FILL_THREADS);
         for (int rb = 0; rb < ring_blocks(x); rb++) b->h_rmap[nr++] = make_int2(i, rb * EDGE_THREADS);

@@ -793,6 +793,11 @@ static int df_eligible(const ibtfo_ctx* x, int nblocks) {
            c.edge == IBTFO_EDGE_REFERENCE && c.scheme != IBTFO_LAX && nblocks <= DF_MAX_REC;
 }
 
+// An RK2 step needs no kernel between its stages when the step kernel writes the
+// periodic ring itself (Regular boundary: no ghost fill): stage 2 then reads U1
+// from the buffer stage 1 wrote, with no finish making it current.
+static bool rk2_in_place(const ibtfo_ctx* x) { return x->sa.edge_fused == 2; }
+
 void ibtfo::build_args(ibtfo_ctx* x) {
     const ibtfo_config& c = x->cfg;
     const Layout& L = x->L;
@@ -813,6 +818,11 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     a.max_iter = c.max_iter;
     a.quirks = c.quirks;
     a.edge_fused = (c.edge == IBTFO_EDGE_REFERENCE && c.scheme != IBTFO_LAX) ? 1 : 0;
+    // a periodic ring with no body is a copy of the new interior: the step kernel
+    // writes it beside its stores, and no edge kernel runs
+    if (c.edge == IBTFO_EDGE_PERIODIC && c.boundary == IBTFO_REGULAR && c.scheme != IBTFO_LAX && L.nx >= L.ng &&
+        L.ny >= L.ng)
+        a.edge_fused = 2;
     // the reference's U_NEW ring is refreshed by Boundary's copy back (GFM family),
     // else it keeps the uploaded values (Regular, SimpleBound)
     a.corner_from_input = c.boundary != IBTFO_REGULAR && c.boundary != IBTFO_SIMPLE_BOUND;
@@ -853,6 +863,7 @@ void ibtfo::build_args(ibtfo_ctx* x) {
     fn.ctl = x->ctl; fn.part = x->mass_part; fn.nblocks = a.nbx * a.nby;
     fn.cfl = c.CFL_number; fn.dx = a.dx; fn.t_end = c.Psy_time; fn.max_iter = c.max_iter;
     fn.advance = 1; fn.reset = 0;
+    fn.keep_cur = c.time == IBTFO_RK2 && rk2_in_place(x);
     fn.fold = x->fold;
     if (c.boundary == IBTFO_CUTCELL && x->ngroups) {  // the merge kernel's records follow the step kernel's
         fn.part2 = x->merge_part;
@@ -897,7 +908,7 @@ static int enqueue_boundaries(ibtfo_ctx* x) {
     if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_fill_levels(x, x->fa));
     if (c.edge == IBTFO_EDGE_REFERENCE && c.scheme == IBTFO_LAX)
         HIPCHK(launch_edge_ref(x->ea, x->corner, x->sa.corner_from_input, x->stream));
-    if (c.edge != IBTFO_EDGE_REFERENCE) HIPCHK(launch_edge(x->ea, x->stream));
+    if (c.edge != IBTFO_EDGE_REFERENCE && x->sa.edge_fused != 2) HIPCHK(launch_edge(x->ea, x->stream));
     if (c.boundary != IBTFO_REGULAR) HIPCHK(launch_scatter(x->fa, x->stream));
     return IBTFO_OK;
 }
@@ -911,18 +922,23 @@ static int enqueue_step(ibtfo_ctx* x, hipEvent_t* ev = nullptr, int k = 0) {
     int rc;
     if (c.time == IBTFO_RK2) {
         // Heun: U1 = U^n + dt L(U^n) into the other buffer, made current, with its
-        // boundaries; then (U^n + U1 + dt L(U1)) / 2 written over U^n
+        // boundaries; then (U^n + U1 + dt L(U1)) / 2 written over U^n. With nothing
+        // to run between the stages, stage 2 reads U1 from the other buffer (stage
+        // 3) instead of a finish making it current, and the step's finish keeps cur.
+        const bool in_place = rk2_in_place(x);
         StepArgs s1 = x->sa;
         s1.stage = 1;
         if (ev) HIPCHK(hipEventRecord(ev[0], x->stream));
         HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, s1, x->stream));
         if (ev) HIPCHK(hipEventRecord(ev[1], x->stream));
-        FinishArgs flip = x->fin;
-        flip.advance = 2;
-        HIPCHK(launch_finish(flip, x->stream));
-        if ((rc = enqueue_boundaries(x))) return rc;
+        if (!in_place) {
+            FinishArgs flip = x->fin;
+            flip.advance = 2;
+            HIPCHK(launch_finish(flip, x->stream));
+            if ((rc = enqueue_boundaries(x))) return rc;
+        }
         StepArgs s2 = x->sa;
-        s2.stage = 2;
+        s2.stage = in_place ? 3 : 2;
         if (ev) HIPCHK(hipEventRecord(ev[2], x->stream));
         HIPCHK(launch_step(c.scheme, c.riemann, c.limiter, c.precision, s2, x->stream));
         if (ev) HIPCHK(hipEventRecord(ev[3], x->stream));

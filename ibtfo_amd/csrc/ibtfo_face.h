@@ -87,6 +87,28 @@ __device__ __forceinline__ void block_partials(double* part, int bid, double ama
     }
 }
 
+// Periodic ring written by the step kernel (StepArgs::edge_fused = 2): ghost
+// (i, j) holds interior (i +- nx, j +- ny), each coordinate wrapped on its own,
+// as the edge kernel copies it (nx, ny >= ng, so every source is interior). The
+// store of interior cell (c, j) writes its images across the x edge, the y edge
+// and the corner: none for a cell more than ng cells from both edges.
+__device__ __forceinline__ void periodic_images(double* __restrict__ Uo, const Layout& L, const int c, const int j,
+                                                const double (&e)[4]) {
+    const int ng = L.ng, nx = L.nx, ny = L.ny;
+    const int xs[3] = {c, c < 2 * ng ? c + nx : -1, c >= nx ? c - nx : -1};
+    const int ys[3] = {j, j < 2 * ng ? j + ny : -1, j >= ny ? j - ny : -1};
+    if (xs[1] < 0 && xs[2] < 0 && ys[1] < 0 && ys[2] < 0) return;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            if ((p == 0 && q == 0) || xs[p] < 0 || ys[q] < 0) continue;
+            const long long o = L.at(xs[p], ys[q]);
+#pragma unroll
+            for (int k = 0; k < 4; k++) Uo[o + k * L.plane] = e[k];
+        }
+}
+
 // Primitive view of an FP64 conserved state in the compute type T. FP64:
 // pstate itself (correctly rounded, the reference's order). Mixed precision:
 // 1/rho, u, v and p = (gamma-1)(E - rho |u|^2 / 2) are recovered in FP64 (no

@@ -81,7 +81,9 @@ struct StepArgs {
     double r_gam, r_gm1, r_dx, r_dy;  // correctly rounded reciprocals of the constant divisors
     int max_iter;
     int quirks;
-    int edge_fused;         // write the reference Global_Boundary ring inside the step
+    int edge_fused;         // the step kernel writes the output's ring: 1 the reference
+                            // Global_Boundary ring, 2 the periodic ring (Regular boundary,
+                            // nx, ny >= ng); 0: an edge kernel writes it after the step
     // SimpleBound: the reference's U_NEW of non-fluid cells, which its Boundary
     // never refreshes (no copy back), as Global_Boundary reads it; else null
     const double* unew_nf;
@@ -90,6 +92,8 @@ struct StepArgs {
     int nbx, nby;           // grid
     int stage;              // 0: single-step update; RK2 (Heun): 1 = U1 = U^n + dt L(U^n),
                             // 2 = (U^n + U1 + dt L(U1)) / 2, written over U^n in place
+                            // (after a finish made U1 current); 3 = stage 2 with no such
+                            // flip: reads U1 from buf[cur ^ 1], writes over U^n in buf[cur]
     const CutCell* cut;     // cut-cell records (type-3 cells), else null
     const int* cut_index;   // plane: record of each type-3 cell
     int rowpar;             // row-parallel form (k_step_rowpar: rows + 1 waves per block)
@@ -203,6 +207,8 @@ struct FinishArgs {
     int advance;  // 1: a step was taken (advance the clock); 0: stand-alone ComputeDt;
                   // 2: RK2 stage boundary (make the stage-1 buffer current, no reduction)
     int reset;    // with advance = 0: start the run clock
+    int keep_cur; // with advance = 1: leave cur as it is (an RK2 step whose stage 2 ran as
+                  // stage 3, over U^n in the current buffer), else make the output current
     // two-level reduction (advance = 1 with many partials): nfold blocks each fold
     // `fold_per` consecutive partials into fold[] first, then one block finishes
     double* fold;

@@ -61,7 +61,7 @@ __device__ __forceinline__ void update_cell(const StepArgs& a, const Layout& L, 
             double d0 = 0.0, d1 = 0.0, d2 = 0.0;
             int db = 0;
             floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, d0, d1, d2, db);
-            if (stage == 2) {
+            if (stage >= 2) {
 #pragma unroll
                 for (int k = 0; k < 4; k++) un[k] = 0.5 * (Uo[o + k * PL] + un[k]);
                 dt_terms(a.gam, a.gm1, un, amax, umax, vmax, bad);
@@ -99,7 +99,8 @@ __device__ __forceinline__ void update_cell(const StepArgs& a, const Layout& L, 
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) Uo[o + k * PL] = un[k];
-    if (a.edge_fused) {
+    if (a.edge_fused == 2) periodic_images(Uo, L, c, j, un);
+    if (a.edge_fused == 1) {
         // Test/Sod/Global_Boudary.cpp:24-59 (rows) and :62-75 (columns), which
         // read U_NEW: the updated state, or for a non-fluid cell under
         // SimpleBound the value its U_NEW kept
@@ -270,7 +271,7 @@ __device__ __forceinline__ void step_rowpar(const StepArgs& a, const int bx, con
             if (bx == 0 && by == 0 && threadIdx.x == 0) ctl->ran = 0;
             return;
         }
-        cur = ctl->cur;
+        cur = ctl->cur ^ (stage == 3 ? 1 : 0);  // the input buffer
         dt = ctl->dt;
     }
     const double* __restrict__ Ui = a.buf[cur];
@@ -292,7 +293,7 @@ __device__ __forceinline__ void step_rowpar(const StepArgs& a, const int bx, con
     __shared__ double sp[ROWPAR_MAX_ROWS + 1][5];
 
     // reference Global_Boundary corners (Test/Sod/Global_Boudary.cpp:62-75), as in step_strip
-    if (a.edge_fused && bx == 0 && by == 0 && wv == 0 && lane < 4 * ng * ng) {
+    if (a.edge_fused == 1 && bx == 0 && by == 0 && wv == 0 && lane < 4 * ng * ng) {
         int side = lane / (2 * ng * ng);
         int rem = lane % (2 * ng * ng);
         int ri = rem / ng, cc = rem % ng;

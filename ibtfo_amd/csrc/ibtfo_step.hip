@@ -53,7 +53,7 @@ __device__ __forceinline__ void finish_clock(const FinishArgs& f, Ctl* ctl, doub
         ctl->now_t = tn;
         ctl->iter = it;
         ctl->done = !(tn < f.t_end && it < f.max_iter);
-        ctl->cur ^= 1;
+        if (!f.keep_cur) ctl->cur ^= 1;
         ctl->ran = 1;
     } else {
         ctl->mass_now = m;
@@ -248,7 +248,9 @@ __global__ __launch_bounds__(FINISH_THREADS) void k_finish_batch(const FinishArg
 // The step body for one 64-lane block: strip (bx, by) of the solve described
 // by `a`; `stage` as StepArgs::stage. Used by the single-solve kernel and by the
 // batched kernel, whose blocks each pick their solve from a block map.
-template <int SCH, int RS, int LIM, typename T, bool CUT>
+// PER: the periodic ring of the output is written beside the stores (edge_fused = 2;
+// its own kernel, so that the other kernels' register allocation is not touched).
+template <int SCH, int RS, int LIM, typename T, bool CUT, bool PER = false>
 __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, const int by, const int stage) {
     constexpr bool FP64 = sizeof(T) == sizeof(double);
     Ctl* ctl = a.ctl;
@@ -258,7 +260,7 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
         return;
     }
     const Layout L = a.L;
-    const int cur = ctl->cur;
+    const int cur = ctl->cur ^ (stage == 3 ? 1 : 0);  // the input buffer
     const double dt = ctl->dt;
     const double* __restrict__ Ui = a.buf[cur];
     double* __restrict__ Uo = a.buf[cur ^ 1];
@@ -291,7 +293,7 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
 
     // reference Global_Boundary corners (Test/Sod/Global_Boudary.cpp:62-75 applied to
     // ring rows): U_OLD(i<ng or i>=N_x+ng, j ring) = U_NEW(ng or N_x+ng-1, j)
-    if (a.edge_fused && bx == 0 && by == 0 && lane < 4 * ng * ng) {
+    if (a.edge_fused == 1 && bx == 0 && by == 0 && lane < 4 * ng * ng) {
         int side = lane / (2 * ng * ng);
         int rem = lane % (2 * ng * ng);
         int ri = rem / ng, cc = rem % ng;
@@ -409,7 +411,7 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
                         double d0 = 0.0, d1 = 0.0, d2 = 0.0;
                         int db = 0;
                         floor_and_dt(a.gam, a.gm1, r_gm1, un, a.quirks, d0, d1, d2, db);
-                        if (stage == 2) {
+                        if (stage >= 2) {
 #pragma unroll
                             for (int k = 0; k < 4; k++) un[k] = 0.5 * (Uo[o + k * PL] + un[k]);
                             dt_terms(a.gam, a.gm1, un, amax, umax, vmax, bad);
@@ -447,7 +449,7 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
                 }
 #pragma unroll
                 for (int k = 0; k < 4; k++) Uo[o + k * PL] = un[k];
-                if (a.edge_fused) {
+                if (a.edge_fused == 1) {
                     // Test/Sod/Global_Boudary.cpp:24-59 (rows) and :62-75 (columns), which
                     // read U_NEW: the updated state, or for a non-fluid cell under
                     // SimpleBound the value its U_NEW kept
@@ -488,12 +490,26 @@ __device__ __forceinline__ void step_strip(const StepArgs& a, const int bx, cons
             w[2][k] = nw[k];
         }
     }
+    if (PER && (bx == 0 || bx == a.nbx - 1 || j0 < 2 * ng || j1 > ny)) {
+        // the strip's cells within ng of an edge, re-read from this lane's own
+        // stores above (after the row loop, where few registers are live)
+        for (int j = j0; j < j1; j++) {
+            if (!upd_col || (c >= 2 * ng && c < nx && j >= 2 * ng && j < ny)) continue;
+            const long long o = L.at(c, j);
+            const double e[4] = {Uo[o], Uo[o + PL], Uo[o + 2 * PL], Uo[o + 3 * PL]};
+            periodic_images(Uo, L, c, j, e);
+        }
+    }
     block_partials(a.mass_part, by * a.nbx + bx, amax, umax, vmax, msum, bad);
 }
 
 template <int SCH, int RS, int LIM, typename T, bool CUT>
 __global__ __launch_bounds__(64) void k_step(StepArgs a) {
     step_strip<SCH, RS, LIM, T, CUT>(a, blockIdx.x, blockIdx.y, a.stage);
+}
+template <int SCH, int RS, int LIM, typename T>
+__global__ __launch_bounds__(64) void k_step_periodic(StepArgs a) {
+    step_strip<SCH, RS, LIM, T, false, true>(a, blockIdx.x, blockIdx.y, a.stage);
 }
 
 // Batched step: one launch over the strips of many solves. Block b runs strip
@@ -990,6 +1006,9 @@ static hipError_t go(int prec, const StepArgs* a, const BatchLaunch* b, hipStrea
     } else if (a->cut) {  // cut-cell variant (its extra registers stay out of the plain kernel)
         if (prec == 1) hipLaunchKernelGGL((k_step<SCH, RS, LIM, float, true>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
         else hipLaunchKernelGGL((k_step<SCH, RS, LIM, double, true>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
+    } else if (a->edge_fused == 2) {  // periodic ring written by the step kernel
+        if (prec == 1) hipLaunchKernelGGL((k_step_periodic<SCH, RS, LIM, float>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
+        else hipLaunchKernelGGL((k_step_periodic<SCH, RS, LIM, double>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
     } else {
         if (prec == 1) hipLaunchKernelGGL((k_step<SCH, RS, LIM, float, false>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
         else hipLaunchKernelGGL((k_step<SCH, RS, LIM, double, false>), dim3(a->nbx, a->nby), dim3(64), 0, s, *a);
