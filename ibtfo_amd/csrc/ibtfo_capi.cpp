@@ -794,9 +794,13 @@ static int df_eligible(const ibtfo_ctx* x, int nblocks) {
 }
 
 // An RK2 step needs no kernel between its stages when the step kernel writes the
-// periodic ring itself (Regular boundary: no ghost fill): stage 2 then reads U1
-// from the buffer stage 1 wrote, with no finish making it current.
-static bool rk2_in_place(const ibtfo_ctx* x) { return x->sa.edge_fused == 2; }
+// ring itself (the reference ring, whose corners on the Regular boundary come
+// from the fixed corner table, or the periodic ring) and there is no ghost fill
+// (Regular boundary): stage 2 then reads U1 from the buffer stage 1 wrote, with
+// no finish making it current.
+static bool rk2_in_place(const ibtfo_ctx* x) {
+    return x->cfg.boundary == IBTFO_REGULAR && (x->sa.edge_fused == 1 || x->sa.edge_fused == 2);
+}
 
 void ibtfo::build_args(ibtfo_ctx* x) {
     const ibtfo_config& c = x->cfg;

@@ -6,7 +6,10 @@ keeps the current buffer). GPU == oracle bit for bit, the whole state with its
 ring and the dt log, for both kernel forms (marching strips and row-parallel),
 single step and RK2, eager and graph-replayed, on grids where every column
 near the x edges has two images (N_x < 2 ng: ng = 3 with N_x = 4, 5; N_x = ng =
-4), and with ragged last strips (ng = 2)."""
+4), and with ragged last strips (ng = 2). The reference ring on the Regular
+boundary (also written by the step kernel) runs its RK2 stages back to back
+too: the same checks on an x-split Sod tube (MUSCL + minmod in reference mode,
+MHM + van Leer in corrected mode)."""
 import pytest
 
 from parity_util import gpu_config, gpu_hooks, nan_equal_bits
@@ -49,6 +52,23 @@ def test_periodic_ring_in_step_kernel(oracle_mod, nx, ny, ng, time):
     assert o["steps"] == n
     for rows, par, chunk in ((4, 0, 0), (3, 1, 0), (7, 0, 5), (2, 1, 5)):
         print("periodic ring:", nx, ny, ng, time, "rows", rows, "row-parallel", par, "graph chunk", chunk, flush=True)
+        g = _run(c, n, rows, par, chunk)
+        assert g["steps"] == n, (rows, par, chunk)
+        assert nan_equal_bits(g["U"], o["U_OLD"]) == 0, (rows, par, chunk)
+        assert nan_equal_bits(g["dt_log"], o["dt"]) == 0, (rows, par, chunk)
+
+
+@pytest.mark.parametrize("limiter,scheme", [("minmod", "MUSCL"), ("VanLeer", "MHM")])
+def test_rk2_reference_ring_in_place(oracle_mod, limiter, scheme):
+    O = oracle_mod
+    c = O.Case(N_x=130, N_y=27, hi_y=0.25, Scheme=scheme, Limiter=limiter, Time="RK2", case="sodx",
+               quirks=1 if scheme == "MUSCL" else 0)
+    n = 20
+    o = O.run_oracle(c, n, root_mode=1)
+    assert o["steps"] == n
+    for rows, par, chunk in ((4, 0, 0), (3, 1, 0), (7, 0, 6), (2, 1, 6)):
+        print("reference ring RK2:", scheme, limiter, "rows", rows, "row-parallel", par, "graph chunk", chunk,
+              flush=True)
         g = _run(c, n, rows, par, chunk)
         assert g["steps"] == n, (rows, par, chunk)
         assert nan_equal_bits(g["U"], o["U_OLD"]) == 0, (rows, par, chunk)
