@@ -785,12 +785,13 @@ extern "C" int ibtfo_initialize(ibtfo_ctx* x, int kind, const double* prm, int n
 // -------------------------------------------------------------- time loop
 // The deferred finish (one kernel per step) applies when nothing runs between
 // the step kernel and its finish: row-parallel strips of a single-step solve on
-// the Regular boundary with the fused reference edge hook, at most DF_MAX_REC
-// blocks, and a single solve (its partial records are its own).
+// the Regular boundary with a ring the step kernel writes (the reference edge
+// hook, or the periodic ring), at most DF_MAX_REC blocks, and a single solve
+// (its partial records are its own).
 static int df_eligible(const ibtfo_ctx* x, int nblocks) {
     const ibtfo_config& c = x->cfg;
     return x->df_part && x->rowpar && c.time == IBTFO_SINGLESTEP && c.boundary == IBTFO_REGULAR &&
-           c.edge == IBTFO_EDGE_REFERENCE && c.scheme != IBTFO_LAX && nblocks <= DF_MAX_REC;
+           (x->sa.edge_fused == 1 || x->sa.edge_fused == 2) && nblocks <= DF_MAX_REC;
 }
 
 // An RK2 step needs no kernel between its stages when the step kernel writes the

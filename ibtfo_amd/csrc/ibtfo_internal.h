@@ -97,8 +97,8 @@ struct StepArgs {
     const CutCell* cut;     // cut-cell records (type-3 cells), else null
     const int* cut_index;   // plane: record of each type-3 cell
     int rowpar;             // row-parallel form (k_step_rowpar: rows + 1 waves per block)
-    // Deferred finish (row-parallel form, single-step, Regular boundary with the
-    // fused reference edge hook, at most DF_MAX_REC blocks): no finish kernel
+    // Deferred finish (row-parallel form, single-step, Regular boundary with a
+    // ring the step kernel writes, at most DF_MAX_REC blocks): no finish kernel
     // between steps. Kernel k of a batch reduces kernel k-1's partial records
     // (df_part[(k-1) & 1], in the one-wave finish's order) and advances the
     // clock itself, every block alike; block (0,0) keeps the log. The batch

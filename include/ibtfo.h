@@ -237,7 +237,8 @@ int ibtfo_tile(const ibtfo_ctx* ctx, int* rows, int* nbx, int* nby);
 int ibtfo_tile_ex(const ibtfo_ctx* ctx, int* rows, int* nbx, int* nby, int* row_parallel);
 /* 1 when a step of this solve runs as one kernel with the current tile (the
  * deferred finish: row-parallel strips of a single-step solve on the Regular
- * boundary with the reference edge hook, at most 256 blocks; each ibtfo_step
+ * boundary with the reference edge hook or periodic edges, at most 256 blocks;
+ * each ibtfo_step
  * call or graph replay then ends with one closing kernel), 0 when a finish
  * kernel follows each step kernel, -1 for a null context. */
 int ibtfo_one_kernel_steps(const ibtfo_ctx* ctx);

@@ -1,6 +1,7 @@
 """The deferred finish of small grids. With row-parallel strips, a single-step
-solve on the Regular boundary with the reference edge hook and at most 256
-blocks runs each step as one kernel: every block of kernel k reduces kernel
+solve on the Regular boundary with the reference edge hook (or periodic edges,
+whose ring the step kernel writes too) and at most 256 blocks runs each step as
+one kernel: every block of kernel k reduces kernel
 k-1's partial records and advances the clock itself, and a flush kernel closes
 each batch of steps (an ibtfo_step call, a graph replay). The run must equal the
 two-kernel form (a finish kernel after each step) bit for bit: state, every
@@ -145,3 +146,25 @@ def test_more_blocks_than_the_deferred_form_takes(oracle_mod):
     b = _run(c, 1, 0, [(120, 100)])
     assert not a["one_kernel"] and not b["one_kernel"]
     _same(a, b, "700 blocks")
+
+
+def test_periodic_ring(oracle_mod):
+    """Periodic edges on a grid ragged in x and y (the ring written by the step
+    kernel, so nothing runs between a step kernel and its finish): one kernel
+    per step, equal to the finish-kernel form and to the oracle."""
+    O = oracle_mod
+    c = O.Case(N_x=130, N_y=27, lo_x=0.0, lo_y=0.0, hi_x=10.0, hi_y=10.0, Psy_time=1e30, max_iter=10 ** 9,
+               CFL_number=0.5, Scheme="MUSCL", Riemann="HLL", Limiter="VanLeer", case="shu", edge="periodic",
+               quirks=0, extra=dict(x0=5.0, y0=5.0, beta=5.0))
+    plan = [(40, 8), (5, 0)]
+    o = O.run_oracle(c, 45, root_mode=1)
+    assert o["steps"] == 45
+    for rows in (1, 4, 7):
+        print("deferred finish, periodic vortex 130x27, rows", rows, flush=True)
+        a = _run(c, rows, 1, plan)
+        b = _run(c, rows, 0, plan)
+        assert a["one_kernel"] and not b["one_kernel"]
+        assert a["it"] == 45
+        _same(a, b, ("periodic", rows))
+        assert nan_equal_bits(a["U"], o["U_OLD"]) == 0, rows
+        assert nan_equal_bits(a["dt_log"], o["dt"]) == 0, rows
